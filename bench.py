@@ -1,0 +1,219 @@
+"""Benchmark: batched 2D MLS-MPM forward on one MI355X.
+
+Workload (the 32 x 80K generation evaluation): 32 seeded 11-gene genomes (seeds 0..31,
+uniform within the evolve.py bounds) filled into 80,000-slot water tanks on 128x128 grids
+by the fill_tank restatement (about 62-65K live particles each, ~2.0M total), actuation
+timing genes applied as evolve.load_batch does. One "step" = one 100-substep block of the
+whole batch (HIP-graph replay). The rate counts live particles:
+value = sum(live) * substeps / seconds of the timed region (GPU-synchronised).
+
+Also reported on the same line:
+* roofline: algorithmic 108 B per live particle-substep (x, v, C, F read and written, 48 B
+  each way; material and fibre read, 12 B) over the fused kernel's average duration, timed
+  with HIP events on the engine's stream; peak 8.0 TB/s HBM3E;
+* cpu_baseline: the oracle (CPU restatement, fp32, one instance per OpenMP thread) on a
+  bounded sample of the same workload on this host;
+* plain_value: the same workload stepped by the plain path (one launch per stage per
+  instance, global atomics, unsorted).
+
+Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/s (32x80K batch, fp32)"
+ALGO_BYTES = 108.0          # per live particle-substep
+HBM_PEAK = 8.0e12           # B/s, MI355X HBM3E
+BLOCK = 100                 # substeps per step
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(M: int, N: int, seeds):
+    from jellyfih_amd import phenotype
+    pos, mat, fib, genomes, infos = phenotype.population(seeds, N, 128)
+    return pos, mat, fib, genomes, infos
+
+
+def load_engine(eng, pos, mat, fib, genomes):
+    for m in range(pos.shape[0]):
+        eng.load_particles(m, pos[m], mat[m], fib[m])
+    eng.set_param("contraction", np.clip(genomes[:, 9], 0.05, 0.60))   # evolve.py:150
+    eng.set_param("freq", np.clip(genomes[:, 10], 0.5, 2.0))           # evolve.py:152
+
+
+def load_tuning(eng) -> dict:
+    path = os.path.join(ROOT, "jellyfih_amd", "tuning.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            t = json.load(f)
+        eng.set_tuning(**{k: v for k, v in t.get("tuning", t).items()
+                          if k in eng.tuning()})
+    return eng.tuning()
+
+
+def traffic_from_profiles():
+    path = os.path.join(ROOT, "profiles", "fused_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch"), d.get("live_particles")
+
+
+def cpu_baseline(pos, mat, fib, genomes, budget_s: float = 12.0) -> dict:
+    """The oracle fp32 on host cores, one instance per thread, sized to ~budget_s."""
+    import oracle
+    cores = len(os.sched_getaffinity(0))
+    M = min(pos.shape[0], max(cores, 1))
+    N = pos.shape[1]
+    orc = oracle.OracleSim(M, N)
+    for m in range(M):
+        orc.load(m, pos[m], mat[m], fib[m])
+    orc.contraction[:] = np.clip(genomes[:M, 9], 0.05, 0.6).astype(np.float32)
+    orc.freq[:] = np.clip(genomes[:M, 10], 0.5, 2.0).astype(np.float32)
+    live = int((mat[:M] >= 0).sum())
+    t0 = time.perf_counter()
+    orc.run(2, nthreads=cores)
+    probe = time.perf_counter() - t0
+    steps = int(max(4, min(2000, budget_s / max(probe / 2, 1e-6))))
+    t0 = time.perf_counter()
+    orc.run(steps, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": live * steps / dt, "unit": "particle-substeps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{M} instances of the same workload ({live} live particles), "
+                      f"{steps} substeps, oracle fp32, one instance per OpenMP thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed 100-substep blocks")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed 100-substep blocks")
+    ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--slots", type=int, default=80000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); instances are "
+              "independent, so a multi-GPU run is one process per GPU on its own batch.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from jellyfih_amd.engine import Engine
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    M, N = args.instances, args.slots
+    log(f"building workload: {M} genomes x {N} slots")
+    pos, mat, fib, genomes, infos = build_workload(M, N, range(M))
+    live = int((mat >= 0).sum())
+    log(f"live particles {live} (slots {M * N})")
+
+    eng = Engine(M, N)
+    load_engine(eng, pos, mat, fib, genomes)
+    tuning = load_tuning(eng)
+    log(f"tuning {tuning}")
+    log(f"warmup {args.warmup} blocks")
+    eng.substep(args.warmup * BLOCK)
+    eng.sync()
+
+    stream = torch.cuda.ExternalStream(eng.stream)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    eng.sync()
+    log(f"timing {args.steps} blocks")
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        eng.substep(BLOCK)
+    ev1.record(stream)
+    eng.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_s = ev0.elapsed_time(ev1) / 1e3
+    substeps = args.steps * BLOCK
+    value = live * substeps / wall
+    log(f"wall {wall:.3f}s events {gpu_s:.3f}s -> {value:.3e} particle-substeps/s")
+
+    # fused-kernel duration with HIP events (3 blocks, not graph-captured)
+    fused = [eng.timed_block(BLOCK) for _ in range(3)]
+    fused_ms = float(np.mean([f[0] for f in fused]))
+    achieved = ALGO_BYTES * live / (fused_ms * 1e-3)
+    traffic, traffic_live = traffic_from_profiles()
+    if traffic is not None and traffic_live and traffic_live != live:
+        traffic = None
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "kernel": "k_fused<G2P,P2G>", "kernel_ms": fused_ms,
+                "algorithmic_bytes_per_launch": ALGO_BYTES * live}
+    log(f"fused kernel {fused_ms * 1e3:.1f} us -> {achieved / 1e9:.0f} GB/s algorithmic")
+
+    plain_value = None
+    if not args.no_plain:
+        log("plain path (one launch per stage per instance)")
+        pe = Engine(M, N, path="plain")
+        load_engine(pe, pos, mat, fib, genomes)
+        pe.substep(20)
+        pe.sync()
+        n_plain = 200
+        t0 = time.perf_counter()
+        pe.substep(n_plain)
+        pe.sync()
+        plain_value = live * n_plain / (time.perf_counter() - t0)
+        pe.close()
+        log(f"plain {plain_value:.3e}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle)")
+        cpu = cpu_baseline(pos, mat, fib, genomes)
+        log(f"cpu {cpu['value']:.3e} on {cpu['cores']} cores")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded genomes -> fill_tank restatement (reference 4-per-cell "
+                "lattice, no jitter)",
+        "config": {"workload": f"{M} instances x {N} slots, 128x128 grids, one fitness "
+                               f"episode's substeps in {BLOCK}-substep graph blocks",
+                   "instances": M, "slots_per_instance": N, "live_particles": live,
+                   "grid": [128, 128], "substeps_per_step": BLOCK, "tuning": tuning},
+        "slot_rate": M * N * substeps / wall,
+        "substeps_per_s": substeps / wall,
+        "gpu_event_s": gpu_s,
+        "plain_value": plain_value,
+        "speedup_vs_plain": (value / plain_value) if plain_value else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

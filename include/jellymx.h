@@ -1,0 +1,178 @@
+/* jellymx — MI355X-native batched 2D MLS-MPM engine: C ABI.
+ *
+ * This is the drop-in boundary for the simulation core of c-m-x-r/jellyfih (`mpm_sim.py`).
+ * The reference exposes that core as a Python module surface (Taichi fields and kernels);
+ * each entry point below names the reference item it replaces (file:line). A ctypes
+ * binding of these functions (jellyfih_amd/_lib.py) backs a Python module with the
+ * reference's names (jellyfih_amd/mpm_sim.py).
+ *
+ * Conventions
+ *  - All functions return a jm_status; on failure jm_last_error() returns a thread-local
+ *    message. Handles are not thread-safe; use one handle per thread.
+ *  - Host arrays use the reference's layout: per-particle fields are (n_instances,
+ *    n_particles, ...) in slot order, grids are (n_instances, n_grid, n_grid_y, ...).
+ *  - Device memory is owned by the library (hipMalloc at jm_create). All work is queued on
+ *    the handle's own HIP stream (jm_stream); functions that return host data synchronise it.
+ *  - Arithmetic is fp32, as in the reference.
+ */
+#ifndef JELLYMX_H
+#define JELLYMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JM_ABI_VERSION 1
+
+typedef enum jm_status {
+  JM_OK = 0,
+  JM_ERR_ARG = 1,    /* invalid argument (size, index, field id, NaN where forbidden) */
+  JM_ERR_HIP = 2,    /* a HIP runtime call failed (message names it) */
+  JM_ERR_STATE = 3,  /* call not valid in the handle's current state */
+  JM_ERR_NOMEM = 4   /* device allocation failed */
+} jm_status;
+
+/* Step paths.
+ * JM_PATH_FUSED: cell-sorted particles, one fused G2P->P2G kernel per substep with
+ *   LDS-tiled gather/scatter and triple-buffered grids, blocks captured as HIP graphs.
+ * JM_PATH_PLAIN: unsorted, one launch per stage (reset, P2G with global atomics, grid op,
+ *   G2P) per instance — the straightforward form kept as the comparison baseline. */
+typedef enum jm_path { JM_PATH_FUSED = 0, JM_PATH_PLAIN = 1 } jm_path;
+
+/* Fields readable with jm_read_field (and, where noted, writable with jm_write_field). */
+typedef enum jm_field {
+  JM_FIELD_X = 0,        /* f32 (M,N,2)   mpm_sim.py:86   rw */
+  JM_FIELD_V = 1,        /* f32 (M,N,2)   mpm_sim.py:87   rw */
+  JM_FIELD_C = 2,        /* f32 (M,N,2,2) mpm_sim.py:88   rw */
+  JM_FIELD_F = 3,        /* f32 (M,N,2,2) mpm_sim.py:89   rw */
+  JM_FIELD_MATERIAL = 4, /* i32 (M,N)     mpm_sim.py:90   r (set by jm_load_particles) */
+  JM_FIELD_JP = 5,       /* f32 (M,N)     mpm_sim.py:91   rw */
+  JM_FIELD_FIBER = 6,    /* f32 (M,N,2)   mpm_sim.py:112  rw */
+  JM_FIELD_GLOW = 7,     /* f32 (M,N)     mpm_sim.py:81   rw (needs track_glow) */
+  JM_FIELD_GRID_V = 8,   /* f32 (M,nx,ny,2) mpm_sim.py:92 r: grid velocity of the last substep */
+  JM_FIELD_GRID_M = 9,   /* f32 (M,nx,ny) mpm_sim.py:93  r: grid mass of the last substep */
+  JM_FIELD_FITNESS = 10  /* f32 (M,3)     mpm_sim.py:109  r: [sum_y, count, sum_x] */
+} jm_field;
+
+/* Per-instance parameters (arrays of n_instances floats). */
+typedef enum jm_param {
+  JM_PARAM_ACTUATION = 0,       /* instance_actuation, mpm_sim.py:116 (default 500) */
+  JM_PARAM_CONTRACTION = 1,     /* instance_act_contraction, mpm_sim.py:121 (default 0.2) */
+  JM_PARAM_FREQ = 2,            /* instance_freq, mpm_sim.py:125 (default 1.0) */
+  JM_PARAM_PAYLOAD_DENSITY = 3, /* instance_payload_density, mpm_sim.py:142 (default 2.5) */
+  JM_PARAM_E_JELLY = 4          /* jelly/muscle Young's modulus, mpm_sim.py:31 (default 700);
+                                   a module constant in the reference, per instance here */
+} jm_param;
+
+/* Stats modes for jm_compute_stats. */
+typedef enum jm_stats_mode {
+  JM_STATS_PAYLOAD = 0, /* compute_payload_stats, mpm_sim.py:746-753 (material 2) */
+  JM_STATS_BODY = 1,    /* compute_body_stats, mpm_sim.py:755-762 (materials 1 and 3) */
+  JM_STATS_ANY = 2      /* _get_stats_any, mpm_sim.py:779-792: payload, else body */
+} jm_stats_mode;
+
+/* Engine configuration; mirrors the reference's import-time settings (mpm_sim.py:11-22). */
+typedef struct jm_config {
+  int32_t abi_version;   /* must be JM_ABI_VERSION */
+  int32_t n_instances;   /* JELLY_INSTANCES, mpm_sim.py:11 */
+  int32_t n_particles;   /* JELLY_PARTICLES, mpm_sim.py:13: slots per instance */
+  int32_t quality;       /* mpm_sim.py:12: n_grid = 128*quality, dt = 2e-5/quality */
+  int32_t n_grid_y;      /* JELLY_GRID_Y, mpm_sim.py:15; 0 means n_grid */
+  int32_t track_glow;    /* 1: run the render-only glow pass (mpm_sim.py:335-338) */
+  int32_t path;          /* jm_path */
+  int32_t reserved;
+  double domain_height;  /* JELLY_DOMAIN_H, mpm_sim.py:16 */
+} jm_config;
+
+/* Launch tuning (chosen by jellyfih_amd/autotune.py; defaults are sensible). */
+typedef struct jm_tuning {
+  int32_t block_substeps;   /* substeps per HIP-graph block (default 100) */
+  int32_t sort_interval;    /* re-sort particles by cell every K substeps (default 100) */
+  int32_t threads;          /* fused kernel: threads per workgroup (128, 256 or 512) */
+  int32_t ppt;              /* fused kernel: particles per thread (2, 4, 8 or 16) */
+  int32_t use_graphs;       /* 1: replay captured HIP graphs per block */
+  int32_t xcd_affinity;     /* 1: keep each instance's workgroups on one XCD (needs M >= 8) */
+  int32_t lazy_vc;          /* 1: keep v, C in registers across fused substeps */
+  int32_t reserved;
+} jm_tuning;
+
+typedef struct jm_handle jm_handle;
+
+/* Fill *cfg with the reference defaults (16 instances x 80000 slots, 128x128, H=1). */
+int jm_default_config(jm_config* cfg);
+
+/* Allocate state for cfg->n_instances instances; replaces the field allocation at
+ * mpm_sim.py:81-149 (ti.init + ti.field). Parameters start at the reference defaults
+ * (mpm_sim.py:143-149); every slot starts dead until loaded. */
+int jm_create(const jm_config* cfg, jm_handle** out);
+int jm_destroy(jm_handle* h);
+
+int jm_set_tuning(jm_handle* h, const jm_tuning* t);
+int jm_get_tuning(jm_handle* h, jm_tuning* t);
+
+/* Load one instance: replaces load_particles / _load_particles_kernel (mpm_sim.py:715-736).
+ * pos (n,2) f32, mat (n,) i32 (material < 0 marks a dead slot), fiber (n,2) f32 or NULL
+ * for the reference default (0,1). n <= n_particles; slots n..N-1 become dead.
+ * Resets v=0, F=I, C=0, Jp=1 for the instance. */
+int jm_load_particles(jm_handle* h, int32_t instance, const float* pos, const int32_t* mat,
+                      const float* fiber, int64_t n);
+
+/* Item access of the per-instance fields (mpm_sim.py:116-149; writes as in
+ * evolve.py:150-152,172). values has n_instances floats. */
+int jm_set_instance_param(jm_handle* h, int32_t param, const float* values);
+int jm_get_instance_param(jm_handle* h, int32_t param, float* values);
+
+/* sim_time[None] (mpm_sim.py:95), fp32 accumulator advanced by dt per substep (:340). */
+int jm_set_time(jm_handle* h, float t);
+int jm_get_time(jm_handle* h, float* t);
+
+/* Advance all instances by n substeps: replaces n calls of substep() (mpm_sim.py:153-340). */
+int jm_substep(jm_handle* h, int64_t n);
+
+/* fitness_buffer after clear_fitness_buffer + compute_payload_stats / compute_body_stats
+ * (mpm_sim.py:740-762): out (M,3) f32 = [sum_y, count, sum_x] for jm_stats_mode. */
+int jm_compute_stats(jm_handle* h, int32_t mode, float* out);
+
+/* run_batch_headless (mpm_sim.py:802-834): t=0, CoM stats, `steps` substeps, stats;
+ * out (M,5) f64 = [init_y, init_x, final_y, final_x, valid]. */
+int jm_run_batch_headless(jm_handle* h, int64_t steps, double* out);
+
+/* Copy a field to/from host memory in the reference layout (Taichi field.to_numpy /
+ * item writes, e.g. helpers/fluid_analysis.py:26-27, helpers/fluid_test.py:145-151).
+ * nbytes must equal the field's size. */
+int jm_read_field(jm_handle* h, int32_t field, void* out, int64_t nbytes);
+int jm_write_field(jm_handle* h, int32_t field, const void* in, int64_t nbytes);
+
+/* Live (material >= 0) particle count of an instance. */
+int jm_live_count(jm_handle* h, int32_t instance, int64_t* n);
+
+/* Device order diagnostics: copy an instance's live particles in the engine's current
+ * (cell-sorted) order. slot (n,) i32 = reference slot index; x (n,2) f32. Either may be
+ * NULL. Used to check the sort permutation against a stable counting sort. */
+int jm_read_device_order(jm_handle* h, int32_t instance, int32_t* slot, float* x, int64_t n);
+
+/* Re-sort all particles by P2G base cell now (normally done every sort_interval). */
+int jm_sort_now(jm_handle* h);
+
+/* Measurement: run one block of n >= 2 fused substeps without graph capture, with HIP
+ * events on the handle's stream around its n-1 fused G2P->P2G kernels. Returns their
+ * average duration and the whole block's duration (ms). Advances the simulation like
+ * jm_substep(h, n). No reference counterpart (the reference has no timing hooks). */
+int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms);
+
+/* ti.sync(): wait for all queued work. */
+int jm_sync(jm_handle* h);
+
+/* The handle's HIP stream (hipStream_t), for event timing and interop. */
+void* jm_stream(jm_handle* h);
+
+/* Thread-local message of the last failure ("" if none). */
+const char* jm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JELLYMX_H */

@@ -1,0 +1,1086 @@
+// jellymx engine: host side of the C ABI declared in include/jellymx.h.
+//
+// Owns the device state of a batch of instances, the per-instance segment layout and the
+// chunk tables the fused kernel walks, launches the substep paths (capturing each block of
+// substeps as a HIP graph), and converts between the engine's cell-sorted compact layout
+// and the reference's (instance, slot) layout for field access.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/jellymx.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(JM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define JM_TRY(expr)            \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != JM_OK) return rc_; \
+  } while (0)
+
+constexpr int kNumXcd = 8;
+
+struct DevState {
+  jm::State s;
+  void* base = nullptr;
+};
+
+}  // namespace
+
+struct jm_handle {
+  jm_config cfg{};
+  jm_tuning tun{};
+  int M = 0, N = 0, nx = 0, ny = 0, G = 0, cap = 0;
+  jm::PhysConsts k{};
+  hipStream_t stream = nullptr;
+
+  // device buffers
+  DevState st, tmp;
+  int* keys = nullptr;
+  int* perm = nullptr;
+  int* hist = nullptr;  // hist, offs, cursor: 3 * M * G ints
+  int* offs = nullptr;
+  int* cursor = nullptr;
+  float* acc = nullptr;       // 3 grid buffers of M*3*G floats
+  float* t_buf = nullptr;     // 2 floats
+  float* params = nullptr;    // 6 * M floats: amp, contraction, freq, pdens, mu_j, la_j
+  int* n_live_d = nullptr;
+  int4* chunks_d = nullptr;
+  int* nchunks_d = nullptr;
+  float* stats_d = nullptr;   // M * 6
+  float* staging = nullptr;   // M * 2 * G floats (grid velocity readout)
+
+  // host mirrors
+  std::vector<float> h_param[5];  // amp, contraction, freq, pdens, E_jelly
+  std::vector<int> n_live;
+  std::vector<float> h_pos, h_fiber;  // loaded values (dead-slot contents), (M,N,2)
+  std::vector<int> h_mat;             // (M,N)
+  std::vector<char> inst_dirty;
+  bool layout_dirty = true;
+  bool params_dirty = true;
+
+  // chunk tables
+  int chunk_size = 0;
+  int total_chunks = 0;
+  int grid_blocks = 0;
+  bool affinity = false;
+  int xcd_off[kNumXcd] = {0}, xcd_cnt[kNumXcd] = {0};
+
+  // rotation / bookkeeping
+  int acc_base = 0;
+  int last_read = -1;        // grid buffer the last G2P gathered from (-1: none yet)
+  bool grid_is_velocity = false;  // plain path leaves velocity in the planes
+  int64_t since_sort = 0;
+
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+
+  ~jm_handle() { release(); }
+
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+
+  void release() {
+    drop_graphs();
+    if (stream) (void)hipStreamSynchronize(stream);
+    void* ptrs[] = {st.base, tmp.base, keys, perm, hist, acc, t_buf, params, n_live_d,
+                    chunks_d, nchunks_d, stats_d, staging};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    st.base = tmp.base = nullptr;
+    keys = perm = hist = nullptr;
+    acc = t_buf = params = stats_d = staging = nullptr;
+    n_live_d = nullptr;
+    chunks_d = nullptr;
+    nchunks_d = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+namespace {
+
+jm::PhysConsts make_consts(const jm_config& c, int nx, int ny) {
+  // mpm_sim.py:12-58, evaluated in double like the reference's Python, then rounded.
+  const double n_grid = 128.0 * c.quality;
+  const double dx = 1.0 / n_grid, inv_dx = n_grid;
+  const double dt = 2e-5 / c.quality;
+  const double p_vol = (dx * 0.5) * (dx * 0.5);
+  const double p_mass = p_vol * 1;
+  const double E_base = 0.1e4, nu_base = 0.2, E_pay = 2.0e5, nu_pay = 0.2;
+  jm::PhysConsts k{};
+  k.nx = nx;
+  k.ny = ny;
+  k.damp_x = nx / 20;
+  k.damp_y = ny / 20;
+  k.dt = (float)dt;
+  k.dx = (float)dx;
+  k.inv_dx = (float)inv_dx;
+  k.p_mass = (float)p_mass;
+  k.gravity_dt = (float)(dt * 10.0);
+  k.stress_scale = (float)(-dt * p_vol * 4 * inv_dx * inv_dx);
+  k.apic_scale = (float)(4 * inv_dx);
+  k.mu_base = (float)(E_base / (2 * (1 + nu_base)));
+  k.la_base = (float)(E_base * nu_base / ((1 + nu_base) * (1 - 2 * nu_base)));
+  k.mu_pay = (float)(E_pay / (2 * (1 + nu_pay)));
+  k.la_pay = (float)(E_pay * nu_pay / ((1 + nu_pay) * (1 - 2 * nu_pay)));
+  k.water_la = 100000.0f;
+  k.act_freq = 1.0f;
+  k.x_lo = 0.001f;
+  k.x_hi = 0.999f;
+  k.y_hi = (float)(c.domain_height - 0.001);
+  k.sig_floor = 1e-6f;
+  k.pay_lo = (float)(1 - 5e-3);
+  k.pay_hi = (float)(1 + 5e-3);
+  k.pi = 3.14159265f;
+  return k;
+}
+
+template <typename T>
+T* carve(char*& p, size_t n) {
+  size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
+  T* r = reinterpret_cast<T*>(p);
+  p += bytes;
+  return r;
+}
+
+size_t state_bytes(size_t n, bool glow) {
+  auto rnd = [](size_t b) { return (b + 255) & ~size_t(255); };
+  size_t b = rnd(n * 8) * 3 + rnd(n * 16) * 2 + rnd(n * 4) * 3;  // x v fiber; C F; Jp mat slot
+  if (glow) b += rnd(n * 4);
+  return b;
+}
+
+int alloc_state(DevState& d, size_t n, bool glow) {
+  void* base = nullptr;
+  size_t bytes = state_bytes(n, glow);
+  if (hipMalloc(&base, bytes) != hipSuccess) return fail(JM_ERR_NOMEM, "hipMalloc(state)");
+  d.base = base;
+  char* p = static_cast<char*>(base);
+  d.s.x = carve<float2>(p, n);
+  d.s.v = carve<float2>(p, n);
+  d.s.fiber = carve<float2>(p, n);
+  d.s.C = carve<float4>(p, n);
+  d.s.F = carve<float4>(p, n);
+  d.s.Jp = carve<float>(p, n);
+  d.s.mat = carve<int>(p, n);
+  d.s.slot = carve<int>(p, n);
+  d.s.glow = glow ? carve<float>(p, n) : nullptr;
+  return JM_OK;
+}
+
+inline unsigned blocks_for(int64_t n, int threads) {
+  return (unsigned)std::max<int64_t>(1, (n + threads - 1) / threads);
+}
+
+jm::InstParams inst_params(const jm_handle* h) {
+  const int M = h->M;
+  return {h->params, h->params + M, h->params + 2 * M, h->params + 3 * M, h->params + 4 * M,
+          h->params + 5 * M};
+}
+
+int upload_params(jm_handle* h) {
+  if (!h->params_dirty) return JM_OK;
+  const int M = h->M;
+  std::vector<float> p(6 * M);
+  for (int m = 0; m < M; ++m) {
+    p[m] = h->h_param[JM_PARAM_ACTUATION][m];
+    p[M + m] = h->h_param[JM_PARAM_CONTRACTION][m];
+    p[2 * M + m] = h->h_param[JM_PARAM_FREQ][m];
+    p[3 * M + m] = h->h_param[JM_PARAM_PAYLOAD_DENSITY][m];
+    // jelly/muscle Lame parameters from E (nu = 0.3), mpm_sim.py:31-34, in double
+    const double E = h->h_param[JM_PARAM_E_JELLY][m], nu = 0.3;
+    p[4 * M + m] = (float)(E / (2 * (1 + nu)));
+    p[5 * M + m] = (float)(E * nu / ((1 + nu) * (1 - 2 * nu)));
+  }
+  HIP_TRY(hipMemcpyAsync(h->params, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->params_dirty = false;
+  return JM_OK;
+}
+
+// Assign chunks of `chunk_size` sorted particles to workgroups. With XCD affinity every
+// chunk of one instance runs on the same XCD (workgroup i is dispatched to XCD i % 8), so
+// its grid's atomics and gathers stay in that XCD's L2; instances are spread over XCDs
+// longest-first to balance the chunk counts.
+int build_chunks(jm_handle* h) {
+  const int M = h->M;
+  const int cs = h->tun.threads * h->tun.ppt;
+  h->chunk_size = cs;
+  std::vector<int> nch(M);
+  for (int m = 0; m < M; ++m) nch[m] = (h->n_live[m] + cs - 1) / cs;
+  h->affinity = h->tun.xcd_affinity && M >= kNumXcd;
+  std::vector<std::vector<int4>> per_xcd(kNumXcd);
+  std::vector<int4> flat;
+  std::vector<int> order(M);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nch[a] > nch[b]; });
+  std::vector<int64_t> load(kNumXcd, 0);
+  for (int m : order) {
+    std::vector<int4> mine;
+    for (int c = 0; c < nch[m]; ++c) {
+      int start = m * h->cap + c * cs;
+      int count = std::min(cs, h->n_live[m] - c * cs);
+      mine.push_back(make_int4(m, start, count, c));
+    }
+    if (h->affinity) {
+      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += nch[m];
+      per_xcd[x].insert(per_xcd[x].end(), mine.begin(), mine.end());
+    } else {
+      flat.insert(flat.end(), mine.begin(), mine.end());
+    }
+  }
+  std::vector<int4> all;
+  if (h->affinity) {
+    int mx = 0;
+    for (int x = 0; x < kNumXcd; ++x) {
+      h->xcd_off[x] = (int)all.size();
+      h->xcd_cnt[x] = (int)per_xcd[x].size();
+      mx = std::max(mx, h->xcd_cnt[x]);
+      all.insert(all.end(), per_xcd[x].begin(), per_xcd[x].end());
+    }
+    h->grid_blocks = kNumXcd * mx;
+  } else {
+    // keep instance order for the non-affine mapping
+    all.clear();
+    for (int m = 0; m < M; ++m)
+      for (int c = 0; c < nch[m]; ++c) {
+        int start = m * h->cap + c * cs;
+        int count = std::min(cs, h->n_live[m] - c * cs);
+        all.push_back(make_int4(m, start, count, c));
+      }
+    for (int x = 0; x < kNumXcd; ++x) h->xcd_off[x] = h->xcd_cnt[x] = 0;
+    h->grid_blocks = (int)all.size();
+  }
+  h->total_chunks = (int)all.size();
+  if (h->chunks_d) (void)hipFree(h->chunks_d);
+  h->chunks_d = nullptr;
+  if (!all.empty()) {
+    HIP_TRY(hipMalloc(&h->chunks_d, all.size() * sizeof(int4)));
+    HIP_TRY(hipMemcpyAsync(h->chunks_d, all.data(), all.size() * sizeof(int4),
+                           hipMemcpyHostToDevice, h->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(h->nchunks_d, nch.data(), M * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+// Upload instances loaded since the last commit, rebuild chunk tables, clear the grids.
+int commit(jm_handle* h) {
+  JM_TRY(upload_params(h));
+  if (!h->layout_dirty) return JM_OK;
+  const int M = h->M, N = h->N;
+  std::vector<float2> px;
+  std::vector<float2> pf;
+  std::vector<int> pm, ps;
+  for (int m = 0; m < M; ++m) {
+    if (!h->inst_dirty[m]) continue;
+    px.clear();
+    pf.clear();
+    pm.clear();
+    ps.clear();
+    for (int p = 0; p < N; ++p) {
+      size_t q = (size_t)m * N + p;
+      if (h->h_mat[q] < 0) continue;
+      px.push_back(make_float2(h->h_pos[2 * q], h->h_pos[2 * q + 1]));
+      pf.push_back(make_float2(h->h_fiber[2 * q], h->h_fiber[2 * q + 1]));
+      pm.push_back(h->h_mat[q]);
+      ps.push_back(p);
+    }
+    const int n = (int)px.size();
+    h->n_live[m] = n;
+    const size_t off = (size_t)m * h->cap;
+    if (n > 0) {
+      HIP_TRY(hipMemcpyAsync(h->st.s.x + off, px.data(), n * sizeof(float2),
+                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->st.s.fiber + off, pf.data(), n * sizeof(float2),
+                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->st.s.mat + off, pm.data(), n * sizeof(int),
+                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->st.s.slot + off, ps.data(), n * sizeof(int),
+                             hipMemcpyHostToDevice, h->stream));
+      hipLaunchKernelGGL(jm::k_init_particles, dim3(blocks_for(n, 256)), dim3(256), 0,
+                         h->stream, h->st.s, (int64_t)off, (int64_t)(off + n));
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));  // host vectors are reused next iteration
+    h->inst_dirty[m] = 0;
+  }
+  HIP_TRY(hipMemcpyAsync(h->n_live_d, h->n_live.data(), M * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  JM_TRY(build_chunks(h));
+  HIP_TRY(hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * sizeof(float), h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->acc_base = 0;
+  h->last_read = -1;
+  h->grid_is_velocity = false;
+  h->since_sort = h->tun.sort_interval;  // sort before the next fused block
+  h->drop_graphs();
+  h->layout_dirty = false;
+  return JM_OK;
+}
+
+jm::SortArgs sort_args(jm_handle* h) {
+  jm::SortArgs a;
+  a.s = h->st.s;
+  a.tmp = h->tmp.s;
+  a.keys = h->keys;
+  a.perm = h->perm;
+  a.hist = h->hist;
+  a.offs = h->offs;
+  a.cursor = h->cursor;
+  a.n_live = h->n_live_d;
+  a.cap = h->cap;
+  a.M = h->M;
+  a.NC = h->G;
+  a.k = h->k;
+  return a;
+}
+
+int enqueue_sort(jm_handle* h) {
+  jm::SortArgs a = sort_args(h);
+  const int64_t nh = (int64_t)h->M * h->G;
+  hipLaunchKernelGGL(jm::k_fill_i32, dim3(blocks_for(nh, 256 * 8)), dim3(256), 0, h->stream,
+                     h->hist, nh, 0);
+  const dim3 pg(blocks_for(h->cap, 256), h->M);
+  hipLaunchKernelGGL(jm::k_sort_hist, pg, dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(jm::k_sort_scan, dim3(h->M), dim3(1024), 0, h->stream, a);
+  hipLaunchKernelGGL(jm::k_sort_scatter, pg, dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(jm::k_sort_fix, dim3(blocks_for(h->G, 256), h->M), dim3(256), 0,
+                     h->stream, a);
+  hipLaunchKernelGGL(jm::k_sort_gather, pg, dim3(256), 0, h->stream, a);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+jm::FusedArgs fused_args(jm_handle* h) {
+  jm::FusedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.s = h->st.s;
+  a.ip = inst_params(h);
+  a.t_buf = h->t_buf;
+  a.chunks = h->chunks_d;
+  a.nchunks = h->nchunks_d;
+  for (int x = 0; x < kNumXcd; ++x) {
+    a.xcd_off[x] = h->xcd_off[x];
+    a.xcd_cnt[x] = h->xcd_cnt[x];
+  }
+  a.affinity = h->affinity ? 1 : 0;
+  a.total_chunks = h->total_chunks;
+  a.G = h->G;
+  a.lazy_vc = h->tun.lazy_vc;
+  a.glow = h->cfg.track_glow;
+  a.k = h->k;
+  return a;
+}
+
+template <int B, int P>
+void launch_fused_t(jm_handle* h, bool g2p, bool p2g, const jm::FusedArgs& a) {
+  dim3 grid(std::max(1, h->grid_blocks)), block(B);
+  if (g2p && p2g)
+    hipLaunchKernelGGL((jm::k_fused<B, P, true, true>), grid, block, 0, h->stream, a);
+  else if (g2p)
+    hipLaunchKernelGGL((jm::k_fused<B, P, true, false>), grid, block, 0, h->stream, a);
+  else
+    hipLaunchKernelGGL((jm::k_fused<B, P, false, true>), grid, block, 0, h->stream, a);
+}
+
+bool tuning_supported(int threads, int ppt) {
+  static const int combos[][2] = {{128, 8}, {128, 16}, {256, 2}, {256, 4}, {256, 8},
+                                  {256, 16}, {512, 2}, {512, 4}, {512, 8}};
+  for (auto& c : combos)
+    if (c[0] == threads && c[1] == ppt) return true;
+  return false;
+}
+
+void launch_fused(jm_handle* h, bool g2p, bool p2g, const jm::FusedArgs& a) {
+  const int T = h->tun.threads, P = h->tun.ppt;
+#define JM_CASE(TT, PP) \
+  if (T == TT && P == PP) return launch_fused_t<TT, PP>(h, g2p, p2g, a);
+  JM_CASE(128, 8)
+  JM_CASE(128, 16)
+  JM_CASE(256, 2)
+  JM_CASE(256, 4)
+  JM_CASE(256, 8)
+  JM_CASE(256, 16)
+  JM_CASE(512, 2)
+  JM_CASE(512, 4)
+  JM_CASE(512, 8)
+#undef JM_CASE
+}
+
+float* grid_buf(jm_handle* h, int b) { return h->acc + (size_t)b * h->M * 3 * h->G; }
+
+// Kernel kk (0..n) of a block of n fused substeps with grid-buffer base b: it gathers
+// from buffer (b+kk-1)%3, scatters into (b+kk)%3 and zeroes (b+kk+1)%3 — the next kernel's
+// scatter target, last read two kernels ago. Kernel 0 only scatters (P2G of substep 0),
+// kernel n only gathers (G2P of substep n-1). The clock ping-pongs between t_buf[0] and
+// t_buf[1] and is back in t_buf[0] after the block.
+void enqueue_fused_kernel(jm_handle* h, jm::FusedArgs& a, int kk, int n, int b) {
+  const bool g2p = kk > 0, p2g = kk < n;
+  a.acc_r = g2p ? grid_buf(h, (b + kk - 1) % 3) : nullptr;
+  a.acc_w = p2g ? grid_buf(h, (b + kk) % 3) : nullptr;
+  a.acc_z = grid_buf(h, (b + kk + 1) % 3);
+  if (p2g) {  // substep kk reads t_kk, writes t_{kk+1}
+    a.t_rd = kk % 2;
+    a.t_wr = (kk + 1) % 2;
+    a.t_adv = 1;
+  } else if (n % 2 == 1) {
+    a.t_rd = 1;
+    a.t_wr = 0;
+    a.t_adv = 0;
+  } else {
+    a.t_rd = 0;
+    a.t_wr = -1;
+    a.t_adv = 0;
+  }
+  launch_fused(h, g2p, p2g, a);
+}
+
+int enqueue_fused_block(jm_handle* h, int n, int b, bool sort) {
+  if (sort) JM_TRY(enqueue_sort(h));
+  if (h->total_chunks == 0) return JM_OK;
+  jm::FusedArgs a = fused_args(h);
+  for (int kk = 0; kk <= n; ++kk) enqueue_fused_kernel(h, a, kk, n, b);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+void finish_block(jm_handle* h, int n, int b, bool sort) {
+  if (sort) h->since_sort = 0;
+  h->since_sort += n;
+  h->last_read = (b + n - 1) % 3;
+  h->acc_base = (b + n + 1) % 3;
+  h->grid_is_velocity = false;
+}
+
+int run_fused(jm_handle* h, int64_t steps) {
+  int64_t left = steps;
+  while (left > 0) {
+    const int n = (int)std::min<int64_t>(left, h->tun.block_substeps);
+    const bool sort = h->since_sort >= h->tun.sort_interval;
+    const int b = h->acc_base;
+    if (h->tun.use_graphs) {
+      auto key = std::make_tuple(n, b, sort ? 1 : 0);
+      auto it = h->graphs.find(key);
+      if (it == h->graphs.end()) {
+        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_fused_block(h, n, b, sort);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (rc != JM_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess)
+          return fail(JM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess)
+          return fail(JM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        it = h->graphs.emplace(key, ex).first;
+      }
+      HIP_TRY(hipGraphLaunch(it->second, h->stream));
+    } else {
+      JM_TRY(enqueue_fused_block(h, n, b, sort));
+    }
+    finish_block(h, n, b, sort);
+    left -= n;
+  }
+  return JM_OK;
+}
+
+jm::PlainArgs plain_args(jm_handle* h, int m0) {
+  jm::PlainArgs a;
+  a.s = h->st.s;
+  a.ip = inst_params(h);
+  a.acc = grid_buf(h, 0);
+  a.t_buf = h->t_buf;
+  a.n_live = h->n_live_d;
+  a.cap = h->cap;
+  a.m0 = m0;
+  a.G = h->G;
+  a.glow = h->cfg.track_glow;
+  a.k = h->k;
+  return a;
+}
+
+// Plain path: per substep, for each instance separately: reset its grid, P2G, grid op, G2P;
+// then the time advance.
+int run_plain(jm_handle* h, int64_t steps) {
+  const int M = h->M, G = h->G;
+  for (int64_t s = 0; s < steps; ++s) {
+    for (int m = 0; m < M; ++m) {
+      if (h->n_live[m] == 0) continue;
+      jm::PlainArgs a = plain_args(h, m);
+      hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for(3 * G, 256 * 4)), dim3(256), 0,
+                         h->stream, grid_buf(h, 0) + (size_t)m * 3 * G, (int64_t)3 * G, 0.f);
+      const dim3 pg(blocks_for(h->n_live[m], 256), 1);
+      hipLaunchKernelGGL(jm::k_plain_p2g, pg, dim3(256), 0, h->stream, a);
+      hipLaunchKernelGGL(jm::k_plain_gridop, dim3(blocks_for(G, 256), 1), dim3(256), 0,
+                         h->stream, a);
+      hipLaunchKernelGGL(jm::k_plain_g2p, pg, dim3(256), 0, h->stream, a);
+    }
+    hipLaunchKernelGGL(jm::k_time_advance, dim3(1), dim3(64), 0, h->stream, h->t_buf, h->k.dt);
+  }
+  HIP_TRY(hipGetLastError());
+  h->last_read = 0;
+  h->grid_is_velocity = true;
+  return JM_OK;
+}
+
+bool valid_field(int f) { return f >= JM_FIELD_X && f <= JM_FIELD_FITNESS; }
+
+int64_t field_bytes(const jm_handle* h, int f) {
+  const int64_t MN = (int64_t)h->M * h->N, MG = (int64_t)h->M * h->G;
+  switch (f) {
+    case JM_FIELD_X:
+    case JM_FIELD_V:
+    case JM_FIELD_FIBER:
+      return MN * 8;
+    case JM_FIELD_C:
+    case JM_FIELD_F:
+      return MN * 16;
+    case JM_FIELD_MATERIAL:
+    case JM_FIELD_JP:
+    case JM_FIELD_GLOW:
+      return MN * 4;
+    case JM_FIELD_GRID_V:
+      return MG * 8;
+    case JM_FIELD_GRID_M:
+      return MG * 4;
+    case JM_FIELD_FITNESS:
+      return (int64_t)h->M * 12;
+  }
+  return -1;
+}
+
+// Per-particle field: device pointer and floats (or ints) per particle.
+bool particle_field(jm_handle* h, int f, void** dptr, int* comps) {
+  jm::State& s = h->st.s;
+  switch (f) {
+    case JM_FIELD_X: *dptr = s.x; *comps = 2; return true;
+    case JM_FIELD_V: *dptr = s.v; *comps = 2; return true;
+    case JM_FIELD_C: *dptr = s.C; *comps = 4; return true;
+    case JM_FIELD_F: *dptr = s.F; *comps = 4; return true;
+    case JM_FIELD_MATERIAL: *dptr = s.mat; *comps = 1; return true;
+    case JM_FIELD_JP: *dptr = s.Jp; *comps = 1; return true;
+    case JM_FIELD_FIBER: *dptr = s.fiber; *comps = 2; return true;
+    case JM_FIELD_GLOW: *dptr = s.glow; *comps = 1; return s.glow != nullptr;
+  }
+  return false;
+}
+
+int read_slots(jm_handle* h, std::vector<int>& slots) {
+  slots.assign((size_t)h->M * h->cap, -1);
+  for (int m = 0; m < h->M; ++m) {
+    const int n = h->n_live[m];
+    if (n == 0) continue;
+    HIP_TRY(hipMemcpyAsync(slots.data() + (size_t)m * h->cap, h->st.s.slot + (size_t)m * h->cap,
+                           n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int fill_stats(jm_handle* h) {
+  hipLaunchKernelGGL(jm::k_stats, dim3(h->M), dim3(256), 0, h->stream, h->st.s, h->n_live_d,
+                     h->cap, h->stats_d);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+
+int check_handle(jm_handle* h) {
+  if (!h) return fail(JM_ERR_ARG, "null handle");
+  return JM_OK;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+
+extern "C" {
+
+const char* jm_last_error(void) { return g_err.c_str(); }
+
+int jm_default_config(jm_config* cfg) {
+  if (!cfg) return fail(JM_ERR_ARG, "null config");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->abi_version = JM_ABI_VERSION;
+  cfg->n_instances = 16;    // mpm_sim.py:11
+  cfg->n_particles = 80000; // mpm_sim.py:13
+  cfg->quality = 1;         // mpm_sim.py:12
+  cfg->n_grid_y = 0;
+  cfg->track_glow = 0;
+  cfg->path = JM_PATH_FUSED;
+  cfg->domain_height = 1.0; // mpm_sim.py:16
+  return JM_OK;
+}
+
+int jm_create(const jm_config* cfg, jm_handle** out) {
+  if (!cfg || !out) return fail(JM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->abi_version != JM_ABI_VERSION) return fail(JM_ERR_ARG, "abi_version mismatch");
+  if (cfg->n_instances <= 0 || cfg->n_particles <= 0 || cfg->quality <= 0)
+    return fail(JM_ERR_ARG, "n_instances, n_particles and quality must be positive");
+  if (cfg->path != JM_PATH_FUSED && cfg->path != JM_PATH_PLAIN)
+    return fail(JM_ERR_ARG, "unknown path");
+  if (!(cfg->domain_height > 0.002)) return fail(JM_ERR_ARG, "domain_height too small");
+  const int nx = 128 * cfg->quality;
+  const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
+  if (ny < 8 || ny % 4 != 0) return fail(JM_ERR_ARG, "n_grid_y must be a multiple of 4, >= 8");
+  if ((int64_t)cfg->n_instances * cfg->n_particles > (int64_t)INT32_MAX / 2)
+    return fail(JM_ERR_ARG, "n_instances * n_particles too large");
+
+  jm_handle* h = new jm_handle();
+  h->cfg = *cfg;
+  h->M = cfg->n_instances;
+  h->N = cfg->n_particles;
+  h->nx = nx;
+  h->ny = ny;
+  h->G = nx * ny;
+  h->cap = cfg->n_particles;
+  h->k = make_consts(*cfg, nx, ny);
+  h->tun = {100, 100, 256, 8, 1, 1, 1, 0};
+  const int M = h->M;
+  h->h_param[JM_PARAM_ACTUATION].assign(M, 500.0f);       // mpm_sim.py:146
+  h->h_param[JM_PARAM_CONTRACTION].assign(M, 0.20f);      // :148
+  h->h_param[JM_PARAM_FREQ].assign(M, 1.0f);              // :149
+  h->h_param[JM_PARAM_PAYLOAD_DENSITY].assign(M, 2.5f);   // :147
+  h->h_param[JM_PARAM_E_JELLY].assign(M, 0.7e3f);         // :31
+  h->n_live.assign(M, 0);
+  h->h_pos.assign((size_t)M * h->N * 2, -1.0f);
+  h->h_fiber.assign((size_t)M * h->N * 2, 0.0f);
+  for (size_t q = 0; q < (size_t)M * h->N; ++q) h->h_fiber[2 * q + 1] = 1.0f;
+  h->h_mat.assign((size_t)M * h->N, -1);
+  h->inst_dirty.assign(M, 0);
+
+  auto cleanup = [&](int rc) {
+    delete h;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(JM_ERR_HIP, "hipStreamCreateWithFlags"));
+  const size_t n = (size_t)M * h->cap;
+  const bool glow = cfg->track_glow != 0;
+  int rc = alloc_state(h->st, n, glow);
+  if (rc == JM_OK) rc = alloc_state(h->tmp, n, glow);
+  if (rc != JM_OK) return cleanup(rc);
+  auto dm = [&](void** p, size_t bytes) {
+    return hipMalloc(p, std::max<size_t>(bytes, 256)) == hipSuccess;
+  };
+  bool ok = dm((void**)&h->keys, n * 4) && dm((void**)&h->perm, n * 4) &&
+            dm((void**)&h->hist, (size_t)3 * M * h->G * 4) &&
+            dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8) &&
+            dm((void**)&h->params, (size_t)6 * M * 4) && dm((void**)&h->n_live_d, M * 4) &&
+            dm((void**)&h->nchunks_d, M * 4) && dm((void**)&h->stats_d, (size_t)M * 6 * 4) &&
+            dm((void**)&h->staging, (size_t)M * 2 * h->G * 4);
+  if (!ok) return cleanup(fail(JM_ERR_NOMEM, "hipMalloc(engine buffers)"));
+  h->offs = h->hist + (size_t)M * h->G;
+  h->cursor = h->offs + (size_t)M * h->G;
+  if (hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * 4, h->stream) != hipSuccess ||
+      hipMemsetAsync(h->t_buf, 0, 8, h->stream) != hipSuccess ||
+      hipMemsetAsync(h->stats_d, 0, (size_t)M * 6 * 4, h->stream) != hipSuccess ||
+      hipMemsetAsync(h->n_live_d, 0, M * 4, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return cleanup(fail(JM_ERR_HIP, "initial clears"));
+  h->layout_dirty = true;
+  *out = h;
+  g_err.clear();
+  return JM_OK;
+}
+
+int jm_destroy(jm_handle* h) {
+  if (!h) return JM_OK;
+  delete h;
+  return JM_OK;
+}
+
+int jm_set_tuning(jm_handle* h, const jm_tuning* t) {
+  JM_TRY(check_handle(h));
+  if (!t) return fail(JM_ERR_ARG, "null tuning");
+  if (t->block_substeps < 1 || t->block_substeps > 100000)
+    return fail(JM_ERR_ARG, "block_substeps out of range");
+  if (t->sort_interval < 1) return fail(JM_ERR_ARG, "sort_interval must be >= 1");
+  if (!tuning_supported(t->threads, t->ppt))
+    return fail(JM_ERR_ARG, "unsupported (threads, ppt) combination");
+  JM_TRY(jm_sync(h));
+  const bool relayout = t->threads * t->ppt != h->tun.threads * h->tun.ppt ||
+                        t->xcd_affinity != h->tun.xcd_affinity;
+  h->tun = *t;
+  h->drop_graphs();
+  if (relayout && !h->layout_dirty) JM_TRY(build_chunks(h));
+  return JM_OK;
+}
+
+int jm_get_tuning(jm_handle* h, jm_tuning* t) {
+  JM_TRY(check_handle(h));
+  if (!t) return fail(JM_ERR_ARG, "null tuning");
+  *t = h->tun;
+  return JM_OK;
+}
+
+int jm_load_particles(jm_handle* h, int32_t instance, const float* pos, const int32_t* mat,
+                      const float* fiber, int64_t n) {
+  JM_TRY(check_handle(h));
+  if (instance < 0 || instance >= h->M) return fail(JM_ERR_ARG, "instance out of range");
+  if (!pos || !mat) return fail(JM_ERR_ARG, "pos and mat are required");
+  if (n < 0 || n > h->N) return fail(JM_ERR_ARG, "n must be in [0, n_particles]");
+  const size_t base = (size_t)instance * h->N;
+  for (int64_t p = 0; p < n; ++p) {
+    const int32_t mt = mat[p];
+    const float x0 = pos[2 * p], x1 = pos[2 * p + 1];
+    if (mt >= 0 && !(std::isfinite(x0) && std::isfinite(x1)))
+      return fail(JM_ERR_ARG, "non-finite position of a live particle");
+  }
+  for (int64_t p = 0; p < h->N; ++p) {
+    const size_t q = base + p;
+    if (p < n) {
+      h->h_pos[2 * q] = pos[2 * p];
+      h->h_pos[2 * q + 1] = pos[2 * p + 1];
+      h->h_mat[q] = mat[p];
+      h->h_fiber[2 * q] = fiber ? fiber[2 * p] : 0.0f;  // default (0,1), mpm_sim.py:733-735
+      h->h_fiber[2 * q + 1] = fiber ? fiber[2 * p + 1] : 1.0f;
+    } else {
+      h->h_pos[2 * q] = -1.0f;
+      h->h_pos[2 * q + 1] = -1.0f;
+      h->h_mat[q] = -1;
+      h->h_fiber[2 * q] = 0.0f;
+      h->h_fiber[2 * q + 1] = 1.0f;
+    }
+  }
+  h->inst_dirty[instance] = 1;
+  h->layout_dirty = true;
+  return JM_OK;
+}
+
+int jm_set_instance_param(jm_handle* h, int32_t param, const float* values) {
+  JM_TRY(check_handle(h));
+  if (param < 0 || param > JM_PARAM_E_JELLY) return fail(JM_ERR_ARG, "unknown param");
+  if (!values) return fail(JM_ERR_ARG, "null values");
+  for (int m = 0; m < h->M; ++m) h->h_param[param][m] = values[m];
+  h->params_dirty = true;
+  return upload_params(h);
+}
+
+int jm_get_instance_param(jm_handle* h, int32_t param, float* values) {
+  JM_TRY(check_handle(h));
+  if (param < 0 || param > JM_PARAM_E_JELLY) return fail(JM_ERR_ARG, "unknown param");
+  if (!values) return fail(JM_ERR_ARG, "null values");
+  for (int m = 0; m < h->M; ++m) values[m] = h->h_param[param][m];
+  return JM_OK;
+}
+
+int jm_set_time(jm_handle* h, float t) {
+  JM_TRY(check_handle(h));
+  HIP_TRY(hipMemcpyAsync(h->t_buf, &t, sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int jm_get_time(jm_handle* h, float* t) {
+  JM_TRY(check_handle(h));
+  if (!t) return fail(JM_ERR_ARG, "null t");
+  HIP_TRY(hipMemcpyAsync(t, h->t_buf, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int jm_substep(jm_handle* h, int64_t n) {
+  JM_TRY(check_handle(h));
+  if (n < 0) return fail(JM_ERR_ARG, "negative step count");
+  if (n == 0) return JM_OK;
+  JM_TRY(commit(h));
+  if (h->cfg.path == JM_PATH_PLAIN) return run_plain(h, n);
+  return run_fused(h, n);
+}
+
+int jm_sort_now(jm_handle* h) {
+  JM_TRY(check_handle(h));
+  JM_TRY(commit(h));
+  JM_TRY(enqueue_sort(h));
+  h->since_sort = 0;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int jm_compute_stats(jm_handle* h, int32_t mode, float* out) {
+  JM_TRY(check_handle(h));
+  if (!out) return fail(JM_ERR_ARG, "null out");
+  if (mode < JM_STATS_PAYLOAD || mode > JM_STATS_ANY) return fail(JM_ERR_ARG, "unknown mode");
+  JM_TRY(commit(h));
+  JM_TRY(fill_stats(h));
+  std::vector<float> raw((size_t)h->M * 6);
+  HIP_TRY(hipMemcpyAsync(raw.data(), h->stats_d, raw.size() * 4, hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int m = 0; m < h->M; ++m) {
+    const float* pay = &raw[(size_t)m * 6];
+    const float* body = pay + 3;
+    const float* src = pay;
+    if (mode == JM_STATS_BODY || (mode == JM_STATS_ANY && pay[1] == 0.0f)) src = body;
+    for (int c = 0; c < 3; ++c) out[3 * m + c] = src[c];
+  }
+  return JM_OK;
+}
+
+int jm_run_batch_headless(jm_handle* h, int64_t steps, double* out) {
+  JM_TRY(check_handle(h));
+  if (!out || steps < 0) return fail(JM_ERR_ARG, "bad arguments");
+  const int M = h->M;
+  JM_TRY(jm_set_time(h, 0.0f));  // mpm_sim.py:808
+  std::vector<float> r0((size_t)M * 3), r1((size_t)M * 3);
+  JM_TRY(jm_compute_stats(h, JM_STATS_ANY, r0.data()));  // :811
+  JM_TRY(jm_substep(h, steps));                            // :814-815
+  JM_TRY(jm_compute_stats(h, JM_STATS_ANY, r1.data()));  // :818
+  // mpm_sim.py:793-800 (host division in double) and :821-834
+  for (int m = 0; m < M; ++m) {
+    double* o = out + 5 * m;
+    for (int c = 0; c < 5; ++c) o[c] = 0.0;
+    const double c0 = r0[3 * m + 1], c1 = r1[3 * m + 1];
+    if (c0 > 0 && c1 > 0) {
+      o[0] = (double)r0[3 * m] / c0;
+      o[1] = (double)r0[3 * m + 2] / c0;
+      o[2] = (double)r1[3 * m] / c1;
+      o[3] = (double)r1[3 * m + 2] / c1;
+      o[4] = o[2] < 0.01 ? 0.0 : 1.0;
+    }
+  }
+  return JM_OK;
+}
+
+int jm_read_field(jm_handle* h, int32_t field, void* out, int64_t nbytes) {
+  JM_TRY(check_handle(h));
+  if (!valid_field(field)) return fail(JM_ERR_ARG, "unknown field");
+  if (!out) return fail(JM_ERR_ARG, "null out");
+  if (nbytes != field_bytes(h, field)) return fail(JM_ERR_ARG, "nbytes does not match field");
+  JM_TRY(commit(h));
+  const int M = h->M, N = h->N, G = h->G, nx = h->nx, ny = h->ny;
+  if (field == JM_FIELD_GRID_V || field == JM_FIELD_GRID_M) {
+    std::vector<float> planes((size_t)M * 3 * G, 0.0f);
+    if (h->last_read >= 0) {
+      if (field == JM_FIELD_GRID_V && !h->grid_is_velocity) {
+        hipLaunchKernelGGL(jm::k_grid_velocity, dim3(blocks_for(G, 256), M), dim3(256), 0,
+                           h->stream, grid_buf(h, h->last_read), h->staging, M, G, h->k);
+        HIP_TRY(hipGetLastError());
+        std::vector<float> vel((size_t)M * 2 * G);
+        HIP_TRY(hipMemcpyAsync(vel.data(), h->staging, vel.size() * 4, hipMemcpyDeviceToHost,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int m = 0; m < M; ++m) {
+          std::memcpy(&planes[(size_t)m * 3 * G], &vel[(size_t)m * 2 * G], (size_t)2 * G * 4);
+        }
+      } else {
+        HIP_TRY(hipMemcpyAsync(planes.data(), grid_buf(h, h->last_read), planes.size() * 4,
+                               hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+      }
+    }
+    float* o = static_cast<float*>(out);
+    for (int m = 0; m < M; ++m) {
+      const float* pl = &planes[(size_t)m * 3 * G];
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+          const size_t node = (size_t)j * nx + i;
+          const size_t q = ((size_t)m * nx + i) * ny + j;
+          if (field == JM_FIELD_GRID_V) {
+            o[2 * q] = pl[node];
+            o[2 * q + 1] = pl[G + node];
+          } else {
+            o[q] = pl[2 * G + node];
+          }
+        }
+    }
+    return JM_OK;
+  }
+  if (field == JM_FIELD_FITNESS) {
+    // the buffer as compute_payload_stats leaves it (mpm_sim.py:109)
+    std::vector<float> raw((size_t)M * 6);
+    HIP_TRY(hipMemcpyAsync(raw.data(), h->stats_d, raw.size() * 4, hipMemcpyDeviceToHost,
+                           h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    float* o = static_cast<float*>(out);
+    for (int m = 0; m < M; ++m)
+      for (int c = 0; c < 3; ++c) o[3 * m + c] = raw[(size_t)m * 6 + c];
+    return JM_OK;
+  }
+  void* dptr = nullptr;
+  int comps = 0;
+  if (!particle_field(h, field, &dptr, &comps))
+    return fail(JM_ERR_STATE, "field not tracked (glow needs track_glow)");
+  // defaults for dead slots: the loaded values (x, material, fiber) or the reset state
+  if (field == JM_FIELD_X) {
+    std::memcpy(out, h->h_pos.data(), (size_t)nbytes);
+  } else if (field == JM_FIELD_FIBER) {
+    std::memcpy(out, h->h_fiber.data(), (size_t)nbytes);
+  } else if (field == JM_FIELD_MATERIAL) {
+    std::memcpy(out, h->h_mat.data(), (size_t)nbytes);
+  } else {
+    float* o = static_cast<float*>(out);
+    const size_t MN = (size_t)M * N;
+    for (size_t q = 0; q < MN; ++q) {
+      if (field == JM_FIELD_F) {
+        o[4 * q] = 1.f;
+        o[4 * q + 1] = 0.f;
+        o[4 * q + 2] = 0.f;
+        o[4 * q + 3] = 1.f;
+      } else if (field == JM_FIELD_JP) {
+        o[q] = 1.f;
+      } else {
+        for (int c = 0; c < comps; ++c) o[comps * q + c] = 0.f;
+      }
+    }
+  }
+  std::vector<int> slots;
+  JM_TRY(read_slots(h, slots));
+  const size_t esz = (size_t)comps * 4;
+  std::vector<char> buf;
+  char* o = static_cast<char*>(out);
+  for (int m = 0; m < M; ++m) {
+    const int n = h->n_live[m];
+    if (n == 0) continue;
+    buf.resize((size_t)n * esz);
+    HIP_TRY(hipMemcpyAsync(buf.data(), static_cast<char*>(dptr) + (size_t)m * h->cap * esz,
+                           buf.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int* sl = slots.data() + (size_t)m * h->cap;
+    for (int i = 0; i < n; ++i)
+      std::memcpy(o + ((size_t)m * N + sl[i]) * esz, buf.data() + (size_t)i * esz, esz);
+  }
+  return JM_OK;
+}
+
+int jm_write_field(jm_handle* h, int32_t field, const void* in, int64_t nbytes) {
+  JM_TRY(check_handle(h));
+  if (!valid_field(field)) return fail(JM_ERR_ARG, "unknown field");
+  if (!in) return fail(JM_ERR_ARG, "null input");
+  if (field == JM_FIELD_MATERIAL || field >= JM_FIELD_GRID_V)
+    return fail(JM_ERR_ARG, "field is read-only (materials are set by jm_load_particles)");
+  if (nbytes != field_bytes(h, field)) return fail(JM_ERR_ARG, "nbytes does not match field");
+  JM_TRY(commit(h));
+  void* dptr = nullptr;
+  int comps = 0;
+  if (!particle_field(h, field, &dptr, &comps))
+    return fail(JM_ERR_STATE, "field not tracked (glow needs track_glow)");
+  const int M = h->M, N = h->N;
+  const size_t esz = (size_t)comps * 4;
+  const char* src = static_cast<const char*>(in);
+  if (field == JM_FIELD_X || field == JM_FIELD_FIBER) {  // keep dead-slot contents too
+    std::vector<float>& hv = field == JM_FIELD_X ? h->h_pos : h->h_fiber;
+    for (size_t q = 0; q < (size_t)M * N; ++q)
+      if (h->h_mat[q] < 0) std::memcpy(&hv[2 * q], src + q * esz, esz);
+  }
+  std::vector<int> slots;
+  JM_TRY(read_slots(h, slots));
+  std::vector<char> buf;
+  for (int m = 0; m < M; ++m) {
+    const int n = h->n_live[m];
+    if (n == 0) continue;
+    buf.resize((size_t)n * esz);
+    const int* sl = slots.data() + (size_t)m * h->cap;
+    for (int i = 0; i < n; ++i)
+      std::memcpy(buf.data() + (size_t)i * esz, src + ((size_t)m * N + sl[i]) * esz, esz);
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(dptr) + (size_t)m * h->cap * esz, buf.data(),
+                           buf.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  if (field == JM_FIELD_X) h->since_sort = h->tun.sort_interval;  // positions moved: re-sort
+  return JM_OK;
+}
+
+int jm_live_count(jm_handle* h, int32_t instance, int64_t* n) {
+  JM_TRY(check_handle(h));
+  if (instance < 0 || instance >= h->M || !n) return fail(JM_ERR_ARG, "bad arguments");
+  JM_TRY(commit(h));
+  *n = h->n_live[instance];
+  return JM_OK;
+}
+
+int jm_read_device_order(jm_handle* h, int32_t instance, int32_t* slot, float* x, int64_t n) {
+  JM_TRY(check_handle(h));
+  if (instance < 0 || instance >= h->M) return fail(JM_ERR_ARG, "instance out of range");
+  JM_TRY(commit(h));
+  if (n != h->n_live[instance]) return fail(JM_ERR_ARG, "n must equal the live count");
+  const size_t off = (size_t)instance * h->cap;
+  if (n > 0 && slot)
+    HIP_TRY(hipMemcpyAsync(slot, h->st.s.slot + off, n * 4, hipMemcpyDeviceToHost, h->stream));
+  if (n > 0 && x)
+    HIP_TRY(hipMemcpyAsync(x, h->st.s.x + off, n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
+  JM_TRY(check_handle(h));
+  if (n < 2) return fail(JM_ERR_ARG, "n must be >= 2");
+  if (h->cfg.path != JM_PATH_FUSED) return fail(JM_ERR_STATE, "timed blocks need the fused path");
+  JM_TRY(commit(h));
+  if (h->total_chunks == 0) return fail(JM_ERR_STATE, "no live particles");
+  const bool sort = h->since_sort >= h->tun.sort_interval;
+  if (sort) JM_TRY(enqueue_sort(h));
+  const int b = h->acc_base;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  jm::FusedArgs a = fused_args(h);
+  int rc = JM_OK;
+  if (hipEventRecord(ev[0], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  enqueue_fused_kernel(h, a, 0, n, b);
+  if (hipEventRecord(ev[1], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  for (int kk = 1; kk < n; ++kk) enqueue_fused_kernel(h, a, kk, n, b);
+  if (hipEventRecord(ev[2], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  enqueue_fused_kernel(h, a, n, n, b);
+  if (hipEventRecord(ev[3], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
+  float mid = 0.f, all = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&mid, ev[1], ev[2]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&all, ev[0], ev[3]);
+  for (auto& ee : ev) (void)hipEventDestroy(ee);
+  if (e != hipSuccess) return fail(JM_ERR_HIP, std::string("timed block: ") + hipGetErrorString(e));
+  if (rc != JM_OK) return fail(rc, "hipEventRecord failed");
+  finish_block(h, n, b, sort);
+  if (fused_ms) *fused_ms = mid / (float)(n - 1);
+  if (block_ms) *block_ms = all;
+  return JM_OK;
+}
+
+int jm_sync(jm_handle* h) {
+  JM_TRY(check_handle(h));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+void* jm_stream(jm_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"

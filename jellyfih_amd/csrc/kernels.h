@@ -1,0 +1,669 @@
+// HIP kernels for gfx950 (MI355X): plain per-stage path, fused G2P->P2G path, cell sort,
+// statistics. Physics lives in physics.h; this file is about data movement.
+//
+// Device layout
+//   particles: structure of arrays in per-instance segments of `cap` slots; instance m's
+//     live particles occupy [m*cap, m*cap + n_live[m]) in cell-sorted order.
+//     x, v, fiber: float2; C, F: float4 (row-major 2x2); Jp, glow: float; mat, slot: int.
+//   grids: per instance three planes (px, py, mass) of nx*ny floats, node (i, j) at j*nx + i.
+//     The fused path keeps three such grid buffers and rotates them: each substep gathers
+//     from one, scatters into the next and zeroes the third.
+#pragma once
+
+#include <limits.h>
+
+#include "physics.h"
+
+namespace jm {
+
+struct State {
+  float2* x;
+  float2* v;
+  float4* C;
+  float4* F;
+  float* Jp;
+  int* mat;
+  float2* fiber;
+  float* glow;
+  int* slot;
+};
+
+__device__ __forceinline__ Mat2 as_mat(float4 a) { return {a.x, a.y, a.z, a.w}; }
+__device__ __forceinline__ float4 as_f4(const Mat2& m) {
+  return make_float4(m.a00, m.a01, m.a10, m.a11);
+}
+
+// ---------------------------------------------------------------------------------------
+// Small utility kernels (graph-safe replacements for memset).
+
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, int64_t n, int v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Reset the dynamic state of particles [begin, end): v=0, F=I, C=0, Jp=1, glow=0
+// (mpm_sim.py:722-725).
+__global__ void k_init_particles(State s, int64_t begin, int64_t end) {
+  for (int64_t i = begin + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < end;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    s.v[i] = make_float2(0.f, 0.f);
+    s.C[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    s.F[i] = make_float4(1.f, 0.f, 0.f, 1.f);
+    s.Jp[i] = 1.f;
+    if (s.glow) s.glow[i] = 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Plain path: one launch per stage, global atomics, any particle order.
+
+struct PlainArgs {
+  State s;
+  InstParams ip;
+  float* acc;          // one grid buffer: [m][3][G]
+  float* t_buf;        // t_buf[0] = sim_time
+  const int* n_live;
+  int cap;
+  int m0;              // first instance of this launch (blockIdx.y adds to it)
+  int G;
+  int glow;
+  PhysConsts k;
+};
+
+__global__ void __launch_bounds__(256) k_plain_p2g(PlainArgs a) {
+  const int m = a.m0 + blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= a.n_live[m]) return;
+  const PhysConsts& k = a.k;
+  const size_t i = (size_t)m * a.cap + local;
+  const int mt = a.s.mat[i];
+  if (mt < 0) return;
+  float2 x = a.s.x[i];
+  int b0, b1;
+  float fx0, fx1;
+  base_frac(x.x, k.inv_dx, b0, fx0);
+  base_frac(x.y, k.inv_dx, b1, fx1);
+  if (!in_range(b0, b1, k)) return;
+  const float act = activation(a.t_buf[0], a.ip.freq[m], a.ip.contraction[m], k);
+  float2 fib = make_float2(0.f, 1.f);
+  if (mt == 3) fib = a.s.fiber[i];
+  const Mat2 C = as_mat(a.s.C[i]);
+  P2GOut o = p2g_update(mt, as_mat(a.s.F[i]), C, fib.x, fib.y, a.ip.amp[m] * act, a.ip.mu_j[m],
+                        a.ip.la_j[m], a.ip.pdens[m], k);
+  a.s.F[i] = as_f4(o.F);
+  if (o.jp_changes) a.s.Jp[i] = (a.s.Jp[i] * o.q0) * o.q1;
+  const float2 v = a.s.v[i];
+  const float mv0 = o.mass * v.x, mv1 = o.mass * v.y;
+  float wx[3], wy[3];
+  bspline(fx0, wx);
+  bspline(fx1, wy);
+  float* px = a.acc + (size_t)m * 3 * a.G;
+  float* py = px + a.G;
+  float* pm = py + a.G;
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      const float d0 = ((float)ii - fx0) * k.dx, d1 = ((float)jj - fx1) * k.dx;
+      const float w = wx[ii] * wy[jj];
+      const int node = (b1 + jj) * k.nx + (b0 + ii);
+      unsafeAtomicAdd(px + node, w * (mv0 + (o.affine.a00 * d0 + o.affine.a01 * d1)));
+      unsafeAtomicAdd(py + node, w * (mv1 + (o.affine.a10 * d0 + o.affine.a11 * d1)));
+      unsafeAtomicAdd(pm + node, w * o.mass);
+    }
+  }
+}
+
+// Grid op in place: momentum planes become velocity (mpm_sim.py:274-309).
+__global__ void __launch_bounds__(256) k_plain_gridop(PlainArgs a) {
+  const int m = a.m0 + blockIdx.y;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= a.G) return;
+  float* px = a.acc + (size_t)m * 3 * a.G;
+  float* py = px + a.G;
+  const float* pm = py + a.G;
+  const int j = node / a.k.nx, i = node - j * a.k.nx;
+  float2 g = grid_node(px[node], py[node], pm[node], i, j, a.k);
+  px[node] = g.x;
+  py[node] = g.y;
+}
+
+// G2P, gravity, advection, clamp (mpm_sim.py:311-333) and the optional glow pass (:335-338).
+__global__ void __launch_bounds__(256) k_plain_g2p(PlainArgs a) {
+  const int m = a.m0 + blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= a.n_live[m]) return;
+  const PhysConsts& k = a.k;
+  const size_t i = (size_t)m * a.cap + local;
+  if (a.s.mat[i] < 0) return;
+  float2 x = a.s.x[i];
+  float2 v = a.s.v[i];
+  int b0, b1;
+  float fx0, fx1;
+  base_frac(x.x, k.inv_dx, b0, fx0);
+  base_frac(x.y, k.inv_dx, b1, fx1);
+  if (in_range(b0, b1, k)) {
+    const float* px = a.acc + (size_t)m * 3 * a.G;
+    const float* py = px + a.G;
+    float wx[3], wy[3];
+    bspline(fx0, wx);
+    bspline(fx1, wy);
+    float nv0 = 0.f, nv1 = 0.f;
+    Mat2 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const int node = (b1 + jj) * k.nx + (b0 + ii);
+        const float g0 = px[node], g1 = py[node];
+        const float d0 = (float)ii - fx0, d1 = (float)jj - fx1;
+        const float w = wx[ii] * wy[jj];
+        nv0 += w * g0;
+        nv1 += w * g1;
+        const float ws = k.apic_scale * w;
+        c.a00 += ws * (g0 * d0);
+        c.a01 += ws * (g0 * d1);
+        c.a10 += ws * (g1 * d0);
+        c.a11 += ws * (g1 * d1);
+      }
+    }
+    v = make_float2(nv0, nv1);
+    a.s.C[i] = as_f4(c);
+  }
+  advect(x, v, k);
+  a.s.v[i] = v;
+  a.s.x[i] = x;
+  if (a.glow) a.s.glow[i] = fmaxf(a.s.glow[i] * 0.9f, fabsf(1.0f - a.s.Jp[i]));
+}
+
+// sim_time += dt (mpm_sim.py:340)
+__global__ void k_time_advance(float* t_buf, float dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) t_buf[0] = t_buf[0] + dt;
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused path. One launch per substep: gather the previous substep's grid through LDS (the
+// grid op is applied on the fly while staging), G2P + advection, then P2G of the next
+// substep scattered into an LDS window and flushed with one global atomic per touched
+// node. DO_G2P=false is the block's first kernel (P2G only), DO_P2G=false its last.
+
+struct FusedArgs {
+  State s;
+  InstParams ip;
+  const float* acc_r;  // gather buffer (null when !DO_G2P)
+  float* acc_w;        // scatter buffer (null when !DO_P2G)
+  float* acc_z;        // buffer zeroed for the substep after next (may be null)
+  float* t_buf;
+  int t_rd, t_wr, t_adv;  // t_buf[t_wr] = t_buf[t_rd] (+dt if t_adv), when t_wr >= 0
+  const int4* chunks;     // (instance, first particle, count, chunk index in instance)
+  const int* nchunks;     // chunks per instance
+  int xcd_off[8];
+  int xcd_cnt[8];
+  int affinity;           // 1: blockIdx -> (XCD = blockIdx % 8, local = blockIdx / 8)
+  int total_chunks;
+  int G;
+  int lazy_vc;            // v, C of in-range particles are not stored between fused substeps
+  int glow;
+  PhysConsts k;
+};
+
+template <int BLOCK>
+__device__ __forceinline__ void block_bbox(int& mn0, int& mx0, int& mn1, int& mx1,
+                                           int (*red)[4]) {
+  int v[4] = {mn0, -mx0, mn1, -mx1};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[c] = min(v[c], __shfl_xor(v[c], off, 64));
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[wave][c] = v[c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    int r = red[0][c];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) r = min(r, red[w][c]);
+    v[c] = r;
+  }
+  __syncthreads();
+  mn0 = v[0];
+  mx0 = -v[1];
+  mn1 = v[2];
+  mx1 = -v[3];
+}
+
+constexpr int kBoxEmpty = 1 << 29;
+
+template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
+__global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
+  constexpr int WMAX = (BLOCK * PPT <= 4096) ? 2048 : 4096;
+  __shared__ float lds[3 * WMAX];
+  __shared__ int red[BLOCK / 64][4];
+  const int tid = threadIdx.x;
+  const PhysConsts& k = a.k;
+
+  if (a.t_wr >= 0 && blockIdx.x == 0 && tid == 0) {
+    const float t = a.t_buf[a.t_rd];
+    a.t_buf[a.t_wr] = a.t_adv ? t + k.dt : t;  // mpm_sim.py:340
+  }
+
+  int ci;
+  if (a.affinity) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    if (local >= a.xcd_cnt[xcd]) return;
+    ci = a.xcd_off[xcd] + local;
+  } else {
+    ci = blockIdx.x;
+    if (ci >= a.total_chunks) return;
+  }
+  const int4 ch = a.chunks[ci];
+  const int m = ch.x, start = ch.y, count = ch.z, cidx = ch.w;
+  const int nx = k.nx, G = a.G;
+
+  // Zero this chunk's share of the grid that the substep after next scatters into.
+  if (a.acc_z) {
+    const int nch = a.nchunks[m];
+    const int64_t n4 = (int64_t)3 * G / 4;
+    const int64_t lo = n4 * cidx / nch, hi = n4 * (cidx + 1) / nch;
+    float4* z = reinterpret_cast<float4*>(a.acc_z + (size_t)m * 3 * G);
+    for (int64_t q = lo + tid; q < hi; q += BLOCK) z[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+
+  float2 x[PPT], v[PPT];
+  float4 C[PPT];
+  int mt[PPT];
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int li = p * BLOCK + tid;
+    mt[p] = -1;
+    if (li < count) {
+      x[p] = a.s.x[start + li];
+      mt[p] = a.s.mat[start + li];
+    }
+  }
+
+  if constexpr (DO_G2P) {
+    // Window of grid nodes touched by this chunk's in-range particles.
+    int mn0 = kBoxEmpty, mx0 = -kBoxEmpty, mn1 = kBoxEmpty, mx1 = -kBoxEmpty;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      if (mt[p] < 0) continue;
+      int b0, b1;
+      float f0, f1;
+      base_frac(x[p].x, k.inv_dx, b0, f0);
+      base_frac(x[p].y, k.inv_dx, b1, f1);
+      if (!in_range(b0, b1, k)) continue;
+      mn0 = min(mn0, b0);
+      mx0 = max(mx0, b0);
+      mn1 = min(mn1, b1);
+      mx1 = max(mx1, b1);
+    }
+    block_bbox<BLOCK>(mn0, mx0, mn1, mx1, red);
+    const int ww = mx0 - mn0 + 3, wh = mx1 - mn1 + 3;
+    const bool any = mn0 <= mx0;
+    const bool use_lds = any && ww * wh <= WMAX;
+    const float* gp = a.acc_r + (size_t)m * 3 * G;
+    float2* lv = reinterpret_cast<float2*>(lds);
+    if (use_lds) {
+      const int nodes = ww * wh;
+      for (int q = tid; q < nodes; q += BLOCK) {
+        const int jj = q / ww, ii = q - jj * ww;
+        const int i = mn0 + ii, j = mn1 + jj;
+        const int node = j * nx + i;
+        lv[q] = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      if (mt[p] < 0) continue;
+      const int li = p * BLOCK + tid;
+      int b0, b1;
+      float fx0, fx1;
+      base_frac(x[p].x, k.inv_dx, b0, fx0);
+      base_frac(x[p].y, k.inv_dx, b1, fx1);
+      if (in_range(b0, b1, k)) {
+        float wx[3], wy[3];
+        bspline(fx0, wx);
+        bspline(fx1, wy);
+        float nv0 = 0.f, nv1 = 0.f;
+        Mat2 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) {
+            float2 g;
+            if (use_lds) {
+              g = lv[(b1 - mn1 + jj) * ww + (b0 - mn0 + ii)];
+            } else {
+              const int i = b0 + ii, j = b1 + jj, node = j * nx + i;
+              g = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
+            }
+            const float d0 = (float)ii - fx0, d1 = (float)jj - fx1;
+            const float w = wx[ii] * wy[jj];
+            nv0 += w * g.x;
+            nv1 += w * g.y;
+            const float ws = k.apic_scale * w;
+            c.a00 += ws * (g.x * d0);
+            c.a01 += ws * (g.x * d1);
+            c.a10 += ws * (g.y * d0);
+            c.a11 += ws * (g.y * d1);
+          }
+        }
+        v[p] = make_float2(nv0, nv1);
+        C[p] = as_f4(c);
+      } else {
+        v[p] = a.s.v[start + li];
+        C[p] = a.s.C[start + li];
+      }
+      advect(x[p], v[p], k);
+      if (a.glow) {
+        const int gi = start + li;
+        a.s.glow[gi] = fmaxf(a.s.glow[gi] * 0.9f, fabsf(1.0f - a.s.Jp[gi]));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      if (mt[p] < 0) continue;
+      const int li = p * BLOCK + tid;
+      v[p] = a.s.v[start + li];
+      C[p] = a.s.C[start + li];
+    }
+  }
+
+  if constexpr (DO_P2G) {
+    const int mi = m;
+    const float act = activation(a.t_buf[a.t_rd], a.ip.freq[mi], a.ip.contraction[mi], k);
+    const float amp_act = a.ip.amp[mi] * act;
+    const float mu_j = a.ip.mu_j[mi], la_j = a.ip.la_j[mi], pdens = a.ip.pdens[mi];
+
+    int mn0 = kBoxEmpty, mx0 = -kBoxEmpty, mn1 = kBoxEmpty, mx1 = -kBoxEmpty;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      if (mt[p] < 0) continue;
+      int b0, b1;
+      float f0, f1;
+      base_frac(x[p].x, k.inv_dx, b0, f0);
+      base_frac(x[p].y, k.inv_dx, b1, f1);
+      if (!in_range(b0, b1, k)) continue;
+      mn0 = min(mn0, b0);
+      mx0 = max(mx0, b0);
+      mn1 = min(mn1, b1);
+      mx1 = max(mx1, b1);
+    }
+    block_bbox<BLOCK>(mn0, mx0, mn1, mx1, red);
+    const int ww = mx0 - mn0 + 3, wh = mx1 - mn1 + 3;
+    const bool any = mn0 <= mx0;
+    const bool use_lds = any && ww * wh <= WMAX;
+    const int nodes = use_lds ? ww * wh : 0;
+    for (int q = tid; q < nodes; q += BLOCK) {
+      lds[q] = 0.f;
+      lds[WMAX + q] = 0.f;
+      lds[2 * WMAX + q] = 0.f;
+    }
+    __syncthreads();
+    float* gw = a.acc_w + (size_t)m * 3 * G;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      if (mt[p] < 0) continue;
+      const int li = p * BLOCK + tid;
+      const int gi = start + li;
+      int b0, b1;
+      float fx0, fx1;
+      base_frac(x[p].x, k.inv_dx, b0, fx0);
+      base_frac(x[p].y, k.inv_dx, b1, fx1);
+      if (!in_range(b0, b1, k)) {
+        // Skipped by P2G (mpm_sim.py:184); its v, C are carried in memory.
+        if (DO_G2P && a.lazy_vc) {
+          a.s.v[gi] = v[p];
+          a.s.C[gi] = C[p];
+        }
+        continue;
+      }
+      float2 fib = make_float2(0.f, 1.f);
+      if (mt[p] == 3) fib = a.s.fiber[gi];
+      const Mat2 Cm = as_mat(C[p]);
+      P2GOut o = p2g_update(mt[p], as_mat(a.s.F[gi]), Cm, fib.x, fib.y, amp_act, mu_j, la_j,
+                            pdens, k);
+      a.s.F[gi] = as_f4(o.F);
+      if (o.jp_changes) a.s.Jp[gi] = (a.s.Jp[gi] * o.q0) * o.q1;
+      const float mv0 = o.mass * v[p].x, mv1 = o.mass * v[p].y;
+      float wx[3], wy[3];
+      bspline(fx0, wx);
+      bspline(fx1, wy);
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+          const float d0 = ((float)ii - fx0) * k.dx, d1 = ((float)jj - fx1) * k.dx;
+          const float w = wx[ii] * wy[jj];
+          const float c0 = w * (mv0 + (o.affine.a00 * d0 + o.affine.a01 * d1));
+          const float c1 = w * (mv1 + (o.affine.a10 * d0 + o.affine.a11 * d1));
+          const float cm = w * o.mass;
+          if (use_lds) {
+            const int q = (b1 - mn1 + jj) * ww + (b0 - mn0 + ii);
+            atomicAdd(&lds[q], c0);
+            atomicAdd(&lds[WMAX + q], c1);
+            atomicAdd(&lds[2 * WMAX + q], cm);
+          } else {
+            const int node = (b1 + jj) * nx + (b0 + ii);
+            unsafeAtomicAdd(gw + node, c0);
+            unsafeAtomicAdd(gw + G + node, c1);
+            unsafeAtomicAdd(gw + 2 * G + node, cm);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int q = tid; q < nodes; q += BLOCK) {
+      const int jj = q / ww, ii = q - jj * ww;
+      const int node = (mn1 + jj) * nx + (mn0 + ii);
+      const float c0 = lds[q], c1 = lds[WMAX + q], cm = lds[2 * WMAX + q];
+      if (c0 != 0.f) unsafeAtomicAdd(gw + node, c0);
+      if (c1 != 0.f) unsafeAtomicAdd(gw + G + node, c1);
+      if (cm != 0.f) unsafeAtomicAdd(gw + 2 * G + node, cm);
+    }
+  }
+
+  if constexpr (DO_G2P) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      if (mt[p] < 0) continue;
+      const int gi = start + p * BLOCK + tid;
+      a.s.x[gi] = x[p];
+      if (!DO_P2G || !a.lazy_vc) {
+        a.s.v[gi] = v[p];
+        a.s.C[gi] = C[p];
+      }
+    }
+  }
+}
+
+// Grid velocity of one buffer as the G2P that read it saw it: (M, 2, G) planes.
+__global__ void k_grid_velocity(const float* acc, float* out, int M, int G, PhysConsts k) {
+  const int m = blockIdx.y;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || node >= G) return;
+  const float* gp = acc + (size_t)m * 3 * G;
+  const int j = node / k.nx, i = node - j * k.nx;
+  float2 g = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
+  out[(size_t)m * 2 * G + node] = g.x;
+  out[(size_t)m * 2 * G + G + node] = g.y;
+}
+
+// ---------------------------------------------------------------------------------------
+// Stable counting sort of each instance's particles by P2G base cell.
+
+struct SortArgs {
+  State s;      // live state (sorted in place)
+  State tmp;    // copy of the state before the sort
+  int* keys;    // per particle
+  int* perm;    // per sorted position: index (within segment) of the particle it receives
+  int* hist;    // [m][NC] counts
+  int* offs;    // [m][NC] exclusive offsets
+  int* cursor;  // [m][NC] scatter cursors
+  const int* n_live;
+  int cap;
+  int M;
+  int NC;       // nx * ny
+  PhysConsts k;
+};
+
+__device__ __forceinline__ int cell_key(float2 x, const PhysConsts& k) {
+  int b0, b1;
+  float f0, f1;
+  base_frac(x.x, k.inv_dx, b0, f0);
+  base_frac(x.y, k.inv_dx, b1, f1);
+  b0 = min(max(b0, 0), k.nx - 1);
+  b1 = min(max(b1, 0), k.ny - 1);
+  return b1 * k.nx + b0;
+}
+
+__global__ void __launch_bounds__(256) k_sort_hist(SortArgs a) {
+  const int m = blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= a.n_live[m]) return;
+  const size_t i = (size_t)m * a.cap + local;
+  const float2 x = a.s.x[i];
+  const int key = cell_key(x, a.k);
+  a.keys[i] = key;
+  atomicAdd(&a.hist[(size_t)m * a.NC + key], 1);
+  a.tmp.x[i] = x;
+  a.tmp.v[i] = a.s.v[i];
+  a.tmp.C[i] = a.s.C[i];
+  a.tmp.F[i] = a.s.F[i];
+  a.tmp.Jp[i] = a.s.Jp[i];
+  a.tmp.mat[i] = a.s.mat[i];
+  a.tmp.fiber[i] = a.s.fiber[i];
+  a.tmp.slot[i] = a.s.slot[i];
+  if (a.s.glow) a.tmp.glow[i] = a.s.glow[i];
+}
+
+// Exclusive scan of one instance's histogram (one workgroup of 1024 per instance).
+__global__ void __launch_bounds__(1024) k_sort_scan(SortArgs a) {
+  __shared__ int part[1024];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const int per = (a.NC + 1023) / 1024;
+  const int* h = a.hist + (size_t)m * a.NC;
+  int sum = 0;
+  for (int q = 0; q < per; ++q) {
+    const int c = tid * per + q;
+    if (c < a.NC) sum += h[c];
+  }
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    const int add = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += add;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int q = 0; q < per; ++q) {
+    const int c = tid * per + q;
+    if (c < a.NC) {
+      a.offs[(size_t)m * a.NC + c] = run;
+      a.cursor[(size_t)m * a.NC + c] = run;
+      run += h[c];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_scatter(SortArgs a) {
+  const int m = blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= a.n_live[m]) return;
+  const size_t i = (size_t)m * a.cap + local;
+  const int pos = atomicAdd(&a.cursor[(size_t)m * a.NC + a.keys[i]], 1);
+  a.perm[(size_t)m * a.cap + pos] = local;
+}
+
+// Within each cell bucket, order by previous position: makes the sort stable and
+// deterministic regardless of the scatter's atomic order.
+__global__ void __launch_bounds__(256) k_sort_fix(SortArgs a) {
+  const int m = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.NC) return;
+  const int n = a.hist[(size_t)m * a.NC + c];
+  if (n < 2) return;
+  int* b = a.perm + (size_t)m * a.cap + a.offs[(size_t)m * a.NC + c];
+  for (int q = 1; q < n; ++q) {
+    const int val = b[q];
+    int r = q - 1;
+    while (r >= 0 && b[r] > val) {
+      b[r + 1] = b[r];
+      --r;
+    }
+    b[r + 1] = val;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_gather(SortArgs a) {
+  const int m = blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= a.n_live[m]) return;
+  const size_t i = (size_t)m * a.cap + local;
+  const size_t src = (size_t)m * a.cap + a.perm[i];
+  a.s.x[i] = a.tmp.x[src];
+  a.s.v[i] = a.tmp.v[src];
+  a.s.C[i] = a.tmp.C[src];
+  a.s.F[i] = a.tmp.F[src];
+  a.s.Jp[i] = a.tmp.Jp[src];
+  a.s.mat[i] = a.tmp.mat[src];
+  a.s.fiber[i] = a.tmp.fiber[src];
+  a.s.slot[i] = a.tmp.slot[src];
+  if (a.s.glow) a.s.glow[i] = a.tmp.glow[src];
+}
+
+// ---------------------------------------------------------------------------------------
+// fitness_buffer sums (mpm_sim.py:746-762) for payload and body at once, accumulated in
+// double in a fixed order: out[m][0..5] = payload (sum_y, count, sum_x), body (same).
+
+__global__ void __launch_bounds__(256) k_stats(State s, const int* n_live, int cap, float* out) {
+  __shared__ double red[4][6];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int n = n_live[m];
+  for (int local = tid; local < n; local += 256) {
+    const size_t i = (size_t)m * cap + local;
+    const int mt = s.mat[i];
+    const float2 x = s.x[i];
+    if (mt == 2) {
+      acc[0] += x.y;
+      acc[1] += 1.0;
+      acc[2] += x.x;
+    } else if (mt == 1 || mt == 3) {
+      acc[3] += x.y;
+      acc[4] += 1.0;
+      acc[5] += x.x;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) red[tid >> 6][c] = acc[c];
+  }
+  __syncthreads();
+  if (tid < 6) {
+    const double r = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    out[(size_t)m * 6 + tid] = (float)r;
+  }
+}
+
+}  // namespace jm

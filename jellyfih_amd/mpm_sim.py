@@ -1,0 +1,292 @@
+"""Drop-in for the simulation core of the reference's ``mpm_sim`` module.
+
+c-m-x-r/jellyfih drives its Taichi MLS-MPM solver through module-level constants, fields
+with item access and a few functions (``mpm_sim.py:11-149,153-340,715-834``), called from
+``evolve.py:127-155,158-245,474-478`` and ``run_population.py:33-57``. This module keeps
+those names and their meaning, backed by the HIP engine (``libjellymx.so``):
+
+* configuration is read from the same environment variables at import time
+  (``JELLY_INSTANCES``, ``JELLY_PARTICLES``, ``JELLY_GRID_Y``, ``JELLY_DOMAIN_H``);
+* fields (``x``, ``v``, ``C``, ``F``, ``material``, ``Jp``, ``fiber_dir``, ``grid_v``,
+  ``grid_m``, ``fitness_buffer``, ``sim_time``, ``instance_*``) support ``to_numpy()``,
+  ``from_numpy()`` and item get/set like Taichi fields;
+* ``substep()``, ``load_particles()``, ``get_payload_stats()``, ``run_batch_headless()``
+  and the fitness-buffer kernels behave as in the reference.
+
+The device engine is created on first use, so importing this module never touches a GPU.
+The renderer (``mpm_sim.py:343-712``) and the axisymmetric variant are not part of this
+engine; their names raise ``NotImplementedError``.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+# --- configuration (mpm_sim.py:11-58) ----------------------------------------------------
+n_instances = int(os.environ.get("JELLY_INSTANCES", "16"))
+quality = 1
+n_particles = int(os.environ.get("JELLY_PARTICLES", "80000"))
+n_grid = 128 * quality
+n_grid_y = int(os.environ.get("JELLY_GRID_Y", str(n_grid)))
+domain_height = float(os.environ.get("JELLY_DOMAIN_H", "1.0"))
+axisym = os.environ.get("JELLY_AXISYM", "0") == "1"
+dx, inv_dx = 1 / n_grid, float(n_grid)
+dt = 2e-5 / quality
+p_vol = (dx * 0.5) ** 2
+p_rho = 1
+p_mass = p_vol * p_rho
+
+E_base = 0.1e4
+nu_base = 0.2
+mu_0_base = E_base / (2 * (1 + nu_base))
+lambda_0_base = E_base * nu_base / ((1 + nu_base) * (1 - 2 * nu_base))
+E_jelly = 0.7e3
+nu_jelly = 0.3
+mu_jelly = E_jelly / (2 * (1 + nu_jelly))
+lambda_jelly = E_jelly * nu_jelly / ((1 + nu_jelly) * (1 - 2 * nu_jelly))
+E_payload = 2.0e5
+nu_payload = 0.2
+mu_payload = E_payload / (2 * (1 + nu_payload))
+lambda_payload = E_payload * nu_payload / ((1 + nu_payload) * (1 - 2 * nu_payload))
+water_lambda = 100000.0
+gravity = 10.0
+actuation_freq = 1
+actuation_strength = 500.0
+ACT_CONTRACTION_END = 0.2
+ACT_RELAXATION_END = 0.6
+
+WEB_PALETTE = [
+    (0, 0.933, 0.949, 0.957),
+    (1, 0.678, 0.882, 0.875),
+    (3, 0.976, 0.757, 0.765),
+    (2, 0.961, 0.824, 0.576),
+]
+video_res = 1024
+grid_side = int(math.ceil(math.sqrt(n_instances)))
+res_sub = video_res // grid_side
+
+if axisym:
+    raise NotImplementedError("JELLY_AXISYM=1: the axisymmetric variant is not in this engine")
+
+_engine = None
+
+
+def engine():
+    """The device engine behind this module (created on first use)."""
+    global _engine
+    if _engine is None:
+        from .engine import Engine
+        _engine = Engine(n_instances=n_instances, n_particles=n_particles, quality=quality,
+                         n_grid_y=n_grid_y, domain_height=domain_height,
+                         track_glow=os.environ.get("JELLY_GLOW", "0") == "1")
+    return _engine
+
+
+# --- fields --------------------------------------------------------------------------------
+class _Field:
+    """Taichi-like field over an engine array: to_numpy / from_numpy / item access."""
+
+    def __init__(self, name: str):
+        self._name = name
+
+    @property
+    def shape(self):
+        from .engine import FIELD_IDS, FIELD_SHAPES
+        fid = FIELD_IDS[self._name]
+        full = engine().field_shape(fid)
+        return full[:len(full) - len(FIELD_SHAPES[fid][1])]
+
+    def to_numpy(self) -> np.ndarray:
+        return engine().read(self._name)
+
+    def from_numpy(self, arr) -> None:
+        engine().write(self._name, arr)
+
+    def __getitem__(self, idx):
+        return self.to_numpy()[idx]
+
+    def __setitem__(self, idx, value) -> None:
+        arr = self.to_numpy()
+        arr[idx] = value
+        self.from_numpy(arr)
+
+
+class _InstanceField:
+    """Per-instance scalar field (``instance_actuation[i] = ...``)."""
+
+    def __init__(self, param: str):
+        self._param = param
+
+    @property
+    def shape(self):
+        return (n_instances,)
+
+    def to_numpy(self) -> np.ndarray:
+        return engine().get_param(self._param)
+
+    def from_numpy(self, arr) -> None:
+        engine().set_param(self._param, arr)
+
+    def __getitem__(self, i):
+        return float(self.to_numpy()[i])
+
+    def __setitem__(self, i, value) -> None:
+        vals = self.to_numpy()
+        vals[i] = value
+        self.from_numpy(vals)
+
+
+class _TimeField:
+    """``sim_time[None]``: the fp32 simulation clock (mpm_sim.py:95,340)."""
+
+    shape = ()
+
+    def __getitem__(self, idx):
+        return engine().get_time()
+
+    def __setitem__(self, idx, value) -> None:
+        engine().set_time(float(value))
+
+    def to_numpy(self) -> np.ndarray:
+        return np.float32(engine().get_time())
+
+
+class _HostField:
+    """Render-only per-instance settings kept on the host (mpm_sim.py:136-138,102)."""
+
+    def __init__(self, values):
+        self._v = np.array(values, dtype=np.float64)
+
+    @property
+    def shape(self):
+        return self._v.shape
+
+    def to_numpy(self):
+        return self._v.copy()
+
+    def from_numpy(self, arr):
+        self._v[...] = arr
+
+    def __getitem__(self, i):
+        return self._v[() if i is None else i]
+
+    def __setitem__(self, i, value):
+        self._v[() if i is None else i] = value
+
+
+class _FitnessBuffer:
+    """fitness_buffer (M,3) = [sum_y, count, sum_x] accumulated by the stats kernels."""
+
+    def __init__(self):
+        self._buf = np.zeros((n_instances, 3), np.float32)
+
+    shape = (n_instances, 3)
+
+    def to_numpy(self) -> np.ndarray:
+        return self._buf.copy()
+
+    def __getitem__(self, idx):
+        return self._buf[idx]
+
+
+x = _Field("x")
+v = _Field("v")
+C = _Field("C")
+F = _Field("F")
+material = _Field("material")
+Jp = _Field("Jp")
+fiber_dir = _Field("fiber_dir")
+visual_glow = _Field("visual_glow")
+grid_v = _Field("grid_v")
+grid_m = _Field("grid_m")
+sim_time = _TimeField()
+fitness_buffer = _FitnessBuffer()
+instance_actuation = _InstanceField("actuation")
+instance_act_contraction = _InstanceField("contraction")
+instance_freq = _InstanceField("freq")
+instance_payload_density = _InstanceField("payload_density")
+instance_e_jelly = _InstanceField("e_jelly")
+instance_hue = _HostField([0.55] * n_instances)
+instance_muscle_hue = _HostField([(0.55 + 0.15) % 1.0] * n_instances)
+water_angle_color = _HostField(0)
+
+
+# --- kernels -------------------------------------------------------------------------------
+def substep() -> None:
+    """One MLS-MPM substep of every instance (mpm_sim.py:153-340)."""
+    engine().substep(1)
+
+
+def step(n: int) -> None:
+    """``n`` substeps in one call (what ``for _ in range(n): substep()`` does)."""
+    engine().substep(int(n))
+
+
+def load_particles(instance, pos_np, mat_np, fiber_np=None) -> None:
+    """Reset one instance with new particles (mpm_sim.py:730-736); fibres default to (0,1)."""
+    engine().load_particles(int(instance), pos_np, mat_np, fiber_np)
+
+
+def clear_fitness_buffer() -> None:
+    """mpm_sim.py:740-744"""
+    fitness_buffer._buf[...] = 0
+
+
+def compute_payload_stats() -> None:
+    """mpm_sim.py:746-753 (adds into fitness_buffer)."""
+    fitness_buffer._buf += engine().raw_stats("payload")
+
+
+def compute_body_stats() -> None:
+    """mpm_sim.py:755-762 (adds into fitness_buffer)."""
+    fitness_buffer._buf += engine().raw_stats("body")
+
+
+def _finish(raw: np.ndarray) -> np.ndarray:
+    stats = np.zeros((raw.shape[0], 3))
+    for i in range(raw.shape[0]):
+        count = raw[i, 1]
+        if count > 0:
+            stats[i, 0] = raw[i, 0] / count
+            stats[i, 1] = raw[i, 2] / count
+            stats[i, 2] = count
+    return stats
+
+
+def get_payload_stats() -> np.ndarray:
+    """Payload CoM per instance: (M,3) = [com_y, com_x, count] (mpm_sim.py:764-777)."""
+    clear_fitness_buffer()
+    compute_payload_stats()
+    return _finish(fitness_buffer.to_numpy())
+
+
+def _get_stats_any() -> np.ndarray:
+    """Payload CoM, or body CoM where an instance has no payload (mpm_sim.py:779-800)."""
+    raw = engine().raw_stats("any")
+    fitness_buffer._buf[...] = raw
+    return _finish(raw)
+
+
+def run_batch_headless(steps: int) -> np.ndarray:
+    """(M,5) = [init_y, init_x, final_y, final_x, valid] (mpm_sim.py:802-834)."""
+    return engine().run_batch_headless(int(steps))
+
+
+def sync() -> None:
+    """ti.sync()"""
+    engine().sync()
+
+
+_RENDER_NAMES = {
+    "clear_frame_buffer", "clear_frame_buffer_white", "render_frame_abyss", "render_flat_pass",
+    "tone_map_and_encode", "compute_vorticity_grid", "render_vorticity_overlay",
+    "render_vorticity_rdbu", "render_vorticity_hueshift", "frame_buffer", "vort_grid",
+}
+
+
+def __getattr__(name):
+    if name in _RENDER_NAMES:
+        raise NotImplementedError(f"mpm_sim.{name}: the renderer is not part of this engine")
+    raise AttributeError(name)

@@ -1,0 +1,58 @@
+"""Shared input builders for the parity tests (host-side, deterministic)."""
+from __future__ import annotations
+
+import numpy as np
+
+from jellyfih_amd import phenotype
+
+
+def small_tank(seed: int, n_slots: int, region=(0.3, 0.7, 0.25, 0.65), ppc_spacing=1 / 256,
+               robot=True, dead_every: int = 0):
+    """A small water block (lattice at the reference spacing, 4 per cell) with an optional
+    jelly/muscle/payload patch in it; optionally interleaved dead slots. Fits n_slots."""
+    rng = np.random.default_rng(seed)
+    x0, x1, y0, y1 = region
+    xs = np.arange(x0, x1, ppc_spacing)
+    ys = np.arange(y0, y1, ppc_spacing)
+    gx, gy = np.meshgrid(xs, ys)
+    pts = np.vstack([gx.ravel(), gy.ravel()]).T
+    mat = np.zeros(len(pts), np.int32)
+    fib = np.zeros((len(pts), 2), np.float32)
+    fib[:, 1] = 1.0
+    if robot:
+        cx, cy = (x0 + x1) / 2, (y0 + y1) / 2
+        d = np.abs(pts - [cx, cy])
+        mat[(d[:, 0] < 0.04) & (d[:, 1] < 0.02)] = 2                        # payload block
+        ring = (np.hypot(pts[:, 0] - cx, pts[:, 1] - cy - 0.06) < 0.035) & (mat == 0)
+        mat[ring] = 1
+        mus = ring & (pts[:, 1] < cy + 0.06)
+        mat[mus] = 3
+        ang = rng.uniform(0, 2 * np.pi, mus.sum())
+        fib[mus] = np.stack([np.cos(ang), np.sin(ang)], 1)
+    order = np.argsort(-(mat > 0), kind="stable")        # robot first, like fill_tank
+    pts, mat, fib = pts[order], mat[order], fib[order]
+    pts = pts + rng.uniform(-0.2, 0.2, pts.shape) * ppc_spacing
+    n = len(pts)
+    assert n <= n_slots, (n, n_slots)
+    pos = np.full((n_slots, 2), -1.0, np.float32)
+    mats = np.full(n_slots, -1, np.int32)
+    fibers = np.zeros((n_slots, 2), np.float32)
+    fibers[:, 1] = 1.0
+    if dead_every:
+        slots = [i for i in range(n_slots) if (i % dead_every) != dead_every - 1][:n]
+        assert len(slots) == n
+        slots = np.array(slots)
+    else:
+        slots = np.arange(n)
+    pos[slots] = pts
+    mats[slots] = mat
+    fibers[slots] = fib
+    return pos, mats, fibers
+
+
+def reference_tank(seed, n_slots=80000, domain_height=1.0, spawn=None):
+    """fill_tank of the seeded genome (make_jelly.py:336-402 restated)."""
+    g = phenotype.seeded_genome(seed) if not isinstance(seed, str) else phenotype.AURELIA_GENOME
+    pos, mat, fib, info = phenotype.fill_tank(g, n_slots, 128, spawn_offset=spawn,
+                                              domain_height=domain_height)
+    return pos, mat, fib, g, info

@@ -1,0 +1,240 @@
+"""Parity of the HIP engine against the oracle (CPU restatement of mpm_sim.py:153-340).
+
+Every test calls the engine through its C ABI (libjellymx.so via ctypes) and the oracle
+(oracle/liboracle.so) on the same seeded inputs. Tolerances:
+
+* integer outputs (materials, live counts, sort permutation, clock bits): bit-exact;
+* grid mass and momentum of one substep: max |gpu - ref| <= 1e-5 * max |ref|;
+* particle x after 1000 substeps of a full reference tank: within 1e-4 of the domain size
+  (1.0) of the fp32 oracle; v, C, F within the stated bounds; the fp32 oracle's own error
+  against the fp64 oracle is reported beside it.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from jellyfih_amd.engine import Engine
+from tests.cases import reference_tank, small_tank
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(cases, path="fused", dtype=np.float32, **tuning):
+    M = len(cases)
+    N = cases[0][0].shape[0]
+    eng = Engine(M, N, path=path)
+    if tuning:
+        eng.set_tuning(**tuning)
+    orc = oracle.OracleSim(M, N, dtype=dtype)
+    for m, (pos, mat, fib) in enumerate(cases):
+        eng.load_particles(m, pos, mat, fib)
+        orc.load(m, pos, mat, fib)
+    return eng, orc
+
+
+def perturb(eng, orc, seed=0, sv=0.05, sc=0.5, sf=0.02, t=0.1, orcs=()):
+    """Give both engines the same non-trivial state: random v, C, F near I, mid-contraction."""
+    rng = np.random.default_rng(seed)
+    live = (orc.material >= 0)
+    v = (rng.normal(size=orc.v.shape) * sv).astype(np.float32) * live[..., None]
+    C = (rng.normal(size=orc.C.shape) * sc).astype(np.float32) * live[..., None, None]
+    F = (np.eye(2) + rng.normal(size=orc.F.shape) * sf).astype(np.float32)
+    F[~live] = np.eye(2, dtype=np.float32)
+    eng.write("v", v)
+    eng.write("C", C)
+    eng.write("F", F)
+    eng.set_time(t)
+    for o in (orc,) + tuple(orcs):
+        o.v[...] = v
+        o.C[...] = C
+        o.F[...] = F
+        o.t[0] = np.float32(t)
+
+
+def relmax(a, b):
+    den = max(float(np.max(np.abs(b))), 1e-30)
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)))) / den
+
+
+def ragged_cases(n_slots=20000):
+    return [
+        small_tank(1, n_slots, region=(0.30, 0.70, 0.25, 0.55)),
+        small_tank(2, n_slots, region=(0.05, 0.45, 0.02, 0.20), dead_every=7),
+        small_tank(3, n_slots, region=(0.55, 0.995, 0.70, 0.995)),   # touches the skip strip
+    ]
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_one_substep_grid_and_particles(require_gpu, path):
+    say(f"one substep grid+particles path={path}")
+    eng, orc = make_pair(ragged_cases(), path=path)
+    perturb(eng, orc, seed=11)
+    eng.substep(1)
+    orc.run(1)
+    gm, gv = eng.read("grid_m"), eng.read("grid_v")
+    assert relmax(gm, orc.grid_m) <= 1e-5, relmax(gm, orc.grid_m)
+    assert relmax(gv, orc.grid_v) <= 1e-5, relmax(gv, orc.grid_v)
+    # momentum (not only velocity) per node
+    mom = gv * gm[..., None]
+    assert relmax(mom, orc.grid_v * orc.grid_m[..., None]) <= 1e-5
+    x, v, C, F, Jp = (eng.read(n) for n in ("x", "v", "C", "F", "Jp"))
+    assert np.max(np.abs(x - orc.x)) <= 2e-7
+    assert relmax(v, orc.v) <= 1e-5
+    assert relmax(C, orc.C) <= 1e-4
+    assert np.max(np.abs(F - orc.F)) <= 1e-6
+    assert np.max(np.abs(Jp - orc.Jp)) <= 1e-6
+    np.testing.assert_array_equal(eng.read("material"), orc.material)
+    assert eng.get_time() == orc.t[0]
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_ragged_batch_200_substeps(require_gpu, path):
+    say(f"ragged batch 200 substeps path={path}")
+    eng, orc = make_pair(ragged_cases(), path=path)
+    perturb(eng, orc, seed=5, sv=0.02, sc=0.2, sf=0.01, t=0.05)
+    eng.substep(200)
+    orc.run(200)
+    x = eng.read("x")
+    assert np.max(np.abs(x - orc.x)) <= 1e-5
+    assert relmax(eng.read("v"), orc.v) <= 1e-3
+    for m in range(3):
+        assert eng.live_count(m) == int((orc.material[m] >= 0).sum())
+
+
+def test_sort_permutation_bitexact(require_gpu):
+    say("sort permutation vs stable counting sort")
+    eng, orc = make_pair(ragged_cases())
+    perturb(eng, orc, seed=3, sv=0.5)
+    eng.set_tuning(sort_interval=1000000)   # no automatic sort inside the steps below
+    eng.substep(50)
+    c = oracle.reference_consts()
+    for m in range(3):
+        slot0, x0 = eng.device_order(m)
+        eng.sort_now()
+        slot1, x1 = eng.device_order(m)
+        perm = oracle.stable_cell_sort(c, x0)
+        np.testing.assert_array_equal(slot1, slot0[perm])
+        np.testing.assert_array_equal(x1, x0[perm])
+        keys = oracle.cell_keys(c, x1)
+        assert np.all(np.diff(keys) >= 0)
+
+
+def test_reference_tank_1000_substeps(require_gpu):
+    """cfg1: one seeded reference tank (~62K live of 80K slots), 1000 substeps."""
+    say("reference tank seed0, 1000 substeps, fused vs oracle f32/f64")
+    pos, mat, fib, g, info = reference_tank(0)
+    eng = Engine(1, 80000)
+    eng.load_particles(0, pos, mat, fib)
+    eng.set_param("contraction", float(np.clip(g[9], 0.05, 0.6)))
+    eng.set_param("freq", float(np.clip(g[10], 0.5, 2.0)))
+    o32 = oracle.OracleSim(1, 80000, dtype=np.float32)
+    o64 = oracle.OracleSim(1, 80000, dtype=np.float64)
+    for o in (o32, o64):
+        o.load(0, pos, mat, fib)
+        o.contraction[:] = np.float32(np.clip(g[9], 0.05, 0.6))
+        o.freq[:] = np.float32(np.clip(g[10], 0.5, 2.0))
+    eng.substep(1000)
+    o32.run(1000, nthreads=8)
+    o64.run(1000, nthreads=8)
+    out = {n: eng.read(n) for n in ("x", "v", "C", "F")}
+    ref32 = {"x": o32.x, "v": o32.v, "C": o32.C, "F": o32.F}
+    ref64 = {"x": o64.x, "v": o64.v, "C": o64.C, "F": o64.F}
+    live = mat >= 0
+    for n in out:
+        e_gpu = float(np.max(np.abs(out[n][0][live] - ref64[n][0][live])))
+        e_f32 = float(np.max(np.abs(ref32[n][0][live] - ref64[n][0][live])))
+        e_32 = float(np.max(np.abs(out[n][0][live] - ref32[n][0][live])))
+        say(f"{n}: |gpu-f64|={e_gpu:.3e} |oracle32-f64|={e_f32:.3e} |gpu-oracle32|={e_32:.3e}")
+    assert np.max(np.abs(out["x"][0][live] - ref32["x"][0][live])) <= 1e-4
+    assert np.max(np.abs(out["x"][0][live] - ref64["x"][0][live])) <= 1e-4
+    assert np.max(np.abs(out["v"][0][live] - ref64["v"][0][live])) <= 1e-2
+    assert np.max(np.abs(out["F"][0][live] - ref64["F"][0][live])) <= 1e-3
+    assert eng.get_time() == o32.t[0]
+
+
+def test_headless_matches_oracle(require_gpu):
+    say("run_batch_headless vs oracle")
+    cases = [reference_tank(s)[:3] for s in (1, 2)]
+    eng, orc = make_pair(cases)
+    r_gpu = eng.run_batch_headless(300)
+    r_ref = orc.run_batch_headless(300, nthreads=8)
+    np.testing.assert_array_equal(r_gpu[:, 4], r_ref[:, 4])
+    assert np.max(np.abs(r_gpu[:, :4] - r_ref[:, :4])) <= 1e-5
+
+
+def test_variants_agree(require_gpu):
+    """Graphs on/off, lazy v/C on/off, XCD affinity and several launch shapes."""
+    say("fused variants")
+    cases = ragged_cases(12000)[:2] * 5          # 10 instances (affinity needs >= 8)
+    base = None
+    variants = [dict(), dict(use_graphs=0), dict(lazy_vc=0), dict(xcd_affinity=0),
+                dict(threads=256, ppt=4), dict(threads=512, ppt=8), dict(threads=128, ppt=16),
+                dict(sort_interval=7, block_substeps=5)]
+    for var in variants:
+        say(f"variant {var}")
+        eng, orc = make_pair(cases, **var)
+        perturb(eng, orc, seed=2, sv=0.02, sc=0.2, sf=0.01)
+        eng.substep(60)
+        x = eng.read("x")
+        if base is None:
+            orc.run(60)
+            base = orc.x.copy()
+        assert np.max(np.abs(x - base)) <= 1e-5, var
+
+
+def test_clock_and_params(require_gpu):
+    say("clock accumulation and per-instance params")
+    eng, orc = make_pair(ragged_cases(8000)[:1])
+    eng.set_time(0.25)
+    eng.substep(333)
+    t = np.float32(0.25)
+    for _ in range(333):
+        t = np.float32(t + np.float32(2e-5))
+    assert eng.get_time() == t
+    eng.set_param("actuation", [123.0])
+    assert eng.get_param("actuation")[0] == np.float32(123.0)
+
+
+def test_empty_and_tiny_instances(require_gpu):
+    say("empty and tiny instances")
+    N = 5000
+    pos, mat, fib = small_tank(4, N, region=(0.4, 0.6, 0.4, 0.5))
+    dead = (np.full((N, 2), -1, np.float32), np.full(N, -1, np.int32), None)
+    tiny_pos = pos.copy()
+    tiny_mat = np.full(N, -1, np.int32)
+    tiny_mat[:3] = mat[:3]
+    eng = Engine(3, N)
+    orc = oracle.OracleSim(3, N)
+    for m, (p, mt, f) in enumerate([(pos, mat, fib), dead, (tiny_pos, tiny_mat, fib)]):
+        eng.load_particles(m, p, mt, f)
+        orc.load(m, p, mt, f)
+    eng.substep(20)
+    orc.run(20)
+    assert eng.live_count(1) == 0
+    assert np.max(np.abs(eng.read("x") - orc.x)) <= 1e-6
+    st = eng.raw_stats("any")
+    ref = orc.raw_stats(False)
+    assert st[1, 1] == 0
+    assert abs(st[0, 0] - ref[0, 0]) <= 1e-4 * max(1.0, abs(ref[0, 0]))
+
+
+def test_field_write_roundtrip(require_gpu):
+    say("field write/read round trip")
+    eng, orc = make_pair(ragged_cases(6000)[:2])
+    rng = np.random.default_rng(9)
+    for name in ("x", "v", "C", "F", "Jp", "fiber_dir"):
+        shape = eng.field_shape({"x": 0, "v": 1, "C": 2, "F": 3, "Jp": 5, "fiber_dir": 6}[name])
+        val = rng.uniform(0.1, 0.9, size=shape).astype(np.float32)
+        eng.write(name, val)
+        back = eng.read(name)
+        live = orc.material >= 0
+        np.testing.assert_array_equal(back[live], val[live])
