@@ -29,7 +29,7 @@ def small_tank(seed: int, n_slots: int, region=(0.3, 0.7, 0.25, 0.65), ppc_spaci
         mat[mus] = 3
         ang = rng.uniform(0, 2 * np.pi, mus.sum())
         fib[mus] = np.stack([np.cos(ang), np.sin(ang)], 1)
-    order = np.argsort(-(mat > 0), kind="stable")        # robot first, like fill_tank
+    order = np.argsort(~(mat > 0), kind="stable")        # robot first, like fill_tank
     pts, mat, fib = pts[order], mat[order], fib[order]
     pts = pts + rng.uniform(-0.2, 0.2, pts.shape) * ppc_spacing
     n = len(pts)

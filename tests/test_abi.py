@@ -1,0 +1,89 @@
+"""The C ABI library: loads, exports every symbol include/jellymx.h declares, validates its
+arguments before touching a device (no compute calls here: this runs without a GPU)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from jellyfih_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "jellymx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void\*|const char\*)\s+(jm_\w+)\s*\(", text,
+                                 re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == syms
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(_lib.JmConfig) == 40
+    assert _lib.JmConfig.domain_height.offset == 32
+    assert ctypes.sizeof(_lib.JmTuning) == 32
+
+
+def test_default_config_is_the_reference_default():
+    cfg = _lib.JmConfig()
+    _lib.call("jm_default_config", ctypes.byref(cfg))
+    assert cfg.abi_version == _lib.JM_ABI_VERSION
+    assert (cfg.n_instances, cfg.n_particles, cfg.quality) == (16, 80000, 1)   # mpm_sim.py:11-13
+    assert cfg.n_grid_y == 0 and cfg.domain_height == 1.0 and cfg.path == _lib.JM_PATH_FUSED
+
+
+@pytest.mark.parametrize("field,value", [("abi_version", 99), ("n_instances", 0),
+                                         ("n_particles", -5), ("quality", 0), ("path", 7),
+                                         ("n_grid_y", 6), ("domain_height", 0.0)])
+def test_create_rejects_bad_config_before_device_work(field, value):
+    cfg = _lib.JmConfig()
+    _lib.call("jm_default_config", ctypes.byref(cfg))
+    setattr(cfg, field, value)
+    h = ctypes.c_void_p()
+    rc = _lib.load().jm_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc == _lib.JM_ERR_ARG
+    assert not h.value
+    assert _lib.load().jm_last_error()
+
+
+def test_null_handle_is_rejected():
+    L = _lib.load()
+    assert L.jm_substep(None, 1) == _lib.JM_ERR_ARG
+    assert L.jm_sync(None) == _lib.JM_ERR_ARG
+    assert L.jm_stream(None) is None
+    with pytest.raises(_lib.JmError):
+        _lib.call("jm_set_time", None, ctypes.c_float(0.0))
+
+
+def test_engine_fails_loudly_without_a_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from jellyfih_amd.engine import Engine
+    with pytest.raises(_lib.JmError):
+        Engine(2, 100)
+
+
+def test_facade_constants_match_reference():
+    from jellyfih_amd import mpm_sim as s
+    assert s.n_grid == 128 and s.dt == 2e-5 and s.inv_dx == 128.0
+    assert s.p_vol == (1 / 256) ** 2 and s.p_mass == s.p_vol
+    assert s.mu_jelly == pytest.approx(269.2307692307692)
+    assert s.lambda_jelly == pytest.approx(403.8461538461538)
+    assert s.mu_payload == pytest.approx(83333.33333333333)
+    assert s.lambda_payload == pytest.approx(55555.555555555555)
+    assert s.mu_0_base == pytest.approx(416.6666666666667)
+    assert s.lambda_0_base == pytest.approx(277.77777777777777)
+    assert s.grid_side == 4 and s.res_sub == 256
+    with pytest.raises(NotImplementedError):
+        s.render_frame_abyss
