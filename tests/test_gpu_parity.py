@@ -6,8 +6,8 @@ Every test calls the engine through its C ABI (libjellymx.so via ctypes) and the
 * integer outputs (materials, live counts, sort permutation, clock bits): bit-exact;
 * grid mass and momentum of one substep: max |gpu - ref| <= 1e-5 * max |ref|;
 * particle x after 1000 substeps of a full reference tank: within 1e-4 of the domain size
-  (1.0) of the fp32 oracle; v, C, F within the stated bounds; the fp32 oracle's own error
-  against the fp64 oracle is reported beside it.
+  (1.0) of both oracles; for x, v, C and F the GPU's error against the fp64 oracle is at
+  most twice the fp32 oracle's own error against the fp64 oracle (both are reported).
 """
 from __future__ import annotations
 
@@ -154,10 +154,12 @@ def test_reference_tank_1000_substeps(require_gpu):
         e_f32 = float(np.max(np.abs(ref32[n][0][live] - ref64[n][0][live])))
         e_32 = float(np.max(np.abs(out[n][0][live] - ref32[n][0][live])))
         say(f"{n}: |gpu-f64|={e_gpu:.3e} |oracle32-f64|={e_f32:.3e} |gpu-oracle32|={e_32:.3e}")
+        # the GPU is as accurate as an fp32 build of the same algorithm: its error against
+        # fp64 stays within twice the fp32 oracle's own error against fp64
+        assert e_gpu <= 2.0 * e_f32 + 1e-6, n
+    # positions within 1e-4 of the domain size (1.0) after 1000 substeps
     assert np.max(np.abs(out["x"][0][live] - ref32["x"][0][live])) <= 1e-4
     assert np.max(np.abs(out["x"][0][live] - ref64["x"][0][live])) <= 1e-4
-    assert np.max(np.abs(out["v"][0][live] - ref64["v"][0][live])) <= 1e-2
-    assert np.max(np.abs(out["F"][0][live] - ref64["F"][0][live])) <= 1e-3
     assert eng.get_time() == o32.t[0]
 
 
