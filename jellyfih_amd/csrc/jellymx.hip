@@ -60,11 +60,10 @@ struct jm_handle {
 
   // device buffers
   DevState st, tmp;
-  int* keys = nullptr;
-  int* perm = nullptr;
-  int* hist = nullptr;  // hist, offs, cursor: 3 * M * G ints
-  int* offs = nullptr;
-  int* cursor = nullptr;
+  int* sortbuf = nullptr;  // key_a, val_a, key_b, val_b, rank: 5 * M * cap ints
+  int* hist = nullptr;     // radix histogram: M * 256 * tmax ints
+  int tmax = 0;            // radix tiles per instance segment
+  int key_bits = 0;        // bits of the largest cell key
   float* acc = nullptr;       // 3 grid buffers of M*3*G floats
   float* t_buf = nullptr;     // 2 floats
   float* params = nullptr;    // 6 * M floats: amp, contraction, freq, pdens, mu_j, la_j
@@ -108,12 +107,12 @@ struct jm_handle {
   void release() {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
-    void* ptrs[] = {st.base, tmp.base, keys, perm, hist, acc, t_buf, params, n_live_d,
+    void* ptrs[] = {st.base, tmp.base, sortbuf, hist, acc, t_buf, params, n_live_d,
                     chunks_d, nchunks_d, stats_d, staging};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     st.base = tmp.base = nullptr;
-    keys = perm = hist = nullptr;
+    sortbuf = hist = nullptr;
     acc = t_buf = params = stats_d = staging = nullptr;
     n_live_d = nullptr;
     chunks_d = nullptr;
@@ -350,33 +349,43 @@ int commit(jm_handle* h) {
 
 jm::SortArgs sort_args(jm_handle* h) {
   jm::SortArgs a;
+  const size_t n = (size_t)h->M * h->cap;
   a.s = h->st.s;
   a.tmp = h->tmp.s;
-  a.keys = h->keys;
-  a.perm = h->perm;
+  a.key_a = h->sortbuf;
+  a.val_a = h->sortbuf + n;
+  a.key_b = h->sortbuf + 2 * n;
+  a.val_b = h->sortbuf + 3 * n;
+  a.rank = h->sortbuf + 4 * n;
   a.hist = h->hist;
-  a.offs = h->offs;
-  a.cursor = h->cursor;
   a.n_live = h->n_live_d;
   a.cap = h->cap;
   a.M = h->M;
-  a.NC = h->G;
+  a.tmax = h->tmax;
   a.k = h->k;
   return a;
 }
 
+// Stable radix sort of every instance's particles by cell key (8 bits per pass), then the
+// gather of all fields into sorted order.
 int enqueue_sort(jm_handle* h) {
   jm::SortArgs a = sort_args(h);
-  const int64_t nh = (int64_t)h->M * h->G;
-  hipLaunchKernelGGL(jm::k_fill_i32, dim3(blocks_for(nh, 256 * 8)), dim3(256), 0, h->stream,
-                     h->hist, nh, 0);
   const dim3 pg(blocks_for(h->cap, 256), h->M);
-  hipLaunchKernelGGL(jm::k_sort_hist, pg, dim3(256), 0, h->stream, a);
-  hipLaunchKernelGGL(jm::k_sort_scan, dim3(h->M), dim3(1024), 0, h->stream, a);
-  hipLaunchKernelGGL(jm::k_sort_scatter, pg, dim3(256), 0, h->stream, a);
-  hipLaunchKernelGGL(jm::k_sort_fix, dim3(blocks_for(h->G, 256), h->M), dim3(256), 0,
-                     h->stream, a);
-  hipLaunchKernelGGL(jm::k_sort_gather, pg, dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(jm::k_sort_keys, pg, dim3(256), 0, h->stream, a);
+  int* kin = a.key_a;
+  int* vin = a.val_a;
+  int* kout = a.key_b;
+  int* vout = a.val_b;
+  for (int shift = 0; shift < h->key_bits; shift += 8) {
+    hipLaunchKernelGGL(jm::k_sort_rank, dim3(h->tmax, h->M), dim3(jm::kSortTile), 0, h->stream,
+                       a, (const int*)kin, shift);
+    hipLaunchKernelGGL(jm::k_sort_scan, dim3(h->M), dim3(1024), 0, h->stream, a);
+    hipLaunchKernelGGL(jm::k_sort_scatter, pg, dim3(256), 0, h->stream, a, (const int*)kin,
+                       (const int*)vin, kout, vout, shift);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  hipLaunchKernelGGL(jm::k_sort_gather, pg, dim3(256), 0, h->stream, a, (const int*)vin);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
@@ -672,7 +681,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   h->G = nx * ny;
   h->cap = cfg->n_particles;
   h->k = make_consts(*cfg, nx, ny);
-  h->tun = {100, 100, 256, 8, 1, 1, 1, 0};
+  h->tun = {100, 100, 256, 4, 1, 1, 1, 0};
   const int M = h->M;
   h->h_param[JM_PARAM_ACTUATION].assign(M, 500.0f);       // mpm_sim.py:146
   h->h_param[JM_PARAM_CONTRACTION].assign(M, 0.20f);      // :148
@@ -700,15 +709,16 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   auto dm = [&](void** p, size_t bytes) {
     return hipMalloc(p, std::max<size_t>(bytes, 256)) == hipSuccess;
   };
-  bool ok = dm((void**)&h->keys, n * 4) && dm((void**)&h->perm, n * 4) &&
-            dm((void**)&h->hist, (size_t)3 * M * h->G * 4) &&
+  h->tmax = (h->cap + jm::kSortTile - 1) / jm::kSortTile;
+  h->key_bits = 8;
+  while ((1LL << h->key_bits) < (int64_t)h->G) h->key_bits += 8;
+  bool ok = dm((void**)&h->sortbuf, n * 5 * 4) &&
+            dm((void**)&h->hist, (size_t)M * 256 * h->tmax * 4) &&
             dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8) &&
             dm((void**)&h->params, (size_t)6 * M * 4) && dm((void**)&h->n_live_d, M * 4) &&
             dm((void**)&h->nchunks_d, M * 4) && dm((void**)&h->stats_d, (size_t)M * 6 * 4) &&
             dm((void**)&h->staging, (size_t)M * 2 * h->G * 4);
   if (!ok) return cleanup(fail(JM_ERR_NOMEM, "hipMalloc(engine buffers)"));
-  h->offs = h->hist + (size_t)M * h->G;
-  h->cursor = h->offs + (size_t)M * h->G;
   if (hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * 4, h->stream) != hipSuccess ||
       hipMemsetAsync(h->t_buf, 0, 8, h->stream) != hipSuccess ||
       hipMemsetAsync(h->stats_d, 0, (size_t)M * 6 * 4, h->stream) != hipSuccess ||

@@ -245,10 +245,35 @@ __device__ __forceinline__ void block_bbox(int& mn0, int& mx0, int& mn1, int& mx
 
 constexpr int kBoxEmpty = 1 << 29;
 
+// P2G accumulates in LDS as 64-bit fixed point with 50 fractional bits: integer LDS
+// atomics run at full rate on gfx950 where fp32 LDS atomics do not, and integer sums are
+// exact and independent of the order of the contributions. A contribution converts with
+// an error below 2^-50 (truncation of c * 2^50). A window gathers at most 4096 particles,
+// so with |c| < 1 every node sum is below 2^12 = 2^62 / 2^50 and fits in int64 (partial
+// sums may wrap; two's-complement addition still yields the exact final sum).
+// Particles with a contribution outside that range (or NaN) scatter straight to the global
+// fp32 grid instead, so non-finite values propagate as they do in fp32.
+constexpr float kFixScale = 1125899906842624.0f;       // 2^50
+constexpr double kFixInv = 1.0 / 1125899906842624.0;   // 2^-50
+constexpr float kFixMax = 1.0f;
+
+__device__ __forceinline__ void add_fixed(unsigned long long* slot, float c) {
+  atomicAdd(slot, (unsigned long long)(long long)(c * kFixScale));
+}
+
+__device__ __forceinline__ float from_fixed(unsigned long long v) {
+  return (float)((double)(long long)v * kFixInv);
+}
+
 template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
 __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
-  constexpr int WMAX = (BLOCK * PPT <= 4096) ? 2048 : 4096;
-  __shared__ float lds[3 * WMAX];
+  // Grid-window capacity (nodes): a chunk of P sorted particles spans about P/512 + 2 rows
+  // of 128 cells, plus the 2-node stencil halo and drift since the last sort.
+  constexpr int WMAX = (BLOCK * PPT <= 1024) ? 1024 : (BLOCK * PPT <= 2048) ? 1536 : 2560;
+  // One LDS buffer: the gather window as float2, then the scatter window as three planes of
+  // 64-bit fixed-point accumulators.
+  __shared__ unsigned long long lds64[3 * WMAX];
+  float* lds = reinterpret_cast<float*>(lds64);
   __shared__ int red[BLOCK / 64][4];
   const int tid = threadIdx.x;
   const PhysConsts& k = a.k;
@@ -409,9 +434,9 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     const bool use_lds = any && ww * wh <= WMAX;
     const int nodes = use_lds ? ww * wh : 0;
     for (int q = tid; q < nodes; q += BLOCK) {
-      lds[q] = 0.f;
-      lds[WMAX + q] = 0.f;
-      lds[2 * WMAX + q] = 0.f;
+      lds64[q] = 0ull;
+      lds64[WMAX + q] = 0ull;
+      lds64[2 * WMAX + q] = 0ull;
     }
     __syncthreads();
     float* gw = a.acc_w + (size_t)m * 3 * G;
@@ -443,6 +468,12 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       float wx[3], wy[3];
       bspline(fx0, wx);
       bspline(fx1, wy);
+      // |contribution| <= |m v| + 1.5 dx (|A0.| + |A1.|) since w <= 1, |dpos| <= 1.5 dx
+      const float bnd = 1.5f * k.dx;
+      const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(o.affine.a00) + fabsf(o.affine.a01)),
+                                    fabsf(mv1) + bnd * (fabsf(o.affine.a10) + fabsf(o.affine.a11))),
+                              o.mass);
+      const bool fixed_ok = use_lds && lim < kFixMax;  // false for NaN
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
 #pragma unroll
@@ -452,11 +483,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
           const float c0 = w * (mv0 + (o.affine.a00 * d0 + o.affine.a01 * d1));
           const float c1 = w * (mv1 + (o.affine.a10 * d0 + o.affine.a11 * d1));
           const float cm = w * o.mass;
-          if (use_lds) {
+          if (fixed_ok) {
             const int q = (b1 - mn1 + jj) * ww + (b0 - mn0 + ii);
-            atomicAdd(&lds[q], c0);
-            atomicAdd(&lds[WMAX + q], c1);
-            atomicAdd(&lds[2 * WMAX + q], cm);
+            add_fixed(&lds64[q], c0);
+            add_fixed(&lds64[WMAX + q], c1);
+            add_fixed(&lds64[2 * WMAX + q], cm);
           } else {
             const int node = (b1 + jj) * nx + (b0 + ii);
             unsafeAtomicAdd(gw + node, c0);
@@ -470,7 +501,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     for (int q = tid; q < nodes; q += BLOCK) {
       const int jj = q / ww, ii = q - jj * ww;
       const int node = (mn1 + jj) * nx + (mn0 + ii);
-      const float c0 = lds[q], c1 = lds[WMAX + q], cm = lds[2 * WMAX + q];
+      const float c0 = from_fixed(lds64[q]), c1 = from_fixed(lds64[WMAX + q]),
+                  cm = from_fixed(lds64[2 * WMAX + q]);
       if (c0 != 0.f) unsafeAtomicAdd(gw + node, c0);
       if (c1 != 0.f) unsafeAtomicAdd(gw + G + node, c1);
       if (cm != 0.f) unsafeAtomicAdd(gw + 2 * G + node, cm);
@@ -504,20 +536,27 @@ __global__ void k_grid_velocity(const float* acc, float* out, int M, int G, Phys
 }
 
 // ---------------------------------------------------------------------------------------
-// Stable counting sort of each instance's particles by P2G base cell.
+// Stable sort of each instance's particles by P2G base cell: an LSD radix sort with 8-bit
+// digits over per-instance tiles of 1024 particles. Each pass ranks a tile's elements per
+// digit in their current order (wave ballots + a per-wave prefix), scans the (digit, tile)
+// histogram of the instance, and scatters; no atomics, so the permutation is deterministic
+// and equals a stable counting sort by cell.
+
+constexpr int kSortTile = 1024;
 
 struct SortArgs {
-  State s;      // live state (sorted in place)
-  State tmp;    // copy of the state before the sort
-  int* keys;    // per particle
-  int* perm;    // per sorted position: index (within segment) of the particle it receives
-  int* hist;    // [m][NC] counts
-  int* offs;    // [m][NC] exclusive offsets
-  int* cursor;  // [m][NC] scatter cursors
+  State s;          // live state (sorted in place by the gather)
+  State tmp;        // copy of the state before the sort
+  int* key_a;       // keys / values, ping-pong between a and b
+  int* val_a;
+  int* key_b;
+  int* val_b;
+  int* rank;        // per element: stable rank within (tile, digit)
+  int* hist;        // [m][256][tmax]: digit counts per tile, then exclusive offsets
   const int* n_live;
   int cap;
   int M;
-  int NC;       // nx * ny
+  int tmax;         // tiles per instance segment (cap / 1024, rounded up)
   PhysConsts k;
 };
 
@@ -531,15 +570,15 @@ __device__ __forceinline__ int cell_key(float2 x, const PhysConsts& k) {
   return b1 * k.nx + b0;
 }
 
-__global__ void __launch_bounds__(256) k_sort_hist(SortArgs a) {
+// Keys of the current order, identity values, and a copy of the state to gather from.
+__global__ void __launch_bounds__(256) k_sort_keys(SortArgs a) {
   const int m = blockIdx.y;
   const int local = blockIdx.x * blockDim.x + threadIdx.x;
   if (local >= a.n_live[m]) return;
   const size_t i = (size_t)m * a.cap + local;
   const float2 x = a.s.x[i];
-  const int key = cell_key(x, a.k);
-  a.keys[i] = key;
-  atomicAdd(&a.hist[(size_t)m * a.NC + key], 1);
+  a.key_a[i] = cell_key(x, a.k);
+  a.val_a[i] = local;
   a.tmp.x[i] = x;
   a.tmp.v[i] = a.s.v[i];
   a.tmp.C[i] = a.s.C[i];
@@ -551,20 +590,57 @@ __global__ void __launch_bounds__(256) k_sort_hist(SortArgs a) {
   if (a.s.glow) a.tmp.glow[i] = a.s.glow[i];
 }
 
-// Exclusive scan of one instance's histogram (one workgroup of 1024 per instance).
+__global__ void __launch_bounds__(kSortTile) k_sort_rank(SortArgs a, const int* keys, int shift) {
+  __shared__ int wcnt[kSortTile / 64][256];
+  const int m = blockIdx.y, t = blockIdx.x, tid = threadIdx.x;
+  const int n = a.n_live[m];
+  if (t * kSortTile >= n) return;
+  for (int q = tid; q < (kSortTile / 64) * 256; q += kSortTile) (&wcnt[0][0])[q] = 0;
+  __syncthreads();
+  const int local = t * kSortTile + tid;
+  const bool valid = local < n;
+  const size_t i = (size_t)m * a.cap + local;
+  const int d = valid ? (keys[i] >> shift) & 255 : 0;
+  unsigned long long same = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1;
+    const unsigned long long bb = __ballot(valid && bit);
+    same &= bit ? bb : ~bb;
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  const int rw = __popcll(same & ((1ull << lane) - 1ull));
+  if (valid && rw == 0) wcnt[wave][d] = __popcll(same);
+  __syncthreads();
+  if (tid < 256) {
+    int run = 0;
+    for (int w = 0; w < kSortTile / 64; ++w) {
+      const int c = wcnt[w][tid];
+      wcnt[w][tid] = run;
+      run += c;
+    }
+    a.hist[((size_t)m * 256 + tid) * a.tmax + t] = run;
+  }
+  __syncthreads();
+  if (valid) a.rank[i] = wcnt[wave][d] + rw;
+}
+
+// Exclusive scan of one instance's (digit, tile) counts in digit-major order.
 __global__ void __launch_bounds__(1024) k_sort_scan(SortArgs a) {
   __shared__ int part[1024];
   const int m = blockIdx.x, tid = threadIdx.x;
-  const int per = (a.NC + 1023) / 1024;
-  const int* h = a.hist + (size_t)m * a.NC;
+  const int T = (a.n_live[m] + kSortTile - 1) / kSortTile;
+  const int E = 256 * T;
+  const int per = (E + 1023) / 1024;
+  int* h = a.hist + (size_t)m * 256 * a.tmax;
   int sum = 0;
   for (int q = 0; q < per; ++q) {
-    const int c = tid * per + q;
-    if (c < a.NC) sum += h[c];
+    const int f = tid * per + q;
+    if (f < E) sum += h[(f / T) * a.tmax + (f % T)];
   }
   part[tid] = sum;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+  for (int off = 1; off < 1024; off <<= 1) {
     const int add = tid >= off ? part[tid - off] : 0;
     __syncthreads();
     part[tid] += add;
@@ -572,50 +648,36 @@ __global__ void __launch_bounds__(1024) k_sort_scan(SortArgs a) {
   }
   int run = part[tid] - sum;
   for (int q = 0; q < per; ++q) {
-    const int c = tid * per + q;
-    if (c < a.NC) {
-      a.offs[(size_t)m * a.NC + c] = run;
-      a.cursor[(size_t)m * a.NC + c] = run;
-      run += h[c];
+    const int f = tid * per + q;
+    if (f < E) {
+      int* e = &h[(f / T) * a.tmax + (f % T)];
+      const int c = *e;
+      *e = run;
+      run += c;
     }
   }
 }
 
-__global__ void __launch_bounds__(256) k_sort_scatter(SortArgs a) {
+__global__ void __launch_bounds__(256) k_sort_scatter(SortArgs a, const int* kin, const int* vin,
+                                                       int* kout, int* vout, int shift) {
   const int m = blockIdx.y;
   const int local = blockIdx.x * blockDim.x + threadIdx.x;
   if (local >= a.n_live[m]) return;
   const size_t i = (size_t)m * a.cap + local;
-  const int pos = atomicAdd(&a.cursor[(size_t)m * a.NC + a.keys[i]], 1);
-  a.perm[(size_t)m * a.cap + pos] = local;
+  const int key = kin[i];
+  const int d = (key >> shift) & 255;
+  const int dst = a.hist[((size_t)m * 256 + d) * a.tmax + local / kSortTile] + a.rank[i];
+  kout[(size_t)m * a.cap + dst] = key;
+  vout[(size_t)m * a.cap + dst] = vin[i];
 }
 
-// Within each cell bucket, order by previous position: makes the sort stable and
-// deterministic regardless of the scatter's atomic order.
-__global__ void __launch_bounds__(256) k_sort_fix(SortArgs a) {
-  const int m = blockIdx.y;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.NC) return;
-  const int n = a.hist[(size_t)m * a.NC + c];
-  if (n < 2) return;
-  int* b = a.perm + (size_t)m * a.cap + a.offs[(size_t)m * a.NC + c];
-  for (int q = 1; q < n; ++q) {
-    const int val = b[q];
-    int r = q - 1;
-    while (r >= 0 && b[r] > val) {
-      b[r + 1] = b[r];
-      --r;
-    }
-    b[r + 1] = val;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_sort_gather(SortArgs a) {
+// Move every field to its sorted position: s[i] = tmp[perm[i]].
+__global__ void __launch_bounds__(256) k_sort_gather(SortArgs a, const int* perm) {
   const int m = blockIdx.y;
   const int local = blockIdx.x * blockDim.x + threadIdx.x;
   if (local >= a.n_live[m]) return;
   const size_t i = (size_t)m * a.cap + local;
-  const size_t src = (size_t)m * a.cap + a.perm[i];
+  const size_t src = (size_t)m * a.cap + perm[i];
   a.s.x[i] = a.tmp.x[src];
   a.s.v[i] = a.tmp.v[src];
   a.s.C[i] = a.tmp.C[src];

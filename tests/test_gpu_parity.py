@@ -81,11 +81,15 @@ def test_one_substep_grid_and_particles(require_gpu, path):
     eng.substep(1)
     orc.run(1)
     gm, gv = eng.read("grid_m"), eng.read("grid_v")
+    # grid mass and momentum per node (momentum after the grid op: velocity x mass)
     assert relmax(gm, orc.grid_m) <= 1e-5, relmax(gm, orc.grid_m)
-    assert relmax(gv, orc.grid_v) <= 1e-5, relmax(gv, orc.grid_v)
-    # momentum (not only velocity) per node
     mom = gv * gm[..., None]
     assert relmax(mom, orc.grid_v * orc.grid_m[..., None]) <= 1e-5
+    # velocity where the node carries mass (a near-massless node's p/m is ill-conditioned;
+    # G2P weighs it by the same vanishing weight)
+    heavy = orc.grid_m >= 1e-3 * orc.grid_m.max()
+    dv = np.abs(gv - orc.grid_v)[heavy]
+    assert dv.max() <= 1e-5 * np.abs(orc.grid_v[heavy]).max(), dv.max()
     x, v, C, F, Jp = (eng.read(n) for n in ("x", "v", "C", "F", "Jp"))
     assert np.max(np.abs(x - orc.x)) <= 2e-7
     assert relmax(v, orc.v) <= 1e-5
@@ -195,7 +199,7 @@ def test_variants_agree(require_gpu):
 
 def test_clock_and_params(require_gpu):
     say("clock accumulation and per-instance params")
-    eng, orc = make_pair(ragged_cases(8000)[:1])
+    eng, orc = make_pair(ragged_cases(12000)[:1])
     eng.set_time(0.25)
     eng.substep(333)
     t = np.float32(0.25)
@@ -231,7 +235,7 @@ def test_empty_and_tiny_instances(require_gpu):
 
 def test_field_write_roundtrip(require_gpu):
     say("field write/read round trip")
-    eng, orc = make_pair(ragged_cases(6000)[:2])
+    eng, orc = make_pair(ragged_cases(12000)[:2])
     rng = np.random.default_rng(9)
     for name in ("x", "v", "C", "F", "Jp", "fiber_dir"):
         shape = eng.field_shape({"x": 0, "v": 1, "C": 2, "F": 3, "Jp": 5, "fiber_dir": 6}[name])
