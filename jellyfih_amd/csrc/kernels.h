@@ -253,16 +253,127 @@ constexpr int kBoxEmpty = 1 << 29;
 // sums may wrap; two's-complement addition still yields the exact final sum).
 // Particles with a contribution outside that range (or NaN) scatter straight to the global
 // fp32 grid instead, so non-finite values propagate as they do in fp32.
-constexpr float kFixScale = 1125899906842624.0f;       // 2^50
 constexpr double kFixInv = 1.0 / 1125899906842624.0;   // 2^-50
+constexpr float kFixPre = 262144.0f;                    // 2^18
 constexpr float kFixMax = 1.0f;
 
-__device__ __forceinline__ void add_fixed(unsigned long long* slot, float c) {
-  atomicAdd(slot, (unsigned long long)(long long)(c * kFixScale));
+// c * 2^50 as a two's-complement int64, split as hi * 2^32 + lo with hi = floor(c * 2^18)
+// and lo = frac(c * 2^18) * 2^32: every step is exact for |c| < 1 (scaling by powers of two,
+// floor, and the subtraction of an integer part), and the final truncation of lo drops
+// less than one unit of 2^-50. Six VALU operations instead of a generic float-to-int64
+// conversion.
+// y = c * 2^18 (the caller pre-scales its per-particle factors, which is exact): the
+// fixed-point value c * 2^50 = hi * 2^32 + lo with hi = floor(y), lo = frac(y) * 2^32.
+__device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float y) {
+  const float fl = floorf(y);
+  const unsigned lo = (unsigned)((y - fl) * 4294967296.0f);
+  const int hi = (int)fl;
+  atomicAdd(slot, ((unsigned long long)(unsigned)hi << 32) | lo);
+}
+
+// Rare path: a particle whose contributions leave the fixed-point range scatters its nine
+// nodes with fp32 global atomics. Node (i, j) receives g_j P0_i + e_j Q0_i,
+// g_j P1_i + e_j Q1_i, g_j M_i.
+struct Axis3 {
+  float v0, v1, v2;
+  __device__ __forceinline__ float operator[](int i) const { return i == 0 ? v0 : (i == 1 ? v1 : v2); }
+};
+
+__device__ __forceinline__ void scatter_global(float* gw, int G, int nx, int b0, int b1, Axis3 wy,
+                                            Axis3 ey, Axis3 P0, Axis3 Q0, Axis3 P1, Axis3 Q1,
+                                            Axis3 Mi) {
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      const int node = (b1 + jj) * nx + (b0 + ii);
+      unsafeAtomicAdd(gw + node, wy[jj] * P0[ii] + ey[jj] * Q0[ii]);
+      unsafeAtomicAdd(gw + G + node, wy[jj] * P1[ii] + ey[jj] * Q1[ii]);
+      unsafeAtomicAdd(gw + 2 * G + node, wy[jj] * Mi[ii]);
+    }
 }
 
 __device__ __forceinline__ float from_fixed(unsigned long long v) {
+#ifdef JM_LDS_F64
+  return (float)__longlong_as_double((long long)v);
+#else
   return (float)((double)(long long)v * kFixInv);
+#endif
+}
+
+// G2P of one in-range particle from a 3x3 block of grid velocities (row stride `ld`):
+// v = sum w g, C = 4 inv_dx sum w g (x) (node - x) (mpm_sim.py:317-327), evaluated in the
+// separable form sum_i wx_i [sum_j wy_j g_ij (x) (d0_i, d1_j)].
+__device__ __forceinline__ void g2p_gather(const float2* g, int ld, float fx0, float fx1,
+                                           const PhysConsts& k, float2& v, float4& C) {
+  float wx[3], wy[3];
+  bspline(fx0, wx);
+  bspline(fx1, wy);
+  float ey[3];
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * ((float)jj - fx1);
+  float v0 = 0.f, v1 = 0.f, c00 = 0.f, c01 = 0.f, c10 = 0.f, c11 = 0.f;
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii) {
+    float s0 = 0.f, s1 = 0.f, t0 = 0.f, t1 = 0.f;
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      const float2 n = g[jj * ld + ii];
+      s0 += wy[jj] * n.x;
+      s1 += wy[jj] * n.y;
+      t0 += ey[jj] * n.x;
+      t1 += ey[jj] * n.y;
+    }
+    const float d0 = (float)ii - fx0;
+    v0 += wx[ii] * s0;
+    v1 += wx[ii] * s1;
+    c00 += (wx[ii] * d0) * s0;
+    c10 += (wx[ii] * d0) * s1;
+    c01 += wx[ii] * t0;
+    c11 += wx[ii] * t1;
+  }
+  v = make_float2(v0, v1);
+  C = make_float4(k.apic_scale * c00, k.apic_scale * c01, k.apic_scale * c10,
+                  k.apic_scale * c11);
+}
+
+// G2P + advection of one particle straight from the global grid (grid op applied per
+// node); the slow path for chunks whose window does not fit in LDS. Reads and writes the
+// particle's x, v, C in memory.
+__device__ __forceinline__ void g2p_global(const float* gp, int G, const PhysConsts* kp,
+                                        float2* xs, float2* vs, float4* Cs, int gi) {
+  const PhysConsts& k = *kp;
+  float2 x = xs[gi], v = vs[gi];
+  float4 C = Cs[gi];
+  int b0, b1;
+  float fx0, fx1;
+  base_frac(x.x, k.inv_dx, b0, fx0);
+  base_frac(x.y, k.inv_dx, b1, fx1);
+  if (in_range(b0, b1, k)) {
+    float wx[3], wy[3];
+    bspline(fx0, wx);
+    bspline(fx1, wy);
+    float v0 = 0.f, v1 = 0.f, c00 = 0.f, c01 = 0.f, c10 = 0.f, c11 = 0.f;
+    for (int ii = 0; ii < 3; ++ii)
+      for (int jj = 0; jj < 3; ++jj) {
+        const int i = b0 + ii, j = b1 + jj, node = j * k.nx + i;
+        const float2 n = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
+        const float w = wx[ii] * wy[jj], d0 = (float)ii - fx0, d1 = (float)jj - fx1;
+        v0 += w * n.x;
+        v1 += w * n.y;
+        c00 += w * n.x * d0;
+        c01 += w * n.x * d1;
+        c10 += w * n.y * d0;
+        c11 += w * n.y * d1;
+      }
+    v = make_float2(v0, v1);
+    C = make_float4(k.apic_scale * c00, k.apic_scale * c01, k.apic_scale * c10,
+                    k.apic_scale * c11);
+  }
+  advect(x, v, k);
+  xs[gi] = x;
+  vs[gi] = v;
+  Cs[gi] = C;
 }
 
 template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
@@ -350,50 +461,47 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       }
     }
     __syncthreads();
+    if (use_lds) {
 #pragma unroll
-    for (int p = 0; p < PPT; ++p) {
-      if (mt[p] < 0) continue;
-      const int li = p * BLOCK + tid;
-      int b0, b1;
-      float fx0, fx1;
-      base_frac(x[p].x, k.inv_dx, b0, fx0);
-      base_frac(x[p].y, k.inv_dx, b1, fx1);
-      if (in_range(b0, b1, k)) {
-        float wx[3], wy[3];
-        bspline(fx0, wx);
-        bspline(fx1, wy);
-        float nv0 = 0.f, nv1 = 0.f;
-        Mat2 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-#pragma unroll
-          for (int jj = 0; jj < 3; ++jj) {
-            float2 g;
-            if (use_lds) {
-              g = lv[(b1 - mn1 + jj) * ww + (b0 - mn0 + ii)];
-            } else {
-              const int i = b0 + ii, j = b1 + jj, node = j * nx + i;
-              g = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
-            }
-            const float d0 = (float)ii - fx0, d1 = (float)jj - fx1;
-            const float w = wx[ii] * wy[jj];
-            nv0 += w * g.x;
-            nv1 += w * g.y;
-            const float ws = k.apic_scale * w;
-            c.a00 += ws * (g.x * d0);
-            c.a01 += ws * (g.x * d1);
-            c.a10 += ws * (g.y * d0);
-            c.a11 += ws * (g.y * d1);
-          }
+      for (int p = 0; p < PPT; ++p) {
+        if (mt[p] < 0) continue;
+        const int li = p * BLOCK + tid;
+        int b0, b1;
+        float fx0, fx1;
+        base_frac(x[p].x, k.inv_dx, b0, fx0);
+        base_frac(x[p].y, k.inv_dx, b1, fx1);
+        if (in_range(b0, b1, k)) {
+          g2p_gather(lv + (b1 - mn1) * ww + (b0 - mn0), ww, fx0, fx1, k, v[p], C[p]);
+        } else {
+          v[p] = a.s.v[start + li];
+          C[p] = a.s.C[start + li];
         }
-        v[p] = make_float2(nv0, nv1);
-        C[p] = as_f4(c);
-      } else {
-        v[p] = a.s.v[start + li];
-        C[p] = a.s.C[start + li];
+        advect(x[p], v[p], k);
       }
-      advect(x[p], v[p], k);
-      if (a.glow) {
+    } else {
+      // Window too large for LDS (particles far from their sorted cells): gather from the
+      // global grid, one particle at a time, through memory (not unrolled: rare path).
+#pragma unroll 1
+      for (int p = 0; p < PPT; ++p) {
+        const int li = p * BLOCK + tid;
+        if (li >= count) continue;
+        g2p_global(gp, G, &a.k, a.s.x, a.s.v, a.s.C, start + li);
+      }
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int li = p * BLOCK + tid;
+        if (li < count) {
+          x[p] = a.s.x[start + li];
+          v[p] = a.s.v[start + li];
+          C[p] = a.s.C[start + li];
+        }
+      }
+    }
+    if (a.glow) {
+#pragma unroll 1
+      for (int p = 0; p < PPT; ++p) {
+        const int li = p * BLOCK + tid;
+        if (li >= count) continue;
         const int gi = start + li;
         a.s.glow[gi] = fmaxf(a.s.glow[gi] * 0.9f, fabsf(1.0f - a.s.Jp[gi]));
       }
@@ -474,27 +582,39 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
                                     fabsf(mv1) + bnd * (fabsf(o.affine.a10) + fabsf(o.affine.a11))),
                               o.mass);
       const bool fixed_ok = use_lds && lim < kFixMax;  // false for NaN
+      // Separable form of w_ij (m v + A dpos): with a_i = wx_i, b_i = wx_i d0_i, g_j = wy_j,
+      // e_j = wy_j d1_j, the momentum is g_j (a_i m v + b_i A.0) + e_j (a_i A.1) and the
+      // mass g_j (a_i m).
+      float P0[3], Q0[3], P1[3], Q1[3], Mi[3], ey[3];
+      const float sc = fixed_ok ? kFixPre : 1.0f;
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
+        const float ai = wx[ii] * sc;
+        const float bi = ai * (((float)ii - fx0) * k.dx);
+        P0[ii] = ai * mv0 + bi * o.affine.a00;
+        Q0[ii] = ai * o.affine.a01;
+        P1[ii] = ai * mv1 + bi * o.affine.a10;
+        Q1[ii] = ai * o.affine.a11;
+        Mi[ii] = ai * o.mass;
+      }
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * (((float)jj - fx1) * k.dx);
+      if (fixed_ok) {
+        const int q0 = (b1 - mn1) * ww + (b0 - mn0);
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
-          const float d0 = ((float)ii - fx0) * k.dx, d1 = ((float)jj - fx1) * k.dx;
-          const float w = wx[ii] * wy[jj];
-          const float c0 = w * (mv0 + (o.affine.a00 * d0 + o.affine.a01 * d1));
-          const float c1 = w * (mv1 + (o.affine.a10 * d0 + o.affine.a11 * d1));
-          const float cm = w * o.mass;
-          if (fixed_ok) {
-            const int q = (b1 - mn1 + jj) * ww + (b0 - mn0 + ii);
-            add_fixed(&lds64[q], c0);
-            add_fixed(&lds64[WMAX + q], c1);
-            add_fixed(&lds64[2 * WMAX + q], cm);
-          } else {
-            const int node = (b1 + jj) * nx + (b0 + ii);
-            unsafeAtomicAdd(gw + node, c0);
-            unsafeAtomicAdd(gw + G + node, c1);
-            unsafeAtomicAdd(gw + 2 * G + node, cm);
+#pragma unroll
+          for (int ii = 0; ii < 3; ++ii) {
+            const int q = q0 + jj * ww + ii;
+            add_fixed_scaled(&lds64[q], wy[jj] * P0[ii] + ey[jj] * Q0[ii]);
+            add_fixed_scaled(&lds64[WMAX + q], wy[jj] * P1[ii] + ey[jj] * Q1[ii]);
+            add_fixed_scaled(&lds64[2 * WMAX + q], wy[jj] * Mi[ii]);
           }
         }
+      } else {
+        scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
+                       {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
+                       {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
       }
     }
     __syncthreads();

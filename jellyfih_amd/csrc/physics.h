@@ -145,6 +145,26 @@ __device__ __forceinline__ P2GOut p2g_update(int mat, const Mat2& F0, const Mat2
     mu = 0.0f;
     la = k.water_la;
   }
+  // Water (mu = 0) needs only J = s0*s1 from the SVD (the deviatoric term is 0*(F - R)F^T,
+  // and F is reset to sqrt(J) I). When det F > 1e-6 * |F|_F both singular values exceed
+  // the 1e-6 floor (s_min = det/s_max >= det/|F|_F), so no clamp applies and s0*s1 = det F:
+  // the SVD is skipped. Degenerate F (det near 0 or negative) takes the full path below.
+  if (mat == 0) {
+    const float det = F.a00 * F.a11 - F.a01 * F.a10;
+    const float fro2 = F.a00 * F.a00 + F.a01 * F.a01 + F.a10 * F.a10 + F.a11 * F.a11;
+    if (det > 0.0f && det * det > 1e-12f * fro2) {
+      const float J = det;
+      const float r = sqrtf(J);
+      const float dg = (la * J) * (J - 1.0f);
+      o.q0 = o.q1 = 1.0f;
+      o.jp_changes = false;
+      o.F = {r, 0.0f, 0.0f, r};
+      o.affine = {k.stress_scale * dg + mass * C.a00, mass * C.a01, mass * C.a10,
+                  k.stress_scale * dg + mass * C.a11};
+      o.mass = mass;
+      return o;
+    }
+  }
   // mpm_sim.py:206-218
   Mat2 U, V;
   float s0, s1;
@@ -156,9 +176,15 @@ __device__ __forceinline__ P2GOut p2g_update(int mat, const Mat2& F0, const Mat2
     n0 = fminf(fmaxf(s0, k.pay_lo), k.pay_hi);
     n1 = fminf(fmaxf(s1, k.pay_lo), k.pay_hi);
   }
-  o.q0 = s0 / n0;
-  o.q1 = s1 / n1;
-  o.jp_changes = (mat == 2) || !(o.q0 == 1.0f && o.q1 == 1.0f);
+  if (mat == 2) {
+    o.q0 = s0 / n0;
+    o.q1 = s1 / n1;
+    o.jp_changes = true;
+  } else {  // n == s: the ratio is exactly 1, or NaN when s = +inf (inf/inf)
+    o.q0 = s0 < INFINITY ? 1.0f : __int_as_float(0x7fc00000);
+    o.q1 = s1 < INFINITY ? 1.0f : __int_as_float(0x7fc00000);
+    o.jp_changes = !(s0 < INFINITY && s1 < INFINITY);
+  }
   float J = n0 * n1;
   // mpm_sim.py:220-223
   if (mat == 0) {
