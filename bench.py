@@ -107,6 +107,8 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=80000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); instances are "
@@ -123,6 +125,14 @@ def main() -> int:
     pos, mat, fib, genomes, infos = build_workload(M, N, range(M))
     live = int((mat >= 0).sum())
     log(f"live particles {live} (slots {M * N})")
+
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
 
     eng = Engine(M, N)
     load_engine(eng, pos, mat, fib, genomes)
