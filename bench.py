@@ -14,7 +14,11 @@ Also reported on the same line:
 * cpu_baseline: the oracle (CPU restatement, fp32, one instance per OpenMP thread) on a
   bounded sample of the same workload on this host;
 * plain_value: the same workload stepped by the plain path (one launch per stage per
-  instance, global atomics, unsorted).
+  instance, global atomics, unsorted);
+* diff: one differentiable episode of one seeded reference tank (seed 0, ~62K live
+  particles, 2000 substeps, checkpoint every 100): forward through the autograd blocks,
+  forward + backward of the payload CoM y, and the fused (non-differentiable) forward of
+  the same episode for comparison.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -98,6 +102,57 @@ def cpu_baseline(pos, mat, fib, genomes, budget_s: float = 12.0) -> dict:
                       f"{steps} substeps, oracle fp32, one instance per OpenMP thread"}
 
 
+def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
+    """Timing of a differentiable episode (wall clock around GPU-synchronised work)."""
+    import torch
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.diff import DiffTank
+    from jellyfih_amd.engine import Engine
+
+    g = phenotype.seeded_genome(0)
+    pos, mat, fib, info = phenotype.fill_tank(g, 80000, 128)
+    con = float(np.clip(g[9], 0.05, 0.60))
+    freq = float(np.clip(g[10], 0.5, 2.0))
+    tank, st0 = DiffTank.from_tanks(pos[None], mat[None], fib[None])
+    dev = tank.device
+
+    def episode(grad: bool) -> float:
+        st = st0.clone().requires_grad_(grad)
+        leaf = lambda v: torch.full((1,), v, device=dev, requires_grad=grad)
+        amp_scale = torch.ones(tank.P, device=dev, requires_grad=grad)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.set_grad_enabled(grad):
+            out = tank.run(st, amp_scale, leaf(500.0), leaf(con), leaf(freq), leaf(700.0), 2.5,
+                           0.0, steps, checkpoint)
+            y = tank.payload_com_y(out).sum()
+            if grad:
+                y.backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    episode(True)                                  # warm-up (allocations, code objects)
+    fwd = min(episode(False) for _ in range(2))
+    fwd_bwd = min(episode(True) for _ in range(2))
+    eng = Engine(1, 80000)
+    eng.load_particles(0, pos, mat, fib)
+    eng.set_param("contraction", con)
+    eng.set_param("freq", freq)
+    eng.substep(steps)
+    eng.sync()
+    eng.set_time(0.0)
+    eng.load_particles(0, pos, mat, fib)
+    t0 = time.perf_counter()
+    eng.substep(steps)
+    eng.sync()
+    fused = time.perf_counter() - t0
+    eng.close()
+    return {"workload": f"1 reference tank (seed 0, {tank.P} live), {steps} substeps, "
+                        f"checkpoint every {checkpoint}, payload CoM y objective",
+            "forward_ms": fwd * 1e3, "fwd_bwd_ms": fwd_bwd * 1e3, "fused_forward_ms": fused * 1e3,
+            "fwd_bwd_over_forward": fwd_bwd / fwd, "fwd_bwd_over_fused_forward": fwd_bwd / fused}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +162,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=80000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -189,6 +245,12 @@ def main() -> int:
         pe.close()
         log(f"plain {plain_value:.3e}")
 
+    diff = None
+    if not args.no_diff:
+        log("differentiable episode (forward, forward + backward)")
+        diff = diff_episode()
+        log(f"diff {diff}")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
@@ -220,6 +282,7 @@ def main() -> int:
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "diff": diff,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -33,7 +33,8 @@ EXPORTS = (
     "jm_load_particles", "jm_set_instance_param", "jm_get_instance_param", "jm_set_time",
     "jm_get_time", "jm_substep", "jm_compute_stats", "jm_run_batch_headless", "jm_read_field",
     "jm_write_field", "jm_live_count", "jm_read_device_order", "jm_sort_now", "jm_timed_block", "jm_sync",
-    "jm_stream", "jm_last_error",
+    "jm_stream", "jm_last_error", "jm_diff_create", "jm_diff_destroy", "jm_fwd_block",
+    "jm_bwd_block",
 )
 
 
@@ -65,6 +66,16 @@ class JmTuning(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class JmDiffInputs(ctypes.Structure):
+    _fields_ = [
+        ("inst", ctypes.c_void_p),
+        ("mat", ctypes.c_void_p),
+        ("fiber", ctypes.c_void_p),
+        ("amp_scale", ctypes.c_void_p),
+        ("params", ctypes.c_void_p),
+    ]
 
 
 class JmError(RuntimeError):
@@ -108,6 +119,13 @@ def load() -> ctypes.CDLL:
         "jm_sort_now": [H],
         "jm_timed_block": [H, ctypes.c_int32, f32p, f32p],
         "jm_sync": [H],
+        "jm_diff_create": [P(JmConfig), ctypes.c_int64, P(H)],
+        "jm_diff_destroy": [H],
+        "jm_fwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
+                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
+        "jm_bwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -19,6 +19,7 @@
 
 #include "../../include/jellymx.h"
 #include "kernels.h"
+#include "diff_kernels.h"
 
 namespace {
 
@@ -1085,6 +1086,9 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
   return JM_OK;
 }
 
+// -----------------------------------------------------------------------------------------
+// Differentiable path (no reference counterpart: the reference has no autodiff).
+
 int jm_sync(jm_handle* h) {
   JM_TRY(check_handle(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1092,5 +1096,163 @@ int jm_sync(jm_handle* h) {
 }
 
 void* jm_stream(jm_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"
+
+struct jm_diff {
+  int M = 0, nx = 0, ny = 0, G = 0;
+  int64_t P = 0;
+  jmd::Consts<float> k{};
+  float* grid = nullptr;   // M * 5 * G
+  float* agrid = nullptr;  // M * 5 * G
+  float* sbuf = nullptr;   // 2 * P * 12 (forward ping-pong)
+  float* gbuf = nullptr;   // 3 * P * 12 (adjoint ping-pong + partial)
+  ~jm_diff() {
+    for (void* p : {(void*)grid, (void*)agrid, (void*)sbuf, (void*)gbuf})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+jmd::Consts<float> diff_consts(const jm::PhysConsts& c) {
+  jmd::Consts<float> k;
+  k.nx = c.nx;
+  k.ny = c.ny;
+  k.damp_x = c.damp_x;
+  k.damp_y = c.damp_y;
+  k.dt = c.dt;
+  k.dx = c.dx;
+  k.inv_dx = c.inv_dx;
+  k.p_mass = c.p_mass;
+  k.gravity_dt = c.gravity_dt;
+  k.stress_scale = c.stress_scale;
+  k.apic_scale = c.apic_scale;
+  k.mu_base = c.mu_base;
+  k.la_base = c.la_base;
+  k.mu_pay = c.mu_pay;
+  k.la_pay = c.la_pay;
+  k.water_la = c.water_la;
+  k.act_freq = c.act_freq;
+  k.x_lo = c.x_lo;
+  k.x_hi = c.x_hi;
+  k.y_hi = c.y_hi;
+  k.sig_floor = c.sig_floor;
+  k.pay_lo = c.pay_lo;
+  k.pay_hi = c.pay_hi;
+  k.pi = c.pi;
+  k.nu_jelly = 0.3f;  // mpm_sim.py:32
+  return k;
+}
+
+int diff_check(jm_diff* d, const jm_diff_inputs* in) {
+  if (!d) return fail(JM_ERR_ARG, "null diff handle");
+  if (!in || !in->inst || !in->mat || !in->fiber || !in->amp_scale || !in->params)
+    return fail(JM_ERR_ARG, "jm_diff_inputs has a null pointer");
+  return JM_OK;
+}
+
+jmd::DiffInputs diff_inputs(const jm_diff_inputs* in) {
+  return {in->inst, in->mat, in->fiber, in->amp_scale, in->params};
+}
+
+}  // namespace
+
+extern "C" {
+
+int jm_diff_create(const jm_config* cfg, int64_t n_particles, jm_diff** out) {
+  if (!cfg || !out) return fail(JM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->abi_version != JM_ABI_VERSION) return fail(JM_ERR_ARG, "abi_version mismatch");
+  if (cfg->n_instances <= 0 || cfg->quality <= 0 || n_particles <= 0 ||
+      n_particles > (int64_t)INT32_MAX / 16)
+    return fail(JM_ERR_ARG, "bad sizes");
+  const int nx = 128 * cfg->quality;
+  const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
+  if (ny < 8) return fail(JM_ERR_ARG, "n_grid_y too small");
+  jm_diff* d = new jm_diff();
+  d->M = cfg->n_instances;
+  d->nx = nx;
+  d->ny = ny;
+  d->G = nx * ny;
+  d->P = n_particles;
+  d->k = diff_consts(make_consts(*cfg, nx, ny));
+  const size_t gbytes = (size_t)d->M * 5 * d->G * sizeof(float);
+  const size_t sbytes = (size_t)d->P * 12 * sizeof(float);
+  if (hipMalloc(&d->grid, gbytes) != hipSuccess || hipMalloc(&d->agrid, gbytes) != hipSuccess ||
+      hipMalloc(&d->sbuf, 2 * sbytes) != hipSuccess ||
+      hipMalloc(&d->gbuf, 3 * sbytes) != hipSuccess) {
+    delete d;
+    return fail(JM_ERR_NOMEM, "hipMalloc(diff buffers)");
+  }
+  *out = d;
+  return JM_OK;
+}
+
+int jm_diff_destroy(jm_diff* d) {
+  delete d;
+  return JM_OK;
+}
+
+int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const float* state_in, float t0,
+                    int32_t nsteps, float* out, int32_t store_all, void* stream) {
+  JM_TRY(diff_check(d, in));
+  if (!state_in || !out || nsteps < 1) return fail(JM_ERR_ARG, "bad state or step count");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const jmd::DiffInputs din = diff_inputs(in);
+  const int P = (int)d->P, G = d->G, M = d->M;
+  const size_t S = (size_t)P * 12;
+  const dim3 pg(blocks_for(P, 256)), ng(blocks_for(G, 256), M);
+  const int64_t gn = (int64_t)M * 5 * G;
+  float t = t0;
+  const float* cur = state_in;
+  for (int s = 0; s < nsteps; ++s) {
+    float* nxt = store_all ? out + (size_t)s * S : (s == nsteps - 1 ? out : d->sbuf + (s % 2) * S);
+    hipLaunchKernelGGL(jmd::k_d_fill, dim3(blocks_for(gn, 256 * 8)), dim3(256), 0, st, d->grid, gn);
+    hipLaunchKernelGGL(jmd::k_d_p2g, pg, dim3(256), 0, st, din, cur, nxt, d->grid, P, G, t, d->k);
+    hipLaunchKernelGGL(jmd::k_d_gridop, ng, dim3(256), 0, st, d->grid, M, G, d->k);
+    hipLaunchKernelGGL(jmd::k_d_g2p, pg, dim3(256), 0, st, din, cur, nxt, d->grid, P, G, d->k);
+    cur = nxt;
+    t = t + d->k.dt;  // mpm_sim.py:340, fp32
+  }
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const float* traj, float t0,
+                     int32_t nsteps, const float* grad_out, float* grad_in, float* grad_amp,
+                     float* grad_pp, void* stream) {
+  JM_TRY(diff_check(d, in));
+  if (!traj || !grad_out || !grad_in || !grad_amp || !grad_pp || nsteps < 1)
+    return fail(JM_ERR_ARG, "bad backward arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const jmd::DiffInputs din = diff_inputs(in);
+  const int P = (int)d->P, G = d->G, M = d->M;
+  const size_t S = (size_t)P * 12;
+  const dim3 pg(blocks_for(P, 256)), ng(blocks_for(G, 256), M);
+  const int64_t gn = (int64_t)M * 5 * G;
+  std::vector<float> times(nsteps + 1);
+  times[0] = t0;
+  for (int s = 0; s < nsteps; ++s) times[s + 1] = times[s] + d->k.dt;
+  float* mid = d->gbuf + 2 * S;
+  const float* bout = grad_out;
+  for (int s = nsteps - 1; s >= 0; --s) {
+    const float* cur = traj + (size_t)s * S;
+    float* bin = s == 0 ? grad_in : d->gbuf + (s % 2) * S;
+    hipLaunchKernelGGL(jmd::k_d_fill, dim3(blocks_for(gn, 256 * 8)), dim3(256), 0, st, d->grid, gn);
+    hipLaunchKernelGGL(jmd::k_d_p2g, pg, dim3(256), 0, st, din, cur, (float*)nullptr, d->grid, P,
+                       G, times[s], d->k);
+    hipLaunchKernelGGL(jmd::k_d_gridop, ng, dim3(256), 0, st, d->grid, M, G, d->k);
+    hipLaunchKernelGGL(jmd::k_d_fill, dim3(blocks_for(gn, 256 * 8)), dim3(256), 0, st, d->agrid, gn);
+    hipLaunchKernelGGL(jmd::k_d_g2p_bwd, pg, dim3(256), 0, st, din, cur, bout, mid, d->grid,
+                       d->agrid, P, G, d->k);
+    hipLaunchKernelGGL(jmd::k_d_gridop_bwd, ng, dim3(256), 0, st, d->grid, d->agrid, M, G, d->k);
+    hipLaunchKernelGGL(jmd::k_d_p2g_bwd, pg, dim3(256), 0, st, din, cur, bout, mid, bin,
+                       d->agrid, P, G, times[s], d->k, grad_amp, grad_pp);
+    bout = bin;
+  }
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
 
 }  // extern "C"

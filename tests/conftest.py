@@ -18,6 +18,13 @@ def golden():
     return np.load(os.path.join(ROOT, "tests", "golden", "phenotype_golden.npz"))
 
 
+@pytest.fixture(scope="session")
+def golden_grad():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "grad_fd_golden.json")) as f:
+        return json.load(f)
+
+
 def gpu_available() -> bool:
     try:
         import torch

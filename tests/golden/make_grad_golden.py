@@ -1,0 +1,61 @@
+"""Writes tests/golden/grad_fd_golden.json: fp64 central differences of the payload CoM y
+of one seeded reference tank (seed 0, ~62K live particles) after 2000 substeps, along the
+seeded directions of tests/harness.py (body initial velocities, actuation amplitude,
+contraction fraction, frequency, E_jelly, mixtures) from the oracle, and along a
+per-particle actuation multiplier direction from the fp64 harness (the multiplier has no
+counterpart in the reference), at eps = 1e-4, 1e-5, 1e-6.
+
+The GPU test recomputes the same directions from their seeds and compares the HIP
+adjoint's directional derivatives with these values. Takes a few minutes on 8 cores:
+
+    python tests/golden/make_grad_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from tests.harness import (amp_scale_direction, gradient_directions,  # noqa: E402
+                           harness_amp_scale_fd, load_harness, oracle_payload_fd,
+                           reference_tank_case)
+
+STEPS = 2000
+EPS = [1e-4, 1e-5, 1e-6]
+OUT = os.path.join(ROOT, "tests", "golden", "grad_fd_golden.json")
+
+
+def main() -> None:
+    t0 = time.time()
+    cs, pos, mat, fib = reference_tank_case(STEPS)
+    kinds, dvs, dps = gradient_directions(cs.mat)
+    nthreads = len(os.sched_getaffinity(0))
+    fd = oracle_payload_fd(cs, pos, mat, fib, dvs, dps, EPS, nthreads=nthreads)
+    print(f"oracle FD done ({time.time() - t0:.0f} s)", flush=True)
+    L = load_harness()
+    fd_amp = harness_amp_scale_fd(L, cs, amp_scale_direction(cs.mat), EPS)
+    print(f"harness FD done ({time.time() - t0:.0f} s)", flush=True)
+    rec = {
+        "case": "reference tank seed 0, 80000 slots, 2000 substeps from t=0, objective = "
+                "payload CoM y after the run",
+        "steps": STEPS,
+        "n_live": int(cs.P),
+        "params": [float(v) for v in cs.params[0]],
+        "eps": EPS,
+        "direction_seed": 2024,
+        "amp_scale_seed": 11,
+        "kinds": kinds,
+        "oracle_fd": fd.tolist(),
+        "harness_amp_scale_fd": fd_amp.tolist(),
+    }
+    with open(OUT, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(f"wrote {OUT}")
+
+
+if __name__ == "__main__":
+    main()
