@@ -65,6 +65,8 @@ class DiffTank:
                  device: str | torch.device = "cuda"):
         load()
         dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
         self.device = dev
         self.inst = torch.as_tensor(inst, dtype=torch.int32).to(dev).contiguous()
         self.mat = torch.as_tensor(mat, dtype=torch.int32).to(dev).contiguous()

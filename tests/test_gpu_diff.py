@@ -40,22 +40,24 @@ GROUPS = {"x": slice(0, 2), "v": slice(2, 4), "C": slice(4, 8), "F": slice(8, 12
 
 
 def test_diff_forward_vs_oracle(require_gpu):
-    say("diff forward: 2x160 perturbed particles, 12 substeps, vs oracle f32/f64")
-    cs = Case(0, steps=12).round_f32()
-    cs.amp_scale[:] = 1.0
+    """One seeded reference tank (~62K live particles), 200 substeps from rest: the final
+    state, and the last state of a stored trajectory (a second run, other atomics order),
+    are as accurate as an fp32 build of the reference substep."""
+    say("diff forward: reference tank seed0, 200 substeps, vs oracle f32/f64")
+    cs, _, _, _ = reference_tank_case(200)
     tank, st, amp, par = _gpu(cs)
     out = tank.forward_raw(st, amp, par, cs.t0, cs.steps).cpu().numpy().astype(np.float64)
     traj = tank.forward_raw(st, amp, par, cs.t0, cs.steps, store_all=True).cpu().numpy()
+    assert traj.shape == (cs.steps, cs.P, 12)
     o64 = cs.oracle_forward()
     o32 = cs.oracle_forward(dtype=np.float32).astype(np.float64)
     for g, sl in GROUPS.items():
-        e_gpu = np.abs(out[:, sl] - o64[:, sl]).max()
         e_32 = np.abs(o32[:, sl] - o64[:, sl]).max()
-        say(f"{g}: |gpu-f64|={e_gpu:.3e} |oracle32-f64|={e_32:.3e}")
-        assert e_gpu <= 2.0 * e_32 + 1e-6, g
-    # the stored trajectory ends where the plain forward ends (atomics order aside)
-    assert traj.shape == (cs.steps, cs.P, 12)
-    assert np.abs(traj[-1] - out).max() <= 1e-5
+        for name, res in (("final", out), ("trajectory", traj[-1])):
+            e_gpu = np.abs(res[:, sl] - o64[:, sl]).max()
+            say(f"{g} {name}: |gpu-f64|={e_gpu:.3e} |oracle32-f64|={e_32:.3e}")
+            assert e_gpu <= 2.0 * e_32 + 1e-6, (g, name)
+    assert np.abs(out[:, 0:2] - o64[:, 0:2]).max() <= 1e-4
 
 
 def test_diff_adjoint_vs_fp64_adjoint(require_gpu):
@@ -133,11 +135,26 @@ def test_reference_tank_gradients_vs_fd(require_gpu, golden_grad):
 
 
 def test_gradients_independent_of_checkpoint_interval(require_gpu):
-    say("reference tank seed0, 300 substeps: checkpoint 300 vs 100 vs 64")
+    """Recomputing from checkpoints every 100 or 64 substeps gives the gradients of one
+    300-substep block, up to the run-to-run noise of fp32 atomics order (measured by
+    repeating the 300-substep run). The full state gradient is reported, not asserted:
+    water resting against the walls sits on the wall clamps' kink (zero grid velocity), so
+    the branch its adjoint takes follows that noise."""
+    say("reference tank seed0, 300 substeps: checkpoint 300 (twice) vs 100 vs 64")
     cs, _, _, _ = reference_tank_case(300)
-    res = [_gpu_gradients(cs, checkpoint=k) for k in (300, 100, 64)]
-    for r in res[1:]:
-        for a, b in zip(r[1:], res[0][1:]):
-            err = np.linalg.norm(a - b) / np.linalg.norm(b)
-            say(f"checkpoint agreement: {err:.2e}")
-            assert err <= 1e-4, err
+    runs = {k: _gpu_gradients(cs, checkpoint=k) for k in (300, 100, 64)}
+    again = _gpu_gradients(cs, checkpoint=300)
+    body = cs.mat > 0
+
+    def groups(r):
+        return {"state": r[1], "body v": r[1][body, 2:4], "amp_scale": r[2], "params": r[3]}
+
+    base = groups(runs[300])
+    floor = {n: np.linalg.norm(v - base[n]) / np.linalg.norm(base[n])
+             for n, v in groups(again).items()}
+    for k in (100, 64):
+        for n, v in groups(runs[k]).items():
+            err = np.linalg.norm(v - base[n]) / np.linalg.norm(base[n])
+            say(f"checkpoint {k} vs 300, {n}: {err:.2e} (repeat of 300: {floor[n]:.2e})")
+            if n != "state":
+                assert err <= max(10 * floor[n], 1e-5) and err <= 1e-3, (k, n, err)
