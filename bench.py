@@ -16,9 +16,10 @@ Also reported on the same line:
 * plain_value: the same workload stepped by the plain path (one launch per stage per
   instance, global atomics, unsorted);
 * diff: one differentiable episode of one seeded reference tank (seed 0, ~62K live
-  particles, 2000 substeps, checkpoint every 100): forward through the autograd blocks,
-  forward + backward of the payload CoM y, and the fused (non-differentiable) forward of
-  the same episode for comparison.
+  particles, 2000 substeps, checkpoint every 100), in fp32 and fp64: forward through the
+  autograd blocks, forward + backward of the payload CoM y with recomputation from the
+  checkpoints, the same keeping every block's stored trajectory (no recomputation), and the
+  fused (non-differentiable, fp32) forward of the same episode for comparison.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -103,7 +104,8 @@ def cpu_baseline(pos, mat, fib, genomes, budget_s: float = 12.0) -> dict:
 
 
 def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
-    """Timing of a differentiable episode (wall clock around GPU-synchronised work)."""
+    """Timing of a differentiable episode (wall clock around GPU-synchronised work), in
+    fp32 and fp64, beside the fused fp32 forward of the same episode."""
     import torch
     from jellyfih_amd import phenotype
     from jellyfih_amd.diff import DiffTank
@@ -113,27 +115,7 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
     pos, mat, fib, info = phenotype.fill_tank(g, 80000, 128)
     con = float(np.clip(g[9], 0.05, 0.60))
     freq = float(np.clip(g[10], 0.5, 2.0))
-    tank, st0 = DiffTank.from_tanks(pos[None], mat[None], fib[None])
-    dev = tank.device
 
-    def episode(grad: bool) -> float:
-        st = st0.clone().requires_grad_(grad)
-        leaf = lambda v: torch.full((1,), v, device=dev, requires_grad=grad)
-        amp_scale = torch.ones(tank.P, device=dev, requires_grad=grad)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        with torch.set_grad_enabled(grad):
-            out = tank.run(st, amp_scale, leaf(500.0), leaf(con), leaf(freq), leaf(700.0), 2.5,
-                           0.0, steps, checkpoint)
-            y = tank.payload_com_y(out).sum()
-            if grad:
-                y.backward()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    episode(True)                                  # warm-up (allocations, code objects)
-    fwd = min(episode(False) for _ in range(2))
-    fwd_bwd = min(episode(True) for _ in range(2))
     eng = Engine(1, 80000)
     eng.load_particles(0, pos, mat, fib)
     eng.set_param("contraction", con)
@@ -147,10 +129,97 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
     eng.sync()
     fused = time.perf_counter() - t0
     eng.close()
-    return {"workload": f"1 reference tank (seed 0, {tank.P} live), {steps} substeps, "
-                        f"checkpoint every {checkpoint}, payload CoM y objective",
-            "forward_ms": fwd * 1e3, "fwd_bwd_ms": fwd_bwd * 1e3, "fused_forward_ms": fused * 1e3,
-            "fwd_bwd_over_forward": fwd_bwd / fwd, "fwd_bwd_over_fused_forward": fwd_bwd / fused}
+    res = {"fused_forward_ms": fused * 1e3}
+
+    for dtype in (torch.float32, torch.float64):
+        tank, st0 = DiffTank.from_tanks(pos[None], mat[None], fib[None], dtype=dtype)
+        dev = tank.device
+
+        def episode(grad: bool, recompute: bool = True) -> float:
+            st = st0.clone().requires_grad_(grad)
+            leaf = lambda v: torch.full((1,), v, dtype=dtype, device=dev, requires_grad=grad)
+            amp_scale = torch.ones(tank.P, dtype=dtype, device=dev, requires_grad=grad)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            with torch.set_grad_enabled(grad):
+                out = tank.run(st, amp_scale, leaf(500.0), leaf(con), leaf(freq), leaf(700.0),
+                               2.5, 0.0, steps, checkpoint, recompute)
+                y = tank.payload_com_y(out).sum()
+                if grad:
+                    y.backward()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        episode(True)                              # warm-up (allocations, code objects)
+        fwd = min(episode(False) for _ in range(2))
+        fb = min(episode(True) for _ in range(2))
+        episode(True, recompute=False)
+        fb_stored = min(episode(True, recompute=False) for _ in range(2))
+        res["workload"] = (f"1 reference tank (seed 0, {tank.P} live), {steps} substeps, "
+                           f"checkpoint every {checkpoint}, payload CoM y objective")
+        res["fp32" if dtype == torch.float32 else "fp64"] = {
+            "forward_ms": fwd * 1e3, "fwd_bwd_ms": fb * 1e3, "fwd_bwd_stored_ms": fb_stored * 1e3,
+            "fwd_bwd_over_forward": fb / fwd, "fwd_bwd_stored_over_forward": fb_stored / fwd,
+            "fwd_bwd_over_fused_forward": fb / fused,
+            "fwd_bwd_stored_over_fused_forward": fb_stored / fused}
+        tank.close()
+    return res
+
+
+def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
+                 checkpoint: int = 200) -> dict:
+    """Batched forward + backward at the tall-tank shape: M seeded genomes in 128x256 tanks
+    of height 2 (spawn (0.5, 0.60), evolve.py:33-36,136), per-instance gradients of the
+    payload CoM y w.r.t. actuation amplitude, contraction and frequency in one adjoint
+    pass; beside the fused forward of the same batch."""
+    import torch
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.diff import DiffTank
+    from jellyfih_amd.engine import Engine
+
+    pos, mat, fib, genomes, infos = phenotype.population(
+        range(M), slots, 128, domain_height=2.0, spawn_offset=phenotype.TALL_SPAWN)
+    con = np.clip(genomes[:, 9], 0.05, 0.60)
+    freq = np.clip(genomes[:, 10], 0.5, 2.0)
+    live = int((mat >= 0).sum())
+    log(f"batched diff: {M} tall tanks, {live} live particles, {steps} substeps")
+    eng = Engine(M, slots, n_grid_y=256, domain_height=2.0)
+    load_engine(eng, pos, mat, fib, genomes)
+    eng.substep(200)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.substep(steps)
+    eng.sync()
+    fused = time.perf_counter() - t0
+    eng.close()
+    tank, st0 = DiffTank.from_tanks(pos, mat, fib, n_grid_y=256, domain_height=2.0)
+    dev = tank.device
+
+    def episode(grad: bool) -> float:
+        st = st0.clone()
+        vec = lambda a: torch.tensor(np.asarray(a, np.float32), device=dev).requires_grad_(grad)
+        amp, c, f = vec(np.full(M, 500.0)), vec(con), vec(freq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.set_grad_enabled(grad):
+            out = tank.run(st, None, amp, c, f, 700.0, 2.5, 0.0, steps, checkpoint)
+            y = tank.payload_com_y(out).sum()
+            if grad:
+                y.backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    fwd = episode(False)
+    log(f"batched diff forward {fwd:.2f} s")
+    fb = episode(True)
+    log(f"batched diff forward + backward {fb:.2f} s")
+    tank.close()
+    return {"workload": f"{M} tall tanks 128x256 (H=2, {slots} slots, {live} live), {steps} "
+                        f"substeps, checkpoint every {checkpoint}, fp32",
+            "live_particles": live, "fused_forward_s": fused, "forward_s": fwd,
+            "fwd_bwd_s": fb, "fwd_bwd_over_forward": fb / fwd,
+            "fwd_bwd_over_fused_forward": fb / fused,
+            "fused_rate": live * steps / fused}
 
 
 def main() -> int:
@@ -163,6 +232,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--diff-batched", action="store_true",
+                    help="also time the batched tall-tank forward + backward (~1 minute)")
+    ap.add_argument("--only-diff", action="store_true",
+                    help="run only the differentiable legs (no throughput line fields)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -176,6 +249,12 @@ def main() -> int:
     from jellyfih_amd.engine import Engine
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.only_diff:
+        out = {"diff": diff_episode()}
+        if args.diff_batched:
+            out["diff"]["batched"] = diff_batched()
+        print(json.dumps(out), flush=True)
+        return 0
     M, N = args.instances, args.slots
     log(f"building workload: {M} genomes x {N} slots")
     pos, mat, fib, genomes, infos = build_workload(M, N, range(M))
@@ -250,6 +329,9 @@ def main() -> int:
         log("differentiable episode (forward, forward + backward)")
         diff = diff_episode()
         log(f"diff {diff}")
+    if args.diff_batched:
+        diff = dict(diff or {}, batched=diff_batched())
+        log(f"diff batched {diff['batched']}")
 
     cpu = None
     if not args.no_cpu:

@@ -164,38 +164,45 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms);
 
 /* ---- Differentiable path -----------------------------------------------------------
  * No counterpart in the reference (it has no autodiff). A batched substep and its adjoint
- * over caller-owned device buffers (e.g. torch tensors), P particles in any order:
- *   state (P,12) f32 per particle: x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11;
- *   inputs: inst (P,) i32 instance index, mat (P,) i32, fiber (P,2) f32, amp_scale (P,) f32
- *   (per-particle multiplier on instance_actuation), params (M,5) f32 = instance_actuation,
+ * over caller-owned device buffers (e.g. torch tensors), P particles in any order, in fp32
+ * or fp64 (chosen at jm_diff_create; every scalar buffer below is of that type):
+ *   state (P,12) per particle: x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11;
+ *   inputs: inst (P,) int32 instance index, mat (P,) int32, fiber (P,2), amp_scale (P,)
+ *   (per-particle multiplier on instance_actuation), params (M,5) = instance_actuation,
  *   instance_act_contraction, instance_freq, instance_payload_density, E_jelly.
  * All pointers are device pointers; work is queued on `stream` (hipStream_t, may be NULL). */
 typedef struct jm_diff jm_diff;
 typedef struct jm_diff_inputs {
   const int32_t* inst;
   const int32_t* mat;
-  const float* fiber;
-  const float* amp_scale;
-  const float* params;
+  const void* fiber;
+  const void* amp_scale;
+  const void* params;
 } jm_diff_inputs;
 
-/* Grid buffers for cfg's grid and n_instances, and scratch for n_particles particles. */
-int jm_diff_create(const jm_config* cfg, int64_t n_particles, jm_diff** out);
+/* Grid buffers for cfg's grid and n_instances, and scratch for n_particles particles;
+ * scalar_bits = 32 (fp32, the reference's precision) or 64 (fp64). */
+int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bits,
+                   jm_diff** out);
 int jm_diff_destroy(jm_diff* d);
 
-/* nsteps substeps from state_in at time t0 (the fp32 clock advances by dt per substep).
- * store_all = 0: out (P,12) is the final state; 1: out (nsteps,P,12) holds every state
- * after each substep (the trajectory the backward pass reads). */
-int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const float* state_in, float t0,
-                    int32_t nsteps, float* out, int32_t store_all, void* stream);
+/* nsteps substeps from state_in at time t0 (the clock advances by dt per substep in the
+ * handle's precision; mpm_sim.py:336-340 does it in fp32). grid_out = NULL: out (P,12) is
+ * the final state. Otherwise out (nsteps,P,12) receives the state after every substep and
+ * grid_out (nsteps,M,3,nx*ny) the momentum x, momentum y and mass planes of every
+ * substep's grid: the trajectory the backward pass reads. */
+int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const void* state_in, double t0,
+                 int32_t nsteps, void* out, void* grid_out, void* stream);
 
-/* Reverse pass: traj (nsteps+1,P,12) holds the states at substeps 0..nsteps; grad_out
- * (P,12) is the adjoint of the last. Writes grad_in (P,12), the adjoint of the first, and
+/* Reverse pass: traj (nsteps+1,P,12) holds the states at substeps 0..nsteps and grids the
+ * nsteps grids, as jm_fwd_block stores them (state_in, then out, from t0); grad_out (P,12)
+ * is the adjoint of the last state. Writes grad_in (P,12), the adjoint of the first, and
  * accumulates grad_amp (P,) and grad_pp (P,4) = per-particle shares of the adjoints of
- * (E_jelly, instance_actuation, contraction, freq), to be summed per instance. */
-int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const float* traj, float t0,
-                     int32_t nsteps, const float* grad_out, float* grad_in, float* grad_amp,
-                     float* grad_pp, void* stream);
+ * (E_jelly, instance_actuation, contraction, freq), to be summed per instance. grad_out
+ * and grad_in may not alias. */
+int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const void* grids,
+                 double t0, int32_t nsteps, const void* grad_out, void* grad_in,
+                 void* grad_amp, void* grad_pp, void* stream);
 
 /* ti.sync(): wait for all queued work. */
 int jm_sync(jm_handle* h);

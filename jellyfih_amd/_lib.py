@@ -119,13 +119,13 @@ def load() -> ctypes.CDLL:
         "jm_sort_now": [H],
         "jm_timed_block": [H, ctypes.c_int32, f32p, f32p],
         "jm_sync": [H],
-        "jm_diff_create": [P(JmConfig), ctypes.c_int64, P(H)],
+        "jm_diff_create": [P(JmConfig), ctypes.c_int64, ctypes.c_int32, P(H)],
         "jm_diff_destroy": [H],
-        "jm_fwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
-                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
-        "jm_bwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
-                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                             ctypes.c_void_p],
+        "jm_fwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+        "jm_bwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                         ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                         ctypes.c_void_p, ctypes.c_void_p],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -8,10 +8,12 @@
 // two parameters promoted to runtime values (jelly Young's modulus per instance, and a
 // per-particle multiplier on the actuation amplitude). The adjoint differentiates:
 //   * P2G (mpm_sim.py:183-271): B-spline weights in x, F <- (I + dt C) F, the constitutive
-//     update (water: J = det F, F <- sqrt(J) I; jelly/muscle: fixed corotated stress with the
-//     polar rotation R; payload: the spectral clamp of the singular values, with taichi
-//     `_svd2d`'s |S01| < 1e-5 diagonal branch) and the APIC affine scatter;
-//   * the grid op (mpm_sim.py:274-309) with the wall clamps and damping as fixed masks;
+//     update (J = product of the clamped singular values of taichi `_svd2d`, including its
+//     |S01| < 1e-5 diagonal branch; water: F <- sqrt(J) I; jelly/muscle: fixed corotated
+//     stress with the polar rotation R; payload: the spectral clamp of the singular values)
+//     and the APIC affine scatter;
+//   * the grid op (mpm_sim.py:274-309) with the wall clamps and damping as fixed masks,
+//     evaluated per stencil node from momentum and mass inside G2P (it is node-local);
 //   * G2P and advection (mpm_sim.py:311-333) with the position clamp as a fixed mask.
 // Non-smooth points (clamps, the range skip, the phase branch) take the derivative of the
 // branch the forward took.
@@ -420,13 +422,10 @@ JMD_HD void gridop_backward(T px, T py, T m, int i, int j, const Consts<T>& k, T
 
 // ---------------------------------------------------------------------------------------
 // Per-particle transfer loops. State layout per particle (12 values): x0 x1 v0 v1 C00 C01
-// C10 C11 F00 F01 F10 F11. Grids of one instance: planes of nx*ny values, node (i, j) at
-// j*nx + i. `Add` performs (atomic) grid accumulation: add(ptr, value).
-
-struct Particle {
-  int mat;
-  float fib0, fib1;
-};
+// C10 C11 F00 F01 F10 F11. Grids of one instance: three planes (momentum x, momentum y,
+// mass) of nx*ny values, node (i, j) at j*nx + i. Scatters hand per-node contributions to
+// an `emit(i, j, c0, c1, c2)` callable, which accumulates them (LDS tile, global atomics,
+// or plain adds on the host).
 
 template <typename T>
 struct PState {
@@ -445,13 +444,15 @@ JMD_HD void store_state(T* s, const PState<T>& p) {
   s[8] = p.F.a; s[9] = p.F.b; s[10] = p.F.c; s[11] = p.F.d;
 }
 
-// P2G of one particle (mpm_sim.py:183-271): updates F in `p` and scatters momentum and mass.
-template <typename T, typename Add>
-JMD_HD void p2g_scatter(PState<T>& p, int mat, T d0, T d1, T beta, T mu_j, T la_j, T pdens,
-                        const Consts<T>& k, T* gp0, T* gp1, T* gm, Add add) {
+// P2G of one particle (mpm_sim.py:183-271): updates F in `p` and hands the momentum and
+// mass contributions of each stencil node to `emit(i, j, momentum_x, momentum_y, mass)`.
+// Returns false (F unchanged, nothing emitted) outside the transfer range.
+template <typename T, typename Emit>
+JMD_HD bool p2g_particle(PState<T>& p, int mat, T d0, T d1, T beta, T mu_j, T la_j, T pdens,
+                         const Consts<T>& k, Emit&& emit) {
   int b0, b1;
   T fx0, fx1;
-  if (!base_cell(p.x0, p.x1, k, b0, b1, fx0, fx1)) return;
+  if (!base_cell(p.x0, p.x1, k, b0, b1, fx0, fx1)) return false;
   T wx[3], wy[3], dwx[3], dwy[3];
   weights(fx0, wx, dwx);
   weights(fx1, wy, dwy);
@@ -463,17 +464,25 @@ JMD_HD void p2g_scatter(PState<T>& p, int mat, T d0, T d1, T beta, T mu_j, T la_
     for (int jj = 0; jj < 3; ++jj) {
       T dp0 = (T(ii) - fx0) * k.dx, dp1 = (T(jj) - fx1) * k.dx;
       T w = wx[ii] * wy[jj];
-      int node = (b1 + jj) * k.nx + (b0 + ii);
-      add(gp0 + node, w * (mass * p.v0 + (A.a * dp0 + A.b * dp1)));
-      add(gp1 + node, w * (mass * p.v1 + (A.c * dp0 + A.d * dp1)));
-      add(gm + node, w * mass);
+      emit(b0 + ii, b1 + jj, w * (mass * p.v0 + (A.a * dp0 + A.b * dp1)),
+           w * (mass * p.v1 + (A.c * dp0 + A.d * dp1)), w * mass);
     }
+  return true;
 }
 
-// G2P + gravity + advection + clamp of one particle (mpm_sim.py:311-333) from grid
-// velocities g0, g1 (after the grid op).
+// Velocity of node (i, j) after the grid op, from its momentum and mass planes.
 template <typename T>
-JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* g0, const T* g1) {
+JMD_HD void node_velocity(const T* gp0, const T* gp1, const T* gm, int i, int j,
+                          const Consts<T>& k, T& a0, T& a1) {
+  const int node = j * k.nx + i;
+  gridop_forward(gp0[node], gp1[node], gm[node], i, j, k, a0, a1);
+}
+
+// G2P + gravity + advection + clamp of one particle (mpm_sim.py:311-333), the grid op
+// (mpm_sim.py:274-309) evaluated at each stencil node from momentum gp0, gp1 and mass gm.
+template <typename T>
+JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* gp0, const T* gp1,
+                         const T* gm) {
   int b0, b1;
   T fx0, fx1;
   if (base_cell(p.x0, p.x1, k, b0, b1, fx0, fx1)) {
@@ -484,10 +493,10 @@ JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* g0, const T*
     Mat<T> C = {T(0), T(0), T(0), T(0)};
     for (int ii = 0; ii < 3; ++ii)
       for (int jj = 0; jj < 3; ++jj) {
-        int node = (b1 + jj) * k.nx + (b0 + ii);
+        T a0, a1;
+        node_velocity(gp0, gp1, gm, b0 + ii, b1 + jj, k, a0, a1);
         T w = wx[ii] * wy[jj];
         T e0 = T(ii) - fx0, e1 = T(jj) - fx1;
-        T a0 = g0[node], a1 = g1[node];
         v0 += w * a0;
         v1 += w * a1;
         T ws = k.apic_scale * w;
@@ -509,17 +518,20 @@ JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* g0, const T*
 
 // Adjoint of g2p_particle. `s` is the particle's state before the substep (its x sets the
 // weights; v, C pass through when out of range), `b_out` the adjoint of the state after.
-// Accumulates the x adjoint and the pass-through v, C adjoints into `b_in`, and scatters the
-// grid-velocity adjoint into gb0, gb1.
-template <typename T, typename Add>
+// Accumulates the x adjoint and the pass-through v, C adjoints into `b_in`. The adjoint of
+// each stencil node's velocity, taken through that node's grid op (linear in it), goes to
+// `emit(i, j, momentum_x_adj, momentum_y_adj, mass_adj)`.
+template <typename T, typename Emit>
 JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
-                         const T* g0, const T* g1, T* gb0, T* gb1, PState<T>& b_in, Add add) {
+                         const T* gp0, const T* gp1, const T* gm, PState<T>& b_in,
+                         Emit&& emit) {
   // forward to the clamp, to know which components were clamped
   T v0n = s.v0, v1n = s.v1;
   int b0, b1;
   T fx0, fx1;
   const bool inr = base_cell(s.x0, s.x1, k, b0, b1, fx0, fx1);
   T wx[3], wy[3], dwx[3], dwy[3];
+  T a0s[9], a1s[9];
   if (inr) {
     weights(fx0, wx, dwx);
     weights(fx1, wy, dwy);
@@ -527,10 +539,10 @@ JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Const
     v1n = T(0);
     for (int ii = 0; ii < 3; ++ii)
       for (int jj = 0; jj < 3; ++jj) {
-        int node = (b1 + jj) * k.nx + (b0 + ii);
+        node_velocity(gp0, gp1, gm, b0 + ii, b1 + jj, k, a0s[3 * ii + jj], a1s[3 * ii + jj]);
         T w = wx[ii] * wy[jj];
-        v0n += w * g0[node];
-        v1n += w * g1[node];
+        v0n += w * a0s[3 * ii + jj];
+        v1n += w * a1s[3 * ii + jj];
       }
   }
   v1n = v1n - k.gravity_dt;
@@ -551,14 +563,16 @@ JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Const
   T f0 = T(0), f1 = T(0);  // d L / d fx
   for (int ii = 0; ii < 3; ++ii)
     for (int jj = 0; jj < 3; ++jj) {
-      int node = (b1 + jj) * k.nx + (b0 + ii);
+      const int i = b0 + ii, j = b1 + jj, node = j * k.nx + i;
       T w = wx[ii] * wy[jj];
       T e0 = T(ii) - fx0, e1 = T(jj) - fx1;
-      T a0 = g0[node], a1 = g1[node];
+      T a0 = a0s[3 * ii + jj], a1 = a1s[3 * ii + jj];
       // L_node = vb . g + apic g^T Cb e
       T ce0 = Cb.a * e0 + Cb.b * e1, ce1 = Cb.c * e0 + Cb.d * e1;  // Cb e
-      add(gb0 + node, w * vb0 + k.apic_scale * w * ce0);
-      add(gb1 + node, w * vb1 + k.apic_scale * w * ce1);
+      T q0, q1, qm;
+      gridop_backward(gp0[node], gp1[node], gm[node], i, j, k, w * vb0 + k.apic_scale * w * ce0,
+                      w * vb1 + k.apic_scale * w * ce1, q0, q1, qm);
+      emit(i, j, q0, q1, qm);
       T lw = vb0 * a0 + vb1 * a1 + k.apic_scale * (a0 * ce0 + a1 * ce1);
       f0 += dwx[ii] * wy[jj] * lw;
       f1 += wx[ii] * dwy[jj] * lw;
@@ -570,7 +584,7 @@ JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Const
   b_in.x1 += k.inv_dx * f1;
 }
 
-// Adjoint of p2g_scatter (gather from the momentum/mass adjoints pb0, pb1, mb). `b_out`
+// Adjoint of p2g_particle (gather from the momentum/mass adjoints pb0, pb1, mb). `b_out`
 // holds the adjoint of the new F (b_out.F); results accumulate into `b_in` and the
 // parameter adjoints (mu_j, la_j, beta).
 template <typename T>

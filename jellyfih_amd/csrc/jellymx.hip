@@ -1086,9 +1086,6 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
   return JM_OK;
 }
 
-// -----------------------------------------------------------------------------------------
-// Differentiable path (no reference counterpart: the reference has no autodiff).
-
 int jm_sync(jm_handle* h) {
   JM_TRY(check_handle(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1099,49 +1096,62 @@ void* jm_stream(jm_handle* h) { return h ? (void*)h->stream : nullptr; }
 
 }  // extern "C"
 
+// -----------------------------------------------------------------------------------------
+// Differentiable path (no reference counterpart: the reference has no autodiff).
+
 struct jm_diff {
   int M = 0, nx = 0, ny = 0, G = 0;
   int64_t P = 0;
-  jmd::Consts<float> k{};
-  float* grid = nullptr;   // M * 5 * G
-  float* agrid = nullptr;  // M * 5 * G
-  float* sbuf = nullptr;   // 2 * P * 12 (forward ping-pong)
-  float* gbuf = nullptr;   // 3 * P * 12 (adjoint ping-pong + partial)
+  int bits = 32;                 // scalar type of states, grids and parameters
+  jmd::Consts<float> kf{};
+  jmd::Consts<double> kd{};
+  void* grid = nullptr;   // 2 * M * 3 * G scalars (forward ping-pong)
+  void* agrid = nullptr;  // 2 * M * 3 * G scalars (adjoint ping-pong)
+  void* sbuf = nullptr;   // 2 * P * 12 scalars (forward ping-pong)
+  void* gbuf = nullptr;   // 3 * P * 12 scalars (adjoint ping-pong + partial)
   ~jm_diff() {
-    for (void* p : {(void*)grid, (void*)agrid, (void*)sbuf, (void*)gbuf})
+    for (void* p : {grid, agrid, sbuf, gbuf})
       if (p) (void)hipFree(p);
   }
 };
 
 namespace {
 
-jmd::Consts<float> diff_consts(const jm::PhysConsts& c) {
-  jmd::Consts<float> k;
-  k.nx = c.nx;
-  k.ny = c.ny;
-  k.damp_x = c.damp_x;
-  k.damp_y = c.damp_y;
-  k.dt = c.dt;
-  k.dx = c.dx;
-  k.inv_dx = c.inv_dx;
-  k.p_mass = c.p_mass;
-  k.gravity_dt = c.gravity_dt;
-  k.stress_scale = c.stress_scale;
-  k.apic_scale = c.apic_scale;
-  k.mu_base = c.mu_base;
-  k.la_base = c.la_base;
-  k.mu_pay = c.mu_pay;
-  k.la_pay = c.la_pay;
-  k.water_la = c.water_la;
-  k.act_freq = c.act_freq;
-  k.x_lo = c.x_lo;
-  k.x_hi = c.x_hi;
-  k.y_hi = c.y_hi;
-  k.sig_floor = c.sig_floor;
-  k.pay_lo = c.pay_lo;
-  k.pay_hi = c.pay_hi;
-  k.pi = c.pi;
-  k.nu_jelly = 0.3f;  // mpm_sim.py:32
+// mpm_sim.py:12-58 evaluated in double like the reference's Python, rounded to T (for
+// float, the same values as make_consts).
+template <typename T>
+jmd::Consts<T> diff_consts(const jm_config& c, int nx, int ny) {
+  const double n_grid = 128.0 * c.quality;
+  const double dx = 1.0 / n_grid, inv_dx = n_grid;
+  const double dt = 2e-5 / c.quality;
+  const double p_vol = (dx * 0.5) * (dx * 0.5);
+  const double E_base = 0.1e4, nu_base = 0.2, E_pay = 2.0e5, nu_pay = 0.2;
+  jmd::Consts<T> k;
+  k.nx = nx;
+  k.ny = ny;
+  k.damp_x = nx / 20;
+  k.damp_y = ny / 20;
+  k.dt = T(dt);
+  k.dx = T(dx);
+  k.inv_dx = T(inv_dx);
+  k.p_mass = T(p_vol);
+  k.gravity_dt = T(dt * 10.0);
+  k.stress_scale = T(-dt * p_vol * 4 * inv_dx * inv_dx);
+  k.apic_scale = T(4 * inv_dx);
+  k.mu_base = T(E_base / (2 * (1 + nu_base)));
+  k.la_base = T(E_base * nu_base / ((1 + nu_base) * (1 - 2 * nu_base)));
+  k.mu_pay = T(E_pay / (2 * (1 + nu_pay)));
+  k.la_pay = T(E_pay * nu_pay / ((1 + nu_pay) * (1 - 2 * nu_pay)));
+  k.water_la = T(100000.0);
+  k.act_freq = T(1.0);
+  k.x_lo = T(0.001);
+  k.x_hi = T(0.999);
+  k.y_hi = T(c.domain_height - 0.001);
+  k.sig_floor = T(1e-6);
+  k.pay_lo = T(1 - 5e-3);
+  k.pay_hi = T(1 + 5e-3);
+  k.pi = T(3.14159265);
+  k.nu_jelly = T(0.3);  // mpm_sim.py:32
   return k;
 }
 
@@ -1152,21 +1162,100 @@ int diff_check(jm_diff* d, const jm_diff_inputs* in) {
   return JM_OK;
 }
 
-jmd::DiffInputs diff_inputs(const jm_diff_inputs* in) {
-  return {in->inst, in->mat, in->fiber, in->amp_scale, in->params};
+template <typename T>
+jmd::DiffInputs<T> diff_inputs(const jm_diff_inputs* in) {
+  return {in->inst, in->mat, static_cast<const T*>(in->fiber),
+          static_cast<const T*>(in->amp_scale), static_cast<const T*>(in->params)};
+}
+
+template <typename T>
+const jmd::Consts<T>& consts_of(const jm_diff* d);
+template <>
+const jmd::Consts<float>& consts_of<float>(const jm_diff* d) { return d->kf; }
+template <>
+const jmd::Consts<double>& consts_of<double>(const jm_diff* d) { return d->kd; }
+
+template <typename T>
+int fwd_block(jm_diff* d, const jm_diff_inputs* in, const T* state_in, double t0_in,
+              int nsteps, T* out, T* grid_out, hipStream_t st) {
+  const jmd::DiffInputs<T> din = diff_inputs<T>(in);
+  const jmd::Consts<T>& k = consts_of<T>(d);
+  const int P = (int)d->P, G = d->G;
+  const size_t S = (size_t)P * 12;
+  const int64_t MG = (int64_t)d->M * 3 * G;  // one substep's grids
+  const dim3 pg(blocks_for(P, jmd::kDiffBlock)), tb(jmd::kDiffBlock);
+  T* grid = static_cast<T*>(d->grid);
+  T* sbuf = static_cast<T*>(d->sbuf);
+  const bool store = grid_out != nullptr;
+  if (store)
+    hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG * nsteps, 256 * 8)), dim3(256), 0,
+                       st, grid_out, MG * nsteps);
+  else
+    hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st, grid,
+                       MG);
+  T t = T(t0_in);
+  const T* cur = state_in;
+  for (int s = 0; s < nsteps; ++s) {
+    T* nxt = store ? out + (size_t)s * S : (s == nsteps - 1 ? out : sbuf + (size_t)(s % 2) * S);
+    T* g = store ? grid_out + (size_t)s * MG : grid + (size_t)(s % 2) * MG;
+    T* zero = (!store && s + 1 < nsteps) ? grid + (size_t)((s + 1) % 2) * MG : nullptr;
+    hipLaunchKernelGGL(jmd::k_d_p2g<T>, pg, tb, 0, st, din, cur, nxt, g, P, G, t, k);
+    hipLaunchKernelGGL(jmd::k_d_g2p<T>, pg, tb, 0, st, din, cur, nxt, (const T*)g, zero, MG, P,
+                       G, k);
+    cur = nxt;
+    t = t + k.dt;  // mpm_sim.py:340 (fp32 in the reference)
+  }
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+template <typename T>
+int bwd_block(jm_diff* d, const jm_diff_inputs* in, const T* traj, const T* grids,
+              double t0_in, int nsteps, const T* grad_out, T* grad_in, T* grad_amp, T* grad_pp,
+              hipStream_t st) {
+  const jmd::DiffInputs<T> din = diff_inputs<T>(in);
+  const jmd::Consts<T>& k = consts_of<T>(d);
+  const int P = (int)d->P, G = d->G;
+  const size_t S = (size_t)P * 12;
+  const int64_t MG = (int64_t)d->M * 3 * G;
+  const dim3 pg(blocks_for(P, jmd::kDiffBlock)), tb(jmd::kDiffBlock);
+  T* agrid = static_cast<T*>(d->agrid);
+  T* gbuf = static_cast<T*>(d->gbuf);
+  std::vector<T> times(nsteps + 1);
+  times[0] = T(t0_in);
+  for (int s = 0; s < nsteps; ++s) times[s + 1] = times[s] + k.dt;
+  T* mid = gbuf + 2 * S;
+  hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st,
+                     agrid + (size_t)((nsteps - 1) % 2) * MG, MG);
+  const T* bout = grad_out;
+  for (int s = nsteps - 1; s >= 0; --s) {
+    const T* cur = traj + (size_t)s * S;
+    T* a = agrid + (size_t)(s % 2) * MG;
+    T* bin = s == 0 ? grad_in : gbuf + (size_t)(s % 2) * S;
+    T* zero = s > 0 ? agrid + (size_t)((s + 1) % 2) * MG : nullptr;
+    hipLaunchKernelGGL(jmd::k_d_g2p_bwd<T>, pg, tb, 0, st, din, cur, bout, mid,
+                       grids + (size_t)s * MG, a, P, G, k);
+    hipLaunchKernelGGL(jmd::k_d_p2g_bwd<T>, pg, tb, 0, st, din, cur, bout, (const T*)mid, bin,
+                       (const T*)a, zero, MG, P, G, times[s], k, grad_amp, grad_pp);
+    bout = bin;
+  }
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
 }
 
 }  // namespace
 
 extern "C" {
 
-int jm_diff_create(const jm_config* cfg, int64_t n_particles, jm_diff** out) {
+int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bits,
+                   jm_diff** out) {
   if (!cfg || !out) return fail(JM_ERR_ARG, "null argument");
   *out = nullptr;
   if (cfg->abi_version != JM_ABI_VERSION) return fail(JM_ERR_ARG, "abi_version mismatch");
   if (cfg->n_instances <= 0 || cfg->quality <= 0 || n_particles <= 0 ||
       n_particles > (int64_t)INT32_MAX / 16)
     return fail(JM_ERR_ARG, "bad sizes");
+  if (scalar_bits != 32 && scalar_bits != 64) return fail(JM_ERR_ARG, "scalar_bits: 32 or 64");
   const int nx = 128 * cfg->quality;
   const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
   if (ny < 8) return fail(JM_ERR_ARG, "n_grid_y too small");
@@ -1176,9 +1265,12 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, jm_diff** out) {
   d->ny = ny;
   d->G = nx * ny;
   d->P = n_particles;
-  d->k = diff_consts(make_consts(*cfg, nx, ny));
-  const size_t gbytes = (size_t)d->M * 5 * d->G * sizeof(float);
-  const size_t sbytes = (size_t)d->P * 12 * sizeof(float);
+  d->bits = scalar_bits;
+  d->kf = diff_consts<float>(*cfg, nx, ny);
+  d->kd = diff_consts<double>(*cfg, nx, ny);
+  const size_t es = scalar_bits / 8;
+  const size_t gbytes = 2 * (size_t)d->M * 3 * d->G * es;
+  const size_t sbytes = (size_t)d->P * 12 * es;
   if (hipMalloc(&d->grid, gbytes) != hipSuccess || hipMalloc(&d->agrid, gbytes) != hipSuccess ||
       hipMalloc(&d->sbuf, 2 * sbytes) != hipSuccess ||
       hipMalloc(&d->gbuf, 3 * sbytes) != hipSuccess) {
@@ -1194,65 +1286,34 @@ int jm_diff_destroy(jm_diff* d) {
   return JM_OK;
 }
 
-int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const float* state_in, float t0,
-                    int32_t nsteps, float* out, int32_t store_all, void* stream) {
+int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const void* state_in, double t0,
+                 int32_t nsteps, void* out, void* grid_out, void* stream) {
   JM_TRY(diff_check(d, in));
   if (!state_in || !out || nsteps < 1) return fail(JM_ERR_ARG, "bad state or step count");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const jmd::DiffInputs din = diff_inputs(in);
-  const int P = (int)d->P, G = d->G, M = d->M;
-  const size_t S = (size_t)P * 12;
-  const dim3 pg(blocks_for(P, 256)), ng(blocks_for(G, 256), M);
-  const int64_t gn = (int64_t)M * 5 * G;
-  float t = t0;
-  const float* cur = state_in;
-  for (int s = 0; s < nsteps; ++s) {
-    float* nxt = store_all ? out + (size_t)s * S : (s == nsteps - 1 ? out : d->sbuf + (s % 2) * S);
-    hipLaunchKernelGGL(jmd::k_d_fill, dim3(blocks_for(gn, 256 * 8)), dim3(256), 0, st, d->grid, gn);
-    hipLaunchKernelGGL(jmd::k_d_p2g, pg, dim3(256), 0, st, din, cur, nxt, d->grid, P, G, t, d->k);
-    hipLaunchKernelGGL(jmd::k_d_gridop, ng, dim3(256), 0, st, d->grid, M, G, d->k);
-    hipLaunchKernelGGL(jmd::k_d_g2p, pg, dim3(256), 0, st, din, cur, nxt, d->grid, P, G, d->k);
-    cur = nxt;
-    t = t + d->k.dt;  // mpm_sim.py:340, fp32
-  }
-  HIP_TRY(hipGetLastError());
-  return JM_OK;
+  if (d->bits == 64)
+    return fwd_block<double>(d, in, static_cast<const double*>(state_in), t0, nsteps,
+                             static_cast<double*>(out), static_cast<double*>(grid_out), st);
+  return fwd_block<float>(d, in, static_cast<const float*>(state_in), t0, nsteps,
+                          static_cast<float*>(out), static_cast<float*>(grid_out), st);
 }
 
-int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const float* traj, float t0,
-                     int32_t nsteps, const float* grad_out, float* grad_in, float* grad_amp,
-                     float* grad_pp, void* stream) {
+int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const void* grids,
+                 double t0, int32_t nsteps, const void* grad_out, void* grad_in, void* grad_amp,
+                 void* grad_pp, void* stream) {
   JM_TRY(diff_check(d, in));
-  if (!traj || !grad_out || !grad_in || !grad_amp || !grad_pp || nsteps < 1)
+  if (!traj || !grids || !grad_out || !grad_in || !grad_amp || !grad_pp || nsteps < 1)
     return fail(JM_ERR_ARG, "bad backward arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const jmd::DiffInputs din = diff_inputs(in);
-  const int P = (int)d->P, G = d->G, M = d->M;
-  const size_t S = (size_t)P * 12;
-  const dim3 pg(blocks_for(P, 256)), ng(blocks_for(G, 256), M);
-  const int64_t gn = (int64_t)M * 5 * G;
-  std::vector<float> times(nsteps + 1);
-  times[0] = t0;
-  for (int s = 0; s < nsteps; ++s) times[s + 1] = times[s] + d->k.dt;
-  float* mid = d->gbuf + 2 * S;
-  const float* bout = grad_out;
-  for (int s = nsteps - 1; s >= 0; --s) {
-    const float* cur = traj + (size_t)s * S;
-    float* bin = s == 0 ? grad_in : d->gbuf + (s % 2) * S;
-    hipLaunchKernelGGL(jmd::k_d_fill, dim3(blocks_for(gn, 256 * 8)), dim3(256), 0, st, d->grid, gn);
-    hipLaunchKernelGGL(jmd::k_d_p2g, pg, dim3(256), 0, st, din, cur, (float*)nullptr, d->grid, P,
-                       G, times[s], d->k);
-    hipLaunchKernelGGL(jmd::k_d_gridop, ng, dim3(256), 0, st, d->grid, M, G, d->k);
-    hipLaunchKernelGGL(jmd::k_d_fill, dim3(blocks_for(gn, 256 * 8)), dim3(256), 0, st, d->agrid, gn);
-    hipLaunchKernelGGL(jmd::k_d_g2p_bwd, pg, dim3(256), 0, st, din, cur, bout, mid, d->grid,
-                       d->agrid, P, G, d->k);
-    hipLaunchKernelGGL(jmd::k_d_gridop_bwd, ng, dim3(256), 0, st, d->grid, d->agrid, M, G, d->k);
-    hipLaunchKernelGGL(jmd::k_d_p2g_bwd, pg, dim3(256), 0, st, din, cur, bout, mid, bin,
-                       d->agrid, P, G, times[s], d->k, grad_amp, grad_pp);
-    bout = bin;
-  }
-  HIP_TRY(hipGetLastError());
-  return JM_OK;
+  if (d->bits == 64)
+    return bwd_block<double>(d, in, static_cast<const double*>(traj),
+                             static_cast<const double*>(grids), t0, nsteps,
+                             static_cast<const double*>(grad_out), static_cast<double*>(grad_in),
+                             static_cast<double*>(grad_amp), static_cast<double*>(grad_pp), st);
+  return bwd_block<float>(d, in, static_cast<const float*>(traj),
+                          static_cast<const float*>(grids), t0, nsteps,
+                          static_cast<const float*>(grad_out), static_cast<float*>(grad_in),
+                          static_cast<float*>(grad_amp), static_cast<float*>(grad_pp), st);
 }
 
 }  // extern "C"

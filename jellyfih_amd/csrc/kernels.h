@@ -548,73 +548,102 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     }
     __syncthreads();
     float* gw = a.acc_w + (size_t)m * 3 * G;
+    const int lane = tid & 63;
+    const unsigned long long lanes_le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
-      if (mt[p] < 0) continue;
       const int li = p * BLOCK + tid;
       const int gi = start + li;
-      int b0, b1;
-      float fx0, fx1;
-      base_frac(x[p].x, k.inv_dx, b0, fx0);
-      base_frac(x[p].y, k.inv_dx, b1, fx1);
-      if (!in_range(b0, b1, k)) {
-        // Skipped by P2G (mpm_sim.py:184); its v, C are carried in memory.
-        if (DO_G2P && a.lazy_vc) {
+      int b0 = 0, b1 = 0;
+      float fx0 = 0.f, fx1 = 0.f;
+      bool inr = false;
+      if (mt[p] >= 0) {
+        base_frac(x[p].x, k.inv_dx, b0, fx0);
+        base_frac(x[p].y, k.inv_dx, b1, fx1);
+        inr = in_range(b0, b1, k);
+        if (!inr && DO_G2P && a.lazy_vc) {
           a.s.v[gi] = v[p];
           a.s.C[gi] = C[p];
         }
-        continue;
       }
-      float2 fib = make_float2(0.f, 1.f);
-      if (mt[p] == 3) fib = a.s.fiber[gi];
-      const Mat2 Cm = as_mat(C[p]);
-      P2GOut o = p2g_update(mt[p], as_mat(a.s.F[gi]), Cm, fib.x, fib.y, amp_act, mu_j, la_j,
-                            pdens, k);
-      a.s.F[gi] = as_f4(o.F);
-      if (o.jp_changes) a.s.Jp[gi] = (a.s.Jp[gi] * o.q0) * o.q1;
-      const float mv0 = o.mass * v[p].x, mv1 = o.mass * v[p].y;
-      float wx[3], wy[3];
-      bspline(fx0, wx);
-      bspline(fx1, wy);
-      // |contribution| <= |m v| + 1.5 dx (|A0.| + |A1.|) since w <= 1, |dpos| <= 1.5 dx
-      const float bnd = 1.5f * k.dx;
-      const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(o.affine.a00) + fabsf(o.affine.a01)),
-                                    fabsf(mv1) + bnd * (fabsf(o.affine.a10) + fabsf(o.affine.a11))),
-                              o.mass);
-      const bool fixed_ok = use_lds && lim < kFixMax;  // false for NaN
-      // Separable form of w_ij (m v + A dpos): with a_i = wx_i, b_i = wx_i d0_i, g_j = wy_j,
-      // e_j = wy_j d1_j, the momentum is g_j (a_i m v + b_i A.0) + e_j (a_i A.1) and the
-      // mass g_j (a_i m).
-      float P0[3], Q0[3], P1[3], Q1[3], Mi[3], ey[3];
-      const float sc = fixed_ok ? kFixPre : 1.0f;
+      float P0[3] = {0.f, 0.f, 0.f}, Q0[3] = {0.f, 0.f, 0.f}, P1[3] = {0.f, 0.f, 0.f},
+            Q1[3] = {0.f, 0.f, 0.f}, Mi[3] = {0.f, 0.f, 0.f}, ey[3] = {0.f, 0.f, 0.f},
+            wy[3] = {0.f, 0.f, 0.f};
+      bool fixed_ok = false;
+      if (inr) {
+        float2 fib = make_float2(0.f, 1.f);
+        if (mt[p] == 3) fib = a.s.fiber[gi];
+        const Mat2 Cm = as_mat(C[p]);
+        P2GOut o = p2g_update(mt[p], as_mat(a.s.F[gi]), Cm, fib.x, fib.y, amp_act, mu_j, la_j,
+                              pdens, k);
+        a.s.F[gi] = as_f4(o.F);
+        if (o.jp_changes) a.s.Jp[gi] = (a.s.Jp[gi] * o.q0) * o.q1;
+        const float mv0 = o.mass * v[p].x, mv1 = o.mass * v[p].y;
+        float wx[3];
+        bspline(fx0, wx);
+        bspline(fx1, wy);
+        const float bnd = 1.5f * k.dx;
+        const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(o.affine.a00) + fabsf(o.affine.a01)),
+                                      fabsf(mv1) + bnd * (fabsf(o.affine.a10) + fabsf(o.affine.a11))),
+                                o.mass);
+        fixed_ok = use_lds && lim < kFixMax;
+        const float sc = fixed_ok ? kFixPre : 1.0f;
 #pragma unroll
-      for (int ii = 0; ii < 3; ++ii) {
-        const float ai = wx[ii] * sc;
-        const float bi = ai * (((float)ii - fx0) * k.dx);
-        P0[ii] = ai * mv0 + bi * o.affine.a00;
-        Q0[ii] = ai * o.affine.a01;
-        P1[ii] = ai * mv1 + bi * o.affine.a10;
-        Q1[ii] = ai * o.affine.a11;
-        Mi[ii] = ai * o.mass;
+        for (int ii = 0; ii < 3; ++ii) {
+          const float ai = wx[ii] * sc;
+          const float bi = ai * (((float)ii - fx0) * k.dx);
+          P0[ii] = ai * mv0 + bi * o.affine.a00;
+          Q0[ii] = ai * o.affine.a01;
+          P1[ii] = ai * mv1 + bi * o.affine.a10;
+          Q1[ii] = ai * o.affine.a11;
+          Mi[ii] = ai * o.mass;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * (((float)jj - fx1) * k.dx);
+        if (!fixed_ok) {
+          scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
+                         {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
+                         {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
+        }
       }
+      // Cell-sorted particles of one base cell sit in adjacent lanes (water: 4 per cell).
+      // Runs of adjacent lanes with the same cell, within a DPP row of 16 lanes, add their
+      // 27 contributions across lanes (row_shl:1, then row_shl:2 when the lane two along
+      // is in the same run), so the first lane of every group of up to 4 in a run holds the
+      // group's sums and alone issues the 27 64-bit LDS atomics: about a quarter of the
+      // atomics of one set per particle, in a fixed order (deterministic). Lanes outside
+      // the fixed-point LDS path carry unique keys and add nothing.
+      const int key = fixed_ok ? (b1 * nx + b0 + 1) : -(1 + lane);
+      const int k1 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x101, 0xF, 0xF, false);
+      const int k2 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x102, 0xF, 0xF, false);
+      const int kp = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x111, 0xF, 0xF, false);
+      const bool same1 = k1 == key, same12 = same1 && (k2 == key);
+      const unsigned long long heads = __ballot(kp != key);
+      const int runstart = 63 - __clzll(heads & lanes_le);
+      const bool emit = fixed_ok && (((lane - runstart) & 3) == 0);
+      const int q0 = (b1 - mn1) * ww + (b0 - mn0);
 #pragma unroll
-      for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * (((float)jj - fx1) * k.dx);
-      if (fixed_ok) {
-        const int q0 = (b1 - mn1) * ww + (b0 - mn0);
+      for (int jj = 0; jj < 3; ++jj) {
 #pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
+        for (int ii = 0; ii < 3; ++ii) {
+          float c[3] = {wy[jj] * P0[ii] + ey[jj] * Q0[ii], wy[jj] * P1[ii] + ey[jj] * Q1[ii],
+                        wy[jj] * Mi[ii]};
 #pragma unroll
-          for (int ii = 0; ii < 3; ++ii) {
+          for (int e = 0; e < 3; ++e) {
+            const float t1 = __int_as_float(__builtin_amdgcn_update_dpp(
+                0, __float_as_int(c[e]), 0x101, 0xF, 0xF, false));
+            const float c1 = same1 ? c[e] + t1 : c[e];
+            const float t2 = __int_as_float(__builtin_amdgcn_update_dpp(
+                0, __float_as_int(c1), 0x102, 0xF, 0xF, false));
+            c[e] = same12 ? c1 + t2 : c1;
+          }
+          if (emit) {
             const int q = q0 + jj * ww + ii;
-            add_fixed_scaled(&lds64[q], wy[jj] * P0[ii] + ey[jj] * Q0[ii]);
-            add_fixed_scaled(&lds64[WMAX + q], wy[jj] * P1[ii] + ey[jj] * Q1[ii]);
-            add_fixed_scaled(&lds64[2 * WMAX + q], wy[jj] * Mi[ii]);
+            add_fixed_scaled(&lds64[q], c[0]);
+            add_fixed_scaled(&lds64[WMAX + q], c[1]);
+            add_fixed_scaled(&lds64[2 * WMAX + q], c[2]);
           }
         }
-      } else {
-        scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
-                       {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
-                       {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
       }
     }
     __syncthreads();

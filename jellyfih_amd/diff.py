@@ -5,7 +5,7 @@ substep behind ``torch.autograd``: the forward and the adjoint are HIP kernels i
 ``libjellymx.so`` (``jm_fwd_block`` / ``jm_bwd_block``), and :class:`_Block` is one
 ``torch.autograd.Function`` per block of ``checkpoint`` substeps. A block keeps only its
 input state (the checkpoint); its backward re-runs the block's forward storing every
-substep's state, then runs the adjoint substeps in reverse.
+substep's state and grid, then runs the adjoint substeps in reverse.
 
 State layout, one row of 12 fp32 per live particle:
 ``x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11``. Particles of all instances share one
@@ -43,9 +43,15 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def clock(t0: float, n: int, dt=None) -> float:
-    """The fp32 clock after ``n`` substeps from ``t0`` (``sim_time += dt`` in fp32,
-    ``mpm_sim.py:336-340``)."""
+def clock(t0: float, n: int, dt=None, fp64: bool = False) -> float:
+    """The clock after ``n`` substeps from ``t0``: ``sim_time += dt`` in fp32 as the
+    reference does (``mpm_sim.py:336-340``), or in fp64 for the fp64 path."""
+    if fp64:
+        t = float(t0)
+        step = 2e-5 if dt is None else float(dt)
+        for _ in range(int(n)):
+            t = t + step
+        return t
     dt = DT if dt is None else np.float32(dt)
     t = np.float32(t0)
     for _ in range(int(n)):
@@ -58,11 +64,18 @@ class DiffTank:
     and scratch the forward and adjoint substeps need.
 
     ``inst`` (P,) instance index, ``mat`` (P,) material (0 water, 1 jelly, 2 payload,
-    3 muscle), ``fiber`` (P, 2) muscle fibre direction."""
+    3 muscle), ``fiber`` (P, 2) muscle fibre direction. ``dtype`` is the arithmetic of
+    states, grids and gradients: float32 (the reference's precision) or float64 (for
+    gradients that must agree with fp64 finite differences: an fp32 trajectory's positions
+    carry rounding of ~1e-5 after thousands of substeps, a percent of a typical payload
+    displacement)."""
 
     def __init__(self, inst, mat, fiber, n_instances: int, quality: int = 1,
                  n_grid_y: int | None = None, domain_height: float = 1.0,
-                 device: str | torch.device = "cuda"):
+                 device: str | torch.device = "cuda", dtype: torch.dtype = torch.float32):
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError("dtype must be torch.float32 or torch.float64")
+        self.dtype = dtype
         load()
         dev = torch.device(device)
         if dev.type == "cuda" and dev.index is None:
@@ -70,7 +83,7 @@ class DiffTank:
         self.device = dev
         self.inst = torch.as_tensor(inst, dtype=torch.int32).to(dev).contiguous()
         self.mat = torch.as_tensor(mat, dtype=torch.int32).to(dev).contiguous()
-        self.fiber = torch.as_tensor(fiber, dtype=torch.float32).to(dev).reshape(-1, 2)
+        self.fiber = torch.as_tensor(fiber, dtype=dtype).to(dev).reshape(-1, 2)
         self.fiber = self.fiber.contiguous()
         self.M = int(n_instances)
         self.P = int(self.inst.numel())
@@ -84,9 +97,14 @@ class DiffTank:
             raise ValueError("instance index out of range")
         if mat_np.min() < 0 or mat_np.max() > 3:
             raise ValueError("materials must be 0..3 (dead slots are dropped when packing)")
-        self._inst_long = self.inst.long()
-        self.payload_count = torch.zeros(self.M, dtype=torch.float64, device=dev).index_add_(
-            0, self._inst_long, (self.mat == 2).double())
+        # per-instance sums of per-particle values: a stable sort by instance (identity when
+        # packed instance by instance) and the segment bounds, for cumulative-sum reductions
+        order = np.argsort(inst_np, kind="stable")
+        self._order = None if np.all(order == np.arange(self.P)) else \
+            torch.from_numpy(order).to(dev)
+        bounds = np.searchsorted(inst_np[order], np.arange(self.M + 1))
+        self._seg = torch.from_numpy(bounds.astype(np.int64)).to(dev)
+        self.payload_count = self.instance_sums((self.mat == 2).double())
         cfg = JmConfig()
         call("jm_default_config", ctypes.byref(cfg))
         cfg.n_instances = self.M
@@ -94,9 +112,13 @@ class DiffTank:
         cfg.quality = int(quality)
         cfg.n_grid_y = int(n_grid_y or 0)
         cfg.domain_height = float(domain_height)
-        self.dt = np.float32(2e-5 / quality)
+        self.dt = 2e-5 / quality
+        self.nx = 128 * int(quality)
+        self.ny = int(n_grid_y or self.nx)
+        self.G = self.nx * self.ny
         h = ctypes.c_void_p()
-        call("jm_diff_create", ctypes.byref(cfg), ctypes.c_int64(self.P), ctypes.byref(h))
+        call("jm_diff_create", ctypes.byref(cfg), ctypes.c_int64(self.P),
+             64 if dtype == torch.float64 else 32, ctypes.byref(h))
         self._h = h
         self._L = load()
 
@@ -108,6 +130,7 @@ class DiffTank:
         pos = np.asarray(pos, np.float32)
         mat = np.asarray(mat, np.int32)
         fiber = np.asarray(fiber, np.float32)
+        dtype = kw.get("dtype", torch.float32)
         M = mat.shape[0]
         live = mat >= 0
         inst = np.repeat(np.arange(M, dtype=np.int32)[:, None], mat.shape[1], 1)[live]
@@ -116,7 +139,7 @@ class DiffTank:
         state[:, 0:2] = pos[live]
         state[:, 8] = 1.0
         state[:, 11] = 1.0
-        return tank, torch.from_numpy(state).to(tank.device)
+        return tank, torch.from_numpy(state).to(tank.device, dtype)
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -135,76 +158,112 @@ class DiffTank:
                             amp_scale.data_ptr(), params.data_ptr())
 
     def _check(self, state, amp_scale, params):
-        if state.shape != (self.P, STATE_COLS) or state.dtype != torch.float32:
-            raise ValueError(f"state must be ({self.P}, {STATE_COLS}) float32")
+        if state.shape != (self.P, STATE_COLS):
+            raise ValueError(f"state must be ({self.P}, {STATE_COLS})")
         if amp_scale.shape != (self.P,) or params.shape != (self.M, 5):
             raise ValueError("amp_scale must be (P,) and params (M, 5)")
+        for t in (state, amp_scale, params):
+            if t.dtype != self.dtype:
+                raise ValueError(f"tensors must be {self.dtype}")
         for t in (state, amp_scale, params):
             if t.device != self.device or not t.is_contiguous():
                 raise ValueError("tensors must be contiguous and on the tank's device")
 
     def forward_raw(self, state, amp_scale, params, t0: float, n: int, out=None,
-                    store_all: bool = False) -> torch.Tensor:
-        """``n`` substeps; returns the final state, or (n, P, 12) every state if
-        ``store_all``. ``params`` (M, 5) = amp, contraction, freq, payload density, E."""
+                    grids=None) -> torch.Tensor:
+        """``n`` substeps; returns the final state (P, 12). With ``grids`` (a tensor
+        (n, M, 3, nx*ny) of the tank's dtype), ``out`` must be (n, P, 12) and receives every
+        substep's state, ``grids`` every substep's momentum and mass planes. ``params``
+        (M, 5) = amp, contraction, freq, payload density, E_jelly."""
         self._check(state, amp_scale, params)
-        if out is None:
-            shape = (n, self.P, STATE_COLS) if store_all else (self.P, STATE_COLS)
-            out = torch.empty(shape, dtype=torch.float32, device=self.device)
+        if grids is not None:
+            if out is None or tuple(out.shape) != (n, self.P, STATE_COLS) or \
+                    tuple(grids.shape) != (n, self.M, 3, self.G) or not grids.is_contiguous():
+                raise ValueError("stored trajectory needs out (n, P, 12) and grids "
+                                 "(n, M, 3, nx*ny)")
+        elif out is None:
+            out = torch.empty((self.P, STATE_COLS), dtype=self.dtype, device=self.device)
         ins = self._inputs(amp_scale, params)
-        call("jm_fwd_block", self._h, ctypes.byref(ins), _ptr(state), ctypes.c_float(t0),
-             int(n), _ptr(out), int(bool(store_all)), _stream())
+        call("jm_fwd_block", self._h, ctypes.byref(ins), _ptr(state), ctypes.c_double(t0),
+             int(n), _ptr(out), None if grids is None else _ptr(grids), _stream())
         return out
 
-    def backward_raw(self, state, amp_scale, params, t0: float, n: int, grad_out):
-        """Adjoint of ``n`` substeps from ``state``: returns the adjoint of the input state
-        (P, 12), of ``amp_scale`` (P,) and of ``params`` (M, 5; column 3 is zero)."""
-        self._check(state, amp_scale, params)
-        traj = torch.empty((n + 1, self.P, STATE_COLS), dtype=torch.float32,
-                           device=self.device)
+    def instance_sums(self, values: torch.Tensor) -> torch.Tensor:
+        """Per-instance fp64 sums of per-particle values, (P,) -> (M,) or (P, C) -> (M, C):
+        differences of a cumulative sum at the instance bounds (the scan runs along the
+        contiguous particle axis)."""
+        v = values.double()
+        if self._order is not None:
+            v = v[self._order]
+        vt = v.reshape(self.P, -1).t().contiguous()                  # (C, P)
+        cs = torch.cat([torch.zeros_like(vt[:, :1]), torch.cumsum(vt, 1)], 1)
+        out = (cs[:, self._seg[1:]] - cs[:, self._seg[:-1]]).t()     # (M, C)
+        return out.reshape((self.M,) + tuple(values.shape[1:]))
+
+    def trajectory(self, state, amp_scale, params, t0: float, n: int):
+        """``n`` substeps from ``state`` with every state and grid stored: returns
+        (traj (n+1, P, 12) starting with ``state``, grids (n, M, 3, nx*ny))."""
+        traj = torch.empty((n + 1, self.P, STATE_COLS), dtype=self.dtype, device=self.device)
         traj[0].copy_(state)
-        self.forward_raw(state, amp_scale, params, t0, n, out=traj[1:], store_all=True)
-        grad_out = grad_out.to(torch.float32).contiguous()
+        grids = torch.empty((n, self.M, 3, self.G), dtype=self.dtype, device=self.device)
+        self.forward_raw(state, amp_scale, params, t0, n, out=traj[1:], grids=grids)
+        return traj, grids
+
+    def backward_raw(self, state, amp_scale, params, t0: float, n: int, grad_out,
+                     stored=None):
+        """Adjoint of ``n`` substeps from ``state``: returns the adjoint of the input state
+        (P, 12), of ``amp_scale`` (P,) and of ``params`` (M, 5; column 3 is zero). The
+        trajectory is recomputed unless ``stored`` = trajectory(...) of the same block."""
+        self._check(state, amp_scale, params)
+        traj, grids = stored if stored is not None else \
+            self.trajectory(state, amp_scale, params, t0, n)
+        grad_out = grad_out.to(self.dtype).contiguous()
         grad_in = torch.empty_like(state)
-        gamp = torch.zeros(self.P, dtype=torch.float32, device=self.device)
-        gpp = torch.zeros((self.P, 4), dtype=torch.float32, device=self.device)
+        gamp = torch.zeros(self.P, dtype=self.dtype, device=self.device)
+        gpp = torch.zeros((self.P, 4), dtype=self.dtype, device=self.device)
         ins = self._inputs(amp_scale, params)
-        call("jm_bwd_block", self._h, ctypes.byref(ins), _ptr(traj), ctypes.c_float(t0),
-             int(n), _ptr(grad_out), _ptr(grad_in), _ptr(gamp), _ptr(gpp), _stream())
-        per_inst = torch.zeros((self.M, 4), dtype=torch.float64, device=self.device)
-        per_inst.index_add_(0, self._inst_long, gpp.double())
-        gpar = torch.zeros((self.M, 5), dtype=torch.float32, device=self.device)
-        gpar[:, 4] = per_inst[:, 0].float()    # E_jelly
-        gpar[:, 0] = per_inst[:, 1].float()    # amplitude
-        gpar[:, 1] = per_inst[:, 2].float()    # contraction
-        gpar[:, 2] = per_inst[:, 3].float()    # freq
+        call("jm_bwd_block", self._h, ctypes.byref(ins), _ptr(traj), _ptr(grids),
+             ctypes.c_double(t0), int(n), _ptr(grad_out), _ptr(grad_in), _ptr(gamp), _ptr(gpp),
+             _stream())
+        per_inst = self.instance_sums(gpp)
+        gpar = torch.zeros((self.M, 5), dtype=self.dtype, device=self.device)
+        gpar[:, 4] = per_inst[:, 0]    # E_jelly
+        gpar[:, 0] = per_inst[:, 1]    # amplitude
+        gpar[:, 1] = per_inst[:, 2]    # contraction
+        gpar[:, 2] = per_inst[:, 3]    # freq
         return grad_in, gamp, gpar
 
     # -- autograd ---------------------------------------------------------------------
     def run(self, state0: torch.Tensor, amp_scale=None, amp=500.0, contraction=0.2, freq=1.0,
             E_jelly=700.0, payload_density=2.5, t0: float = 0.0, nsteps: int = 100,
-            checkpoint: int = 100) -> torch.Tensor:
+            checkpoint: int = 100, recompute: bool = True) -> torch.Tensor:
         """``nsteps`` substeps from ``state0`` at time ``t0``, as autograd blocks of
-        ``checkpoint`` substeps. Per-instance arguments are scalars or (M,) tensors."""
+        ``checkpoint`` substeps. Per-instance arguments are scalars or (M,) tensors.
+
+        ``recompute`` (default): a block keeps only its input state and re-runs its
+        substeps in the backward pass. ``recompute=False`` keeps every block's states and
+        grids from the forward pass instead: no second forward, at ((checkpoint + 1) * P * 12
+        + checkpoint * M * 3 * nx * ny) scalars of device memory per block."""
         if amp_scale is None:
-            amp_scale = torch.ones(self.P, dtype=torch.float32, device=self.device)
+            amp_scale = torch.ones(self.P, dtype=self.dtype, device=self.device)
 
         def inst_vec(v):
             if isinstance(v, torch.Tensor):
-                return v.to(self.device, torch.float32).expand(self.M)
-            return torch.full((self.M,), float(v), dtype=torch.float32, device=self.device)
+                return v.to(self.device, self.dtype).expand(self.M)
+            return torch.full((self.M,), float(v), dtype=self.dtype, device=self.device)
 
         amp, contraction, freq, E_jelly = (inst_vec(v) for v in (amp, contraction, freq,
                                                                  E_jelly))
         pdens = inst_vec(payload_density).detach()
         state = state0
-        t = float(np.float32(t0))
+        fp64 = self.dtype == torch.float64
+        t = float(t0) if fp64 else float(np.float32(t0))
         done = 0
         while done < nsteps:
             n = min(int(checkpoint), nsteps - done)
             state = _Block.apply(state, amp_scale, amp, contraction, freq, E_jelly, pdens,
-                                 self, t, n)
-            t = clock(t, n, self.dt)
+                                 self, t, n, recompute)
+            t = clock(t, n, self.dt, fp64)
             done += n
         return state
 
@@ -212,20 +271,26 @@ class DiffTank:
         """Per-instance mean y of the payload particles (the payload CoM the reference's
         fitness reads, ``mpm_sim.py:746-777``), differentiable in ``state``."""
         w = (self.mat == 2).to(state.dtype)
-        s = torch.zeros(self.M, dtype=state.dtype, device=self.device)
-        s = s.index_add(0, self._inst_long, state[:, 1] * w)
-        return s / self.payload_count.clamp(min=1).to(state.dtype)
+        s = self.instance_sums(state[:, 1] * w)
+        return (s / self.payload_count.clamp(min=1)).to(state.dtype)
 
 
 class _Block(torch.autograd.Function):
-    """``n`` substeps as one autograd node; saves only its input state."""
+    """``n`` substeps as one autograd node; keeps its input state (and, without
+    recomputation, the block's stored states and grids)."""
 
     @staticmethod
-    def forward(ctx, state, amp_scale, amp, contraction, freq, E_jelly, pdens, tank, t0, n):
+    def forward(ctx, state, amp_scale, amp, contraction, freq, E_jelly, pdens, tank, t0, n,
+                recompute):
         params = torch.stack([amp, contraction, freq, pdens, E_jelly], 1).detach().contiguous()
         state_c = state.detach().contiguous()
-        amp_c = amp_scale.detach().to(torch.float32).contiguous()
-        out = tank.forward_raw(state_c, amp_c, params, t0, n)
+        amp_c = amp_scale.detach().to(tank.dtype).contiguous()
+        if recompute:
+            out = tank.forward_raw(state_c, amp_c, params, t0, n)
+            ctx.stored = None
+        else:
+            ctx.stored = tank.trajectory(state_c, amp_c, params, t0, n)
+            out = ctx.stored[0][-1].clone()
         ctx.save_for_backward(state_c, amp_c, params)
         ctx.tank, ctx.t0, ctx.n = tank, t0, n
         return out
@@ -233,6 +298,8 @@ class _Block(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         state, amp_c, params = ctx.saved_tensors
-        gin, gamp, gpar = ctx.tank.backward_raw(state, amp_c, params, ctx.t0, ctx.n, grad_out)
+        gin, gamp, gpar = ctx.tank.backward_raw(state, amp_c, params, ctx.t0, ctx.n, grad_out,
+                                                stored=ctx.stored)
+        ctx.stored = None
         return (gin, gamp, gpar[:, 0], gpar[:, 1], gpar[:, 2], gpar[:, 4], None, None, None,
-                None)
+                None, None)

@@ -1,7 +1,9 @@
 // TEST HARNESS: the engine's adjoint math (jellyfih_amd/csrc/adjoint.h) compiled for the
-// host in fp64, single-threaded, for small particle sets. tests/test_adjoint_cpu.py checks
-// its forward against the oracle and its gradients against central differences; the GPU
-// tests then check the HIP kernels (same header, fp32) against it.
+// host, single-threaded, for small particle sets; in fp64 by default, in fp32 with
+// -DHARNESS_FP32 (arrays stay double, holding the fp32 values exactly; the clock is fp32).
+// tests/test_adjoint_cpu.py checks its forward against the oracle and its gradients
+// against central differences; the GPU tests then check the HIP kernels (same header)
+// against it.
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -18,7 +20,21 @@ struct OracleConsts {  // same layout as oracle/mpm_oracle.cpp
 
 namespace {
 
+#ifdef HARNESS_FP32
+using T = float;
+#else
 using T = double;
+#endif
+
+jmd::PState<T> load(const double* s) {
+  return {T(s[0]), T(s[1]), T(s[2]), T(s[3]), {T(s[4]), T(s[5]), T(s[6]), T(s[7])},
+          {T(s[8]), T(s[9]), T(s[10]), T(s[11])}};
+}
+void store(double* s, const jmd::PState<T>& p) {
+  s[0] = p.x0; s[1] = p.x1; s[2] = p.v0; s[3] = p.v1;
+  s[4] = p.C.a; s[5] = p.C.b; s[6] = p.C.c; s[7] = p.C.d;
+  s[8] = p.F.a; s[9] = p.F.b; s[10] = p.F.c; s[11] = p.F.d;
+}
 
 jmd::Consts<T> make(const OracleConsts& c) {
   jmd::Consts<T> k;
@@ -50,13 +66,42 @@ jmd::Consts<T> make(const OracleConsts& c) {
   return k;
 }
 
-struct Add {
-  void operator()(T* p, T v) const { *p += v; }
+// Plain accumulation into the three planes (momentum x, momentum y, mass) of one grid.
+struct Planes {
+  T* g;
+  size_t G;
+  int nx;
+  void operator()(int i, int j, T c0, T c1, T c2) const {
+    const size_t n = (size_t)j * nx + i;
+    g[n] += c0;
+    g[G + n] += c1;
+    g[2 * G + n] += c2;
+  }
 };
 
 void lame(T E, T nu, T& mu, T& la) {
   mu = E / (2 * (1 + nu));
   la = E * nu / ((1 + nu) * (1 - 2 * nu));
+}
+
+std::vector<T> tv(const double* a, size_t n) { return std::vector<T>(a, a + n); }
+
+// Momentum and mass of every instance's grid from the states `st` at time t.
+void p2g_all(const jmd::Consts<T>& k, int M, int P, const int32_t* inst, const int32_t* mat,
+             const T* fiber, const T* amp_scale, const T* params, T t,
+             std::vector<jmd::PState<T>>& st, std::vector<T>& grid, size_t G) {
+  std::fill(grid.begin(), grid.end(), 0.0);
+  for (int p = 0; p < P; ++p) {
+    if (mat[p] < 0) continue;
+    const int m = inst[p];
+    const T* pr = params + 5 * m;
+    jmd::Act<T> act = jmd::activation(t, pr[2], pr[1], k);
+    T mu, la;
+    lame(pr[4], T(0.3), mu, la);
+    jmd::p2g_particle(st[p], mat[p], fiber[2 * p], fiber[2 * p + 1],
+                      act.a * pr[0] * amp_scale[p], mu, la, pr[3], k,
+                      Planes{grid.data() + (size_t)m * 3 * G, G, k.nx});
+  }
 }
 
 }  // namespace
@@ -69,41 +114,23 @@ int adj_forward(const OracleConsts* c, int M, int P, const int32_t* inst, const 
                 const double* state_in, double t0, int nsteps, double* traj) {
   jmd::Consts<T> k = make(*c);
   const size_t G = (size_t)k.nx * k.ny;
-  std::vector<T> grid((size_t)M * 5 * G);
-  std::memcpy(traj, state_in, sizeof(double) * 12 * P);
-  T t = t0;
+  std::vector<T> grid((size_t)M * 3 * G);
+  const std::vector<T> fib = tv(fiber, 2 * (size_t)P), amps = tv(amp_scale, P),
+                       par = tv(params, 5 * (size_t)M);
+  for (size_t i = 0; i < 12 * (size_t)P; ++i) traj[i] = T(state_in[i]);
+  T t = T(t0);
   for (int s = 0; s < nsteps; ++s) {
-    const T* cur = traj + (size_t)s * P * 12;
-    T* nxt = traj + (size_t)(s + 1) * P * 12;
-    std::fill(grid.begin(), grid.end(), 0.0);
+    const double* cur = traj + (size_t)s * P * 12;
+    double* nxt = traj + (size_t)(s + 1) * P * 12;
     std::vector<jmd::PState<T>> st(P);
-    for (int p = 0; p < P; ++p) st[p] = jmd::load_state(cur + 12 * p);
-    for (int p = 0; p < P; ++p) {
-      if (mat[p] < 0) continue;
-      const int m = inst[p];
-      const T* pr = params + 5 * m;
-      jmd::Act<T> act = jmd::activation(t, pr[2], pr[1], k);
-      T mu, la;
-      lame(pr[4], 0.3, mu, la);
-      T* g = grid.data() + (size_t)m * 5 * G;
-      jmd::p2g_scatter(st[p], mat[p], fiber[2 * p], fiber[2 * p + 1],
-                       act.a * pr[0] * amp_scale[p], mu, la, pr[3], k, g, g + G, g + 2 * G,
-                       Add());
-    }
-    for (int m = 0; m < M; ++m) {
-      T* g = grid.data() + (size_t)m * 5 * G;
-      for (int j = 0; j < k.ny; ++j)
-        for (int i = 0; i < k.nx; ++i) {
-          size_t n = (size_t)j * k.nx + i;
-          jmd::gridop_forward(g[n], g[G + n], g[2 * G + n], i, j, k, g[3 * G + n], g[4 * G + n]);
-        }
-    }
+    for (int p = 0; p < P; ++p) st[p] = load(cur + 12 * p);
+    p2g_all(k, M, P, inst, mat, fib.data(), amps.data(), par.data(), t, st, grid, G);
     for (int p = 0; p < P; ++p) {
       if (mat[p] >= 0) {
-        T* g = grid.data() + (size_t)inst[p] * 5 * G;
-        jmd::g2p_particle(st[p], k, g + 3 * G, g + 4 * G);
+        T* g = grid.data() + (size_t)inst[p] * 3 * G;
+        jmd::g2p_particle(st[p], k, g, g + G, g + 2 * G);
       }
-      jmd::store_state(nxt + 12 * p, st[p]);
+      store(nxt + 12 * p, st[p]);
     }
     t = t + k.dt;
   }
@@ -120,90 +147,59 @@ int adj_backward(const OracleConsts* c, int M, int P, const int32_t* inst, const
                  double* grad_in, double* grad_amp_scale, double* grad_pp) {
   jmd::Consts<T> k = make(*c);
   const size_t G = (size_t)k.nx * k.ny;
+  const std::vector<T> fib = tv(fiber, 2 * (size_t)P), amps = tv(amp_scale, P),
+                       par = tv(params, 5 * (size_t)M);
   std::vector<T> times(nsteps + 1);
-  times[0] = t0;
+  times[0] = T(t0);
   for (int s = 0; s < nsteps; ++s) times[s + 1] = times[s] + k.dt;
-  std::vector<T> grid((size_t)M * 5 * G), adj((size_t)M * 5 * G);
+  std::vector<T> grid((size_t)M * 3 * G), adj((size_t)M * 3 * G);
   std::vector<jmd::PState<T>> bout(P), bin(P);
-  for (int p = 0; p < P; ++p) bout[p] = jmd::load_state(grad_out + 12 * p);
+  for (int p = 0; p < P; ++p) bout[p] = load(grad_out + 12 * p);
   for (int s = nsteps - 1; s >= 0; --s) {
     const T t = times[s];
-    const T* cur = traj + (size_t)s * P * 12;
-    std::fill(grid.begin(), grid.end(), 0.0);
+    const double* cur = traj + (size_t)s * P * 12;
     std::fill(adj.begin(), adj.end(), 0.0);
     std::vector<jmd::PState<T>> st(P), tmp(P);
-    for (int p = 0; p < P; ++p) st[p] = tmp[p] = jmd::load_state(cur + 12 * p);
-    // recompute the grid of substep s
-    for (int p = 0; p < P; ++p) {
-      if (mat[p] < 0) continue;
-      const int m = inst[p];
-      const T* pr = params + 5 * m;
-      jmd::Act<T> act = jmd::activation(t, pr[2], pr[1], k);
-      T mu, la;
-      lame(pr[4], 0.3, mu, la);
-      T* g = grid.data() + (size_t)m * 5 * G;
-      jmd::p2g_scatter(tmp[p], mat[p], fiber[2 * p], fiber[2 * p + 1],
-                       act.a * pr[0] * amp_scale[p], mu, la, pr[3], k, g, g + G, g + 2 * G,
-                       Add());
-    }
-    for (int m = 0; m < M; ++m) {
-      T* g = grid.data() + (size_t)m * 5 * G;
-      for (int j = 0; j < k.ny; ++j)
-        for (int i = 0; i < k.nx; ++i) {
-          size_t n = (size_t)j * k.nx + i;
-          jmd::gridop_forward(g[n], g[G + n], g[2 * G + n], i, j, k, g[3 * G + n], g[4 * G + n]);
-        }
-    }
-    // G2P adjoint (scatter into the grid-velocity adjoint)
+    for (int p = 0; p < P; ++p) st[p] = tmp[p] = load(cur + 12 * p);
+    p2g_all(k, M, P, inst, mat, fib.data(), amps.data(), par.data(), t, tmp, grid, G);
+    // G2P adjoint, through each node's grid op, into the momentum/mass adjoints
     for (int p = 0; p < P; ++p) {
       bin[p] = jmd::PState<T>{0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
       if (mat[p] < 0) {
         bin[p] = bout[p];
         continue;
       }
-      T* g = grid.data() + (size_t)inst[p] * 5 * G;
-      T* a = adj.data() + (size_t)inst[p] * 5 * G;
-      jmd::g2p_backward(st[p], bout[p], k, g + 3 * G, g + 4 * G, a + 3 * G, a + 4 * G, bin[p],
-                        Add());
-    }
-    // grid-op adjoint
-    for (int m = 0; m < M; ++m) {
-      T* g = grid.data() + (size_t)m * 5 * G;
-      T* a = adj.data() + (size_t)m * 5 * G;
-      for (int j = 0; j < k.ny; ++j)
-        for (int i = 0; i < k.nx; ++i) {
-          size_t n = (size_t)j * k.nx + i;
-          jmd::gridop_backward(g[n], g[G + n], g[2 * G + n], i, j, k, a[3 * G + n], a[4 * G + n],
-                               a[n], a[G + n], a[2 * G + n]);
-        }
+      T* g = grid.data() + (size_t)inst[p] * 3 * G;
+      T* a = adj.data() + (size_t)inst[p] * 3 * G;
+      jmd::g2p_backward(st[p], bout[p], k, g, g + G, g + 2 * G, bin[p], Planes{a, G, k.nx});
     }
     // P2G adjoint (gather)
     for (int p = 0; p < P; ++p) {
       if (mat[p] < 0) continue;
       const int m = inst[p];
-      const T* pr = params + 5 * m;
+      const T* pr = par.data() + 5 * m;
       jmd::Act<T> act = jmd::activation(t, pr[2], pr[1], k);
       T mu, la;
-      lame(pr[4], 0.3, mu, la);
-      const T* a = adj.data() + (size_t)m * 5 * G;
+      lame(pr[4], T(0.3), mu, la);
+      const T* a = adj.data() + (size_t)m * 3 * G;
       T mub = 0, lab = 0, betab = 0;
-      jmd::p2g_backward_particle(st[p], mat[p], (T)fiber[2 * p], (T)fiber[2 * p + 1],
-                                 act.a * pr[0] * amp_scale[p], mu, la, pr[3], k, a, a + G,
+      jmd::p2g_backward_particle(st[p], mat[p], fib[2 * p], fib[2 * p + 1],
+                                 act.a * pr[0] * amps[p], mu, la, pr[3], k, a, a + G,
                                  a + 2 * G, bout[p].F, bin[p], mub, lab, betab);
       if (mat[p] == 1 || mat[p] == 3) {
-        const T nu = 0.3;
+        const T nu = T(0.3);
         grad_pp[4 * p] += mub / (2 * (1 + nu)) + lab * nu / ((1 + nu) * (1 - 2 * nu));
       }
       if (mat[p] == 3) {
         grad_amp_scale[p] += betab * act.a * pr[0];
-        grad_pp[4 * p + 1] += betab * act.a * amp_scale[p];
-        grad_pp[4 * p + 2] += betab * pr[0] * amp_scale[p] * act.da_dc;
-        grad_pp[4 * p + 3] += betab * pr[0] * amp_scale[p] * act.da_df;
+        grad_pp[4 * p + 1] += betab * act.a * amps[p];
+        grad_pp[4 * p + 2] += betab * pr[0] * amps[p] * act.da_dc;
+        grad_pp[4 * p + 3] += betab * pr[0] * amps[p] * act.da_df;
       }
     }
     bout.swap(bin);
   }
-  for (int p = 0; p < P; ++p) jmd::store_state(grad_in + 12 * p, bout[p]);
+  for (int p = 0; p < P; ++p) store(grad_in + 12 * p, bout[p]);
   return 0;
 }
 

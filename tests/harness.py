@@ -16,20 +16,22 @@ BUILD = os.path.join(ROOT, "tests", "_build")
 SRC = os.path.join(ROOT, "tests", "adjoint_harness.cpp")
 
 
-def build_harness(force: bool = False) -> str:
+def build_harness(force: bool = False, fp32: bool = False) -> str:
+    """Compile the harness (fp64, or fp32 arithmetic with ``fp32``) into tests/_build."""
     os.makedirs(BUILD, exist_ok=True)
-    out = os.path.join(BUILD, "libadjharness.so")
+    out = os.path.join(BUILD, "libadjharness32.so" if fp32 else "libadjharness.so")
     deps = [SRC, os.path.join(ROOT, "jellyfih_amd", "csrc", "adjoint.h")]
     if force or not os.path.exists(out) or \
             os.path.getmtime(out) < max(os.path.getmtime(d) for d in deps):
         subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
-                        "-o", out + ".tmp", SRC], check=True)
+                        *(["-DHARNESS_FP32"] if fp32 else []), "-o", out + ".tmp", SRC],
+                       check=True)
         os.replace(out + ".tmp", out)
     return out
 
 
-def load_harness():
-    L = ctypes.CDLL(build_harness())
+def load_harness(fp32: bool = False):
+    L = ctypes.CDLL(build_harness(fp32=fp32))
     D = ctypes.POINTER(ctypes.c_double)
     I = ctypes.POINTER(ctypes.c_int32)
     L.adj_forward.argtypes = [ctypes.POINTER(oracle.OracleConsts), ctypes.c_int, ctypes.c_int,
@@ -235,3 +237,49 @@ def harness_amp_scale_fd(L, cs, d, eps_list):
              for sg in (1, -1)]
         out.append((v[0] - v[1]) / (2 * eps))
     return np.array(out)
+
+
+def harness_payload_gradient(L, cs, block: int = 100):
+    """fp64 adjoint of the payload CoM y after ``cs.steps`` substeps (harness build of the
+    engine's adjoint math), with checkpoints every ``block`` substeps so that memory stays
+    at one block's trajectory. Returns (grad state (P,12), grad amp_scale (P,),
+    grad params (M,5))."""
+    pay = cs.mat == 2
+    n_blocks = (cs.steps + block - 1) // block
+    starts, t0s = [cs.state.copy()], [cs.t0]
+    steps_total, t0_total = cs.steps, cs.t0
+    t = cs.t0
+    try:
+        for b in range(n_blocks):
+            n = min(block, steps_total - b * block)
+            cs.steps, cs.t0 = n, t0s[-1]
+            tr = cs.forward(L, state=starts[-1])
+            for _ in range(n):
+                t = t + cs.c.dt
+            if b + 1 < n_blocks:
+                starts.append(tr[-1].copy())
+                t0s.append(t)
+        gout = np.zeros((cs.P, 12))
+        gout[pay, 1] = 1.0 / pay.sum()
+        gamp = np.zeros(cs.P)
+        gpp = np.zeros((cs.P, 4))
+        for b in reversed(range(n_blocks)):
+            n = min(block, steps_total - b * block)
+            cs.steps, cs.t0 = n, t0s[b]
+            tr = cs.forward(L, state=starts[b])
+            gin = np.zeros((cs.P, 12))
+            L.adj_backward(ctypes.byref(cs.c), cs.M, cs.P, _p(cs.inst, ctypes.c_int32),
+                           _p(cs.mat, ctypes.c_int32), _p(np.ascontiguousarray(cs.fiber)),
+                           _p(cs.amp_scale), _p(cs.params), _p(tr), cs.t0, n,
+                           _p(np.ascontiguousarray(gout)), _p(gin), _p(gamp), _p(gpp))
+            gout = gin
+    finally:
+        cs.steps, cs.t0 = steps_total, t0_total
+    gpar = np.zeros((cs.M, 5))
+    for m in range(cs.M):
+        sel = cs.inst == m
+        gpar[m, 4] = gpp[sel, 0].sum()
+        gpar[m, 0] = gpp[sel, 1].sum()
+        gpar[m, 1] = gpp[sel, 2].sum()
+        gpar[m, 2] = gpp[sel, 3].sum()
+    return gout, gamp, gpar

@@ -1,0 +1,47 @@
+"""Host-side logic of the differentiable path (jellyfih_amd/diff.py) that runs without a
+GPU: the per-instance reductions and the fp32 clock."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from jellyfih_amd.diff import DiffTank, clock
+
+
+class _Packed:
+    """The fields instance_sums reads, for particles in a given instance order."""
+
+    def __init__(self, inst, M):
+        inst = np.asarray(inst)
+        self.P, self.M = len(inst), M
+        order = np.argsort(inst, kind="stable")
+        self._order = None if np.all(order == np.arange(self.P)) else torch.from_numpy(order)
+        self._seg = torch.from_numpy(
+            np.searchsorted(inst[order], np.arange(M + 1)).astype(np.int64))
+
+
+def test_instance_sums_match_index_add():
+    rng = np.random.default_rng(0)
+    for inst, M in ((rng.integers(0, 5, 1000), 6), (np.repeat(np.arange(4), 50), 4),
+                    (np.zeros(7, int), 1)):
+        t = _Packed(inst, M)
+        for shape in ((t.P,), (t.P, 4)):
+            v = torch.from_numpy(rng.normal(size=shape))
+            ref = torch.zeros((M,) + shape[1:], dtype=torch.float64).index_add_(
+                0, torch.from_numpy(np.asarray(inst)), v)
+            np.testing.assert_allclose(DiffTank.instance_sums(t, v).numpy(), ref.numpy(),
+                                       rtol=0, atol=1e-12)
+
+
+def test_instance_sums_are_differentiable():
+    t = _Packed([1, 0, 1, 1, 0], 2)
+    v = torch.arange(5, dtype=torch.float64, requires_grad=True)
+    (DiffTank.instance_sums(t, v) * torch.tensor([2.0, 3.0], dtype=torch.float64)).sum().backward()
+    np.testing.assert_array_equal(v.grad.numpy(), [3, 2, 3, 3, 2])
+
+
+def test_clock_is_fp32_accumulation():
+    """sim_time += dt in fp32 (mpm_sim.py:336-340): 150K substeps from 0 end at
+    3.0033383 (not 3.0), and blocks compose."""
+    assert np.float32(clock(0.0, 150000)) == np.float32(3.0033383)
+    assert clock(clock(0.0, 700), 1300) == clock(0.0, 2000)
