@@ -11,7 +11,8 @@
 // stencil nodes when the workgroup's particles are of one instance and close together
 // (particles packed instance by instance in spatial order, as tanks are filled), then add
 // the tile to the grid with one global atomic per touched node and plane; otherwise they
-// use global atomics directly. Every grid a kernel accumulates into starts zeroed: the
+// use global atomics directly. The tile is fp64 for both scalar types: gfx950's fp64 LDS
+// atomic add is much faster than its fp32 one, and the sums only gain precision. Every grid a kernel accumulates into starts zeroed: the
 // kernel before it clears it (grid-stride), or a fill kernel at the start of a block.
 #pragma once
 
@@ -22,7 +23,7 @@
 namespace jmd {
 
 constexpr int kDiffBlock = 256;   // threads per workgroup, one particle each
-constexpr int kTileNodes = 2048;  // LDS tile: 3 planes x 2048 nodes = 24 KB
+constexpr int kTileNodes = 2048;  // LDS tile: 3 planes x 2048 nodes of fp64 = 48 KB
 
 template <typename T>
 struct DiffInputs {
@@ -106,14 +107,15 @@ __device__ Tile block_tile(bool active, int inst, int b0, int b1) {
 
 // Adds the LDS tile (3 planes of tw*th) into the instance's grid planes.
 template <typename T>
-__device__ __forceinline__ void tile_flush(const T* lds, const Tile& t, T* g, int nx, int G) {
+__device__ __forceinline__ void tile_flush(const double* lds, const Tile& t, T* g, int nx,
+                                           int G) {
   const int n = t.tw * t.th;
   for (int li = threadIdx.x; li < n; li += kDiffBlock) {
     const int jj = li / t.tw;
     const int node = (t.oy + jj) * nx + t.ox + (li - jj * t.tw);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T v = lds[c * n + li];
+      const T v = T(lds[c * n + li]);
       if (v != T(0)) unsafeAtomicAdd(g + (size_t)c * G + node, v);
     }
   }
@@ -139,7 +141,7 @@ template <typename T>
 __global__ void __launch_bounds__(kDiffBlock) k_d_p2g(DiffInputs<T> in, const T* cur,
                                                       T* nxt, T* grid, int P, int G,
                                                       T t, Consts<T> k) {
-  __shared__ T lds[3 * kTileNodes];
+  __shared__ double lds[3 * kTileNodes];
   const int p = blockIdx.x * kDiffBlock + threadIdx.x;
   const bool valid = p < P;
   PState<T> s = {T(0), T(0), T(0), T(0), {T(1), T(0), T(0), T(1)}, {T(1), T(0), T(0), T(1)}};
@@ -164,7 +166,7 @@ __global__ void __launch_bounds__(kDiffBlock) k_d_p2g(DiffInputs<T> in, const T*
   const int n = tl.tw * tl.th;
   const GlobalEmit<T> direct{grid + (size_t)m * 3 * G, k.nx, G};
   if (tl.use) {
-    for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) lds[i] = T(0);
+    for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) lds[i] = 0.0;
     __syncthreads();
   }
   if (active)
@@ -172,9 +174,9 @@ __global__ void __launch_bounds__(kDiffBlock) k_d_p2g(DiffInputs<T> in, const T*
                  [&](int i, int j, T c0, T c1, T c2) {
                    if (tl.use) {
                      const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-                     atomicAdd(&lds[li], c0);
-                     atomicAdd(&lds[n + li], c1);
-                     atomicAdd(&lds[2 * n + li], c2);
+                     atomicAdd(&lds[li], double(c0));
+                     atomicAdd(&lds[n + li], double(c1));
+                     atomicAdd(&lds[2 * n + li], double(c2));
                    } else {
                      direct(i, j, c0, c1, c2);
                    }
@@ -217,7 +219,7 @@ __global__ void __launch_bounds__(kDiffBlock) k_d_g2p_bwd(DiffInputs<T> in, cons
                                                           const T* bout, T* bmid,
                                                           const T* grid, T* agrid,
                                                           int P, int G, Consts<T> k) {
-  __shared__ T lds[3 * kTileNodes];
+  __shared__ double lds[3 * kTileNodes];
   const int p = blockIdx.x * kDiffBlock + threadIdx.x;
   const bool valid = p < P;
   PState<T> s = {T(0), T(0), T(0), T(0), {T(1), T(0), T(0), T(1)}, {T(1), T(0), T(0), T(1)}};
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(kDiffBlock) k_d_g2p_bwd(DiffInputs<T> in, cons
   const int n = tl.tw * tl.th;
   const GlobalEmit<T> direct{agrid + (size_t)m * 3 * G, k.nx, G};
   if (tl.use) {
-    for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) lds[i] = T(0);
+    for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) lds[i] = 0.0;
     __syncthreads();
   }
   if (valid && mt >= 0)
@@ -247,9 +249,9 @@ __global__ void __launch_bounds__(kDiffBlock) k_d_g2p_bwd(DiffInputs<T> in, cons
                  [&](int i, int j, T c0, T c1, T c2) {
                    if (tl.use) {
                      const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-                     atomicAdd(&lds[li], c0);
-                     atomicAdd(&lds[n + li], c1);
-                     atomicAdd(&lds[2 * n + li], c2);
+                     atomicAdd(&lds[li], double(c0));
+                     atomicAdd(&lds[n + li], double(c1));
+                     atomicAdd(&lds[2 * n + li], double(c2));
                    } else {
                      direct(i, j, c0, c1, c2);
                    }

@@ -258,16 +258,17 @@ constexpr float kFixPre = 262144.0f;                    // 2^18
 constexpr float kFixMax = 1.0f;
 
 // c * 2^50 as a two's-complement int64, split as hi * 2^32 + lo with hi = floor(c * 2^18)
-// and lo = frac(c * 2^18) * 2^32: every step is exact for |c| < 1 (scaling by powers of two,
-// floor, and the subtraction of an integer part), and the final truncation of lo drops
-// less than one unit of 2^-50. Six VALU operations instead of a generic float-to-int64
-// conversion.
+// and lo = frac(c * 2^18) * 2^32: scaling by powers of two and taking the fraction are
+// exact for |c| < 1, and the truncation of lo drops less than one unit of 2^-50 (for a
+// tiny negative y, v_fract's clamp below 1.0 costs at most 256 units, 2.3e-13). Four VALU
+// operations (fract, mul, cvt_u32, cvt_flr_i32) instead of a generic float-to-int64
+// conversion; the compiler does not fuse floor + convert into v_cvt_flr_i32_f32 by itself.
 // y = c * 2^18 (the caller pre-scales its per-particle factors, which is exact): the
 // fixed-point value c * 2^50 = hi * 2^32 + lo with hi = floor(y), lo = frac(y) * 2^32.
 __device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float y) {
-  const float fl = floorf(y);
-  const unsigned lo = (unsigned)((y - fl) * 4294967296.0f);
-  const int hi = (int)fl;
+  const unsigned lo = (unsigned)(__builtin_amdgcn_fractf(y) * 4294967296.0f);
+  int hi;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(y));
   atomicAdd(slot, ((unsigned long long)(unsigned)hi << 32) | lo);
 }
 
@@ -607,20 +608,20 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
         }
       }
       // Cell-sorted particles of one base cell sit in adjacent lanes (water: 4 per cell).
-      // Runs of adjacent lanes with the same cell, within a DPP row of 16 lanes, add their
-      // 27 contributions across lanes (row_shl:1, then row_shl:2 when the lane two along
-      // is in the same run), so the first lane of every group of up to 4 in a run holds the
-      // group's sums and alone issues the 27 64-bit LDS atomics: about a quarter of the
-      // atomics of one set per particle, in a fixed order (deterministic). Lanes outside
-      // the fixed-point LDS path carry unique keys and add nothing.
+      // Pairs of adjacent lanes with the same cell, within a DPP row of 16 lanes, add their
+      // 27 contributions across lanes (row_shl:1), so the first lane of every pair in a run
+      // holds the pair's sums and alone issues the 27 64-bit LDS atomics: half the atomics
+      // of one set per particle, in a fixed order (deterministic). Lanes outside the
+      // fixed-point LDS path carry unique keys and add nothing. (A second step over groups
+      // of 4 halves the atomics again but costs more VALU than it saves: measured 49.5 us
+      // against 48.0 us per substep at 32 x 80K.)
       const int key = fixed_ok ? (b1 * nx + b0 + 1) : -(1 + lane);
       const int k1 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x101, 0xF, 0xF, false);
-      const int k2 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x102, 0xF, 0xF, false);
       const int kp = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x111, 0xF, 0xF, false);
-      const bool same1 = k1 == key, same12 = same1 && (k2 == key);
+      const bool same1 = k1 == key;
       const unsigned long long heads = __ballot(kp != key);
       const int runstart = 63 - __clzll(heads & lanes_le);
-      const bool emit = fixed_ok && (((lane - runstart) & 3) == 0);
+      const bool emit = fixed_ok && (((lane - runstart) & 1) == 0);
       const int q0 = (b1 - mn1) * ww + (b0 - mn0);
 #pragma unroll
       for (int jj = 0; jj < 3; ++jj) {
@@ -632,10 +633,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
           for (int e = 0; e < 3; ++e) {
             const float t1 = __int_as_float(__builtin_amdgcn_update_dpp(
                 0, __float_as_int(c[e]), 0x101, 0xF, 0xF, false));
-            const float c1 = same1 ? c[e] + t1 : c[e];
-            const float t2 = __int_as_float(__builtin_amdgcn_update_dpp(
-                0, __float_as_int(c1), 0x102, 0xF, 0xF, false));
-            c[e] = same12 ? c1 + t2 : c1;
+            c[e] = same1 ? c[e] + t1 : c[e];
           }
           if (emit) {
             const int q = q0 + jj * ww + ii;
