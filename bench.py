@@ -236,6 +236,8 @@ def main() -> int:
                     help="also time the batched tall-tank forward + backward (~1 minute)")
     ap.add_argument("--only-diff", action="store_true",
                     help="run only the differentiable legs (no throughput line fields)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched tall-tank forward + backward")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -249,6 +251,9 @@ def main() -> int:
     from jellyfih_amd.engine import Engine
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.only_batched:
+        print(json.dumps({"diff": {"batched": diff_batched()}}), flush=True)
+        return 0
     if args.only_diff:
         out = {"diff": diff_episode()}
         if args.diff_batched:

@@ -196,13 +196,13 @@ int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const void* state_in, dou
 
 /* Reverse pass: traj (nsteps+1,P,12) holds the states at substeps 0..nsteps and grids the
  * nsteps grids, as jm_fwd_block stores them (state_in, then out, from t0); grad_out (P,12)
- * is the adjoint of the last state. Writes grad_in (P,12), the adjoint of the first, and
- * accumulates grad_amp (P,) and grad_pp (P,4) = per-particle shares of the adjoints of
- * (E_jelly, instance_actuation, contraction, freq), to be summed per instance. grad_out
- * and grad_in may not alias. */
+ * is the adjoint of the last state. Writes grad_in (P,12), the adjoint of the first;
+ * accumulates grad_amp (P,), the adjoint of amp_scale, and grad_params (M,4, always fp64),
+ * the adjoints of (E_jelly, instance_actuation, contraction, freq). grad_out and grad_in
+ * may not alias. */
 int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const void* grids,
                  double t0, int32_t nsteps, const void* grad_out, void* grad_in,
-                 void* grad_amp, void* grad_pp, void* stream);
+                 void* grad_amp, double* grad_params, void* stream);
 
 /* ti.sync(): wait for all queued work. */
 int jm_sync(jm_handle* h);

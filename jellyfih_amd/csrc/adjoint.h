@@ -400,26 +400,6 @@ JMD_HD void gridop_forward(T px, T py, T m, int i, int j, const Consts<T>& k, T&
   if (j > k.ny - 3 && g1 > T(0)) g1 = T(0);
 }
 
-// Adjoint of gridop_forward: from the velocity adjoint (gb0, gb1) to the adjoints of the
-// momentum (pb0, pb1) and mass (mb).
-template <typename T>
-JMD_HD void gridop_backward(T px, T py, T m, int i, int j, const Consts<T>& k, T gb0, T gb1,
-                            T& pb0, T& pb1, T& mb) {
-  if (!(m > T(0))) {
-    pb0 = gb0;
-    pb1 = gb1;
-    mb = T(0);
-    return;
-  }
-  T damp = damping(i, j, k);
-  T v0 = (px / m) * damp, v1 = (py / m) * damp;
-  if ((i < 3 && v0 < T(0)) || (i > k.nx - 3 && v0 > T(0))) gb0 = T(0);
-  if ((j < 3 && v1 < T(0)) || (j > k.ny - 3 && v1 > T(0))) gb1 = T(0);
-  pb0 = gb0 * damp / m;
-  pb1 = gb1 * damp / m;
-  mb = -(gb0 * v0 + gb1 * v1) / m;
-}
-
 // ---------------------------------------------------------------------------------------
 // Per-particle transfer loops. State layout per particle (12 values): x0 x1 v0 v1 C00 C01
 // C10 C11 F00 F01 F10 F11. Grids of one instance: three planes (momentum x, momentum y,
@@ -478,11 +458,41 @@ JMD_HD void node_velocity(const T* gp0, const T* gp1, const T* gm, int i, int j,
   gridop_forward(gp0[node], gp1[node], gm[node], i, j, k, a0, a1);
 }
 
-// G2P + gravity + advection + clamp of one particle (mpm_sim.py:311-333), the grid op
-// (mpm_sim.py:274-309) evaluated at each stencil node from momentum gp0, gp1 and mass gm.
+// What the G2P adjoint needs of one node: its velocity after the grid op (v0, v1) and the
+// grid op's adjoint as a linear map from the velocity adjoint (g0, g1) to the momentum and
+// mass adjoints: q0 = A0 g0, q1 = A1 g1, qm = B0 g0 + B1 g1 (with the wall-clamp masks
+// folded into A and B: the clamped components carry no adjoint).
 template <typename T>
-JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* gp0, const T* gp1,
-                         const T* gm) {
+struct NodeCoef {
+  T v0, v1, A0, A1, B0, B1;
+};
+
+template <typename T>
+JMD_HD NodeCoef<T> node_coef(T px, T py, T m, int i, int j, const Consts<T>& k) {
+  NodeCoef<T> c;
+  gridop_forward(px, py, m, i, j, k, c.v0, c.v1);
+  if (!(m > T(0))) {
+    c.A0 = T(1);
+    c.A1 = T(1);
+    c.B0 = T(0);
+    c.B1 = T(0);
+    return c;
+  }
+  const T damp = damping(i, j, k);
+  const T u0 = (px / m) * damp, u1 = (py / m) * damp;  // before the wall clamps
+  const bool k0 = !((i < 3 && u0 < T(0)) || (i > k.nx - 3 && u0 > T(0)));
+  const bool k1 = !((j < 3 && u1 < T(0)) || (j > k.ny - 3 && u1 > T(0)));
+  c.A0 = k0 ? damp / m : T(0);
+  c.A1 = k1 ? damp / m : T(0);
+  c.B0 = k0 ? -u0 / m : T(0);
+  c.B1 = k1 ? -u1 / m : T(0);
+  return c;
+}
+
+// G2P + gravity + advection + clamp of one particle (mpm_sim.py:311-333). `vel(i, j, a0, a1)`
+// gives the velocity of node (i, j) after the grid op (mpm_sim.py:274-309).
+template <typename T, typename Vel>
+JMD_HD void g2p_particle_v(PState<T>& p, const Consts<T>& k, Vel&& vel) {
   int b0, b1;
   T fx0, fx1;
   if (base_cell(p.x0, p.x1, k, b0, b1, fx0, fx1)) {
@@ -494,7 +504,7 @@ JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* gp0, const T
     for (int ii = 0; ii < 3; ++ii)
       for (int jj = 0; jj < 3; ++jj) {
         T a0, a1;
-        node_velocity(gp0, gp1, gm, b0 + ii, b1 + jj, k, a0, a1);
+        vel(b0 + ii, b1 + jj, a0, a1);
         T w = wx[ii] * wy[jj];
         T e0 = T(ii) - fx0, e1 = T(jj) - fx1;
         v0 += w * a0;
@@ -516,22 +526,29 @@ JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* gp0, const T
   p.x1 = p.x1 < k.x_lo ? k.x_lo : (p.x1 > k.y_hi ? k.y_hi : p.x1);
 }
 
+// The same, with the grid op evaluated per stencil node from momentum gp0, gp1, mass gm.
+template <typename T>
+JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* gp0, const T* gp1,
+                         const T* gm) {
+  g2p_particle_v(p, k, [&](int i, int j, T& a0, T& a1) {
+    node_velocity(gp0, gp1, gm, i, j, k, a0, a1);
+  });
+}
+
 // Adjoint of g2p_particle. `s` is the particle's state before the substep (its x sets the
 // weights; v, C pass through when out of range), `b_out` the adjoint of the state after.
-// Accumulates the x adjoint and the pass-through v, C adjoints into `b_in`. The adjoint of
-// each stencil node's velocity, taken through that node's grid op (linear in it), goes to
-// `emit(i, j, momentum_x_adj, momentum_y_adj, mass_adj)`.
-template <typename T, typename Emit>
-JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
-                         const T* gp0, const T* gp1, const T* gm, PState<T>& b_in,
-                         Emit&& emit) {
+// Accumulates the x adjoint and the pass-through v, C adjoints into `b_in`. `coef(i, j)`
+// gives node (i, j)'s NodeCoef; the adjoint of each stencil node's velocity, taken through
+// that node's grid op, goes to `emit(i, j, momentum_x_adj, momentum_y_adj, mass_adj)`.
+template <typename T, typename Coef, typename Emit>
+JMD_HD void g2p_backward_c(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
+                           Coef&& coef, PState<T>& b_in, Emit&& emit) {
   // forward to the clamp, to know which components were clamped
   T v0n = s.v0, v1n = s.v1;
   int b0, b1;
   T fx0, fx1;
   const bool inr = base_cell(s.x0, s.x1, k, b0, b1, fx0, fx1);
   T wx[3], wy[3], dwx[3], dwy[3];
-  T a0s[9], a1s[9];
   if (inr) {
     weights(fx0, wx, dwx);
     weights(fx1, wy, dwy);
@@ -539,10 +556,10 @@ JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Const
     v1n = T(0);
     for (int ii = 0; ii < 3; ++ii)
       for (int jj = 0; jj < 3; ++jj) {
-        node_velocity(gp0, gp1, gm, b0 + ii, b1 + jj, k, a0s[3 * ii + jj], a1s[3 * ii + jj]);
+        const NodeCoef<T> c = coef(b0 + ii, b1 + jj);
         T w = wx[ii] * wy[jj];
-        v0n += w * a0s[3 * ii + jj];
-        v1n += w * a1s[3 * ii + jj];
+        v0n += w * c.v0;
+        v1n += w * c.v1;
       }
   }
   v1n = v1n - k.gravity_dt;
@@ -563,16 +580,15 @@ JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Const
   T f0 = T(0), f1 = T(0);  // d L / d fx
   for (int ii = 0; ii < 3; ++ii)
     for (int jj = 0; jj < 3; ++jj) {
-      const int i = b0 + ii, j = b1 + jj, node = j * k.nx + i;
+      const int i = b0 + ii, j = b1 + jj;
+      const NodeCoef<T> c = coef(i, j);
       T w = wx[ii] * wy[jj];
       T e0 = T(ii) - fx0, e1 = T(jj) - fx1;
-      T a0 = a0s[3 * ii + jj], a1 = a1s[3 * ii + jj];
+      const T a0 = c.v0, a1 = c.v1;
       // L_node = vb . g + apic g^T Cb e
       T ce0 = Cb.a * e0 + Cb.b * e1, ce1 = Cb.c * e0 + Cb.d * e1;  // Cb e
-      T q0, q1, qm;
-      gridop_backward(gp0[node], gp1[node], gm[node], i, j, k, w * vb0 + k.apic_scale * w * ce0,
-                      w * vb1 + k.apic_scale * w * ce1, q0, q1, qm);
-      emit(i, j, q0, q1, qm);
+      const T g0 = w * vb0 + k.apic_scale * w * ce0, g1 = w * vb1 + k.apic_scale * w * ce1;
+      emit(i, j, c.A0 * g0, c.A1 * g1, c.B0 * g0 + c.B1 * g1);
       T lw = vb0 * a0 + vb1 * a1 + k.apic_scale * (a0 * ce0 + a1 * ce1);
       f0 += dwx[ii] * wy[jj] * lw;
       f1 += wx[ii] * dwy[jj] * lw;
@@ -582,6 +598,20 @@ JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Const
     }
   b_in.x0 += k.inv_dx * f0;
   b_in.x1 += k.inv_dx * f1;
+}
+
+// The same, with each node's NodeCoef computed from momentum gp0, gp1 and mass gm.
+template <typename T, typename Emit>
+JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
+                         const T* gp0, const T* gp1, const T* gm, PState<T>& b_in,
+                         Emit&& emit) {
+  g2p_backward_c(
+      s, b_out, k,
+      [&](int i, int j) {
+        const int node = j * k.nx + i;
+        return node_coef(gp0[node], gp1[node], gm[node], i, j, k);
+      },
+      b_in, emit);
 }
 
 // Adjoint of p2g_particle (gather from the momentum/mass adjoints pb0, pb1, mb). `b_out`

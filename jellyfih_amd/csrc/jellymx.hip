@@ -1105,12 +1105,12 @@ struct jm_diff {
   int bits = 32;                 // scalar type of states, grids and parameters
   jmd::Consts<float> kf{};
   jmd::Consts<double> kd{};
-  void* grid = nullptr;   // 2 * M * 3 * G scalars (forward ping-pong)
-  void* agrid = nullptr;  // 2 * M * 3 * G scalars (adjoint ping-pong)
-  void* sbuf = nullptr;   // 2 * P * 12 scalars (forward ping-pong)
-  void* gbuf = nullptr;   // 3 * P * 12 scalars (adjoint ping-pong + partial)
+  void* grid = nullptr;   // 3 * M * 3 * G scalars (forward grid ring)
+  void* agrid = nullptr;  // 3 * M * 3 * G scalars (adjoint grid ring)
+  void* sbuf = nullptr;   // 2 * P * 12 scalars (forward states, ping-pong)
+  void* pbuf = nullptr;   // 32 * P scalars: F_new (2 x 4), adjoint m (2 x 8), adjoint F (2 x 4)
   ~jm_diff() {
-    for (void* p : {grid, agrid, sbuf, gbuf})
+    for (void* p : {grid, agrid, sbuf, pbuf})
       if (p) (void)hipFree(p);
   }
 };
@@ -1180,65 +1180,91 @@ int fwd_block(jm_diff* d, const jm_diff_inputs* in, const T* state_in, double t0
               int nsteps, T* out, T* grid_out, hipStream_t st) {
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
   const jmd::Consts<T>& k = consts_of<T>(d);
-  const int P = (int)d->P, G = d->G;
+  const int P = (int)d->P, G = d->G, n = nsteps;
   const size_t S = (size_t)P * 12;
   const int64_t MG = (int64_t)d->M * 3 * G;  // one substep's grids
   const dim3 pg(blocks_for(P, jmd::kDiffBlock)), tb(jmd::kDiffBlock);
-  T* grid = static_cast<T*>(d->grid);
+  T* ring = static_cast<T*>(d->grid);
   T* sbuf = static_cast<T*>(d->sbuf);
+  T* fnew[2] = {static_cast<T*>(d->pbuf), static_cast<T*>(d->pbuf) + 4 * (size_t)P};
   const bool store = grid_out != nullptr;
+  // grid of substep s; state s (s >= 1) as the kernel that finishes substep s-1 writes it
+  auto grid = [&](int s) { return store ? grid_out + (size_t)s * MG : ring + (size_t)(s % 3) * MG; };
+  auto state = [&](int s) -> T* {
+    if (store) return out + (size_t)(s - 1) * S;
+    return s == n ? out : sbuf + (size_t)(s % 2) * S;
+  };
   if (store)
-    hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG * nsteps, 256 * 8)), dim3(256), 0,
-                       st, grid_out, MG * nsteps);
+    hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG * n, 256 * 8)), dim3(256), 0, st,
+                       grid_out, MG * n);
   else
-    hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st, grid,
+    hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st, ring,
                        MG);
   T t = T(t0_in);
-  const T* cur = state_in;
-  for (int s = 0; s < nsteps; ++s) {
-    T* nxt = store ? out + (size_t)s * S : (s == nsteps - 1 ? out : sbuf + (size_t)(s % 2) * S);
-    T* g = store ? grid_out + (size_t)s * MG : grid + (size_t)(s % 2) * MG;
-    T* zero = (!store && s + 1 < nsteps) ? grid + (size_t)((s + 1) % 2) * MG : nullptr;
-    hipLaunchKernelGGL(jmd::k_d_p2g<T>, pg, tb, 0, st, din, cur, nxt, g, P, G, t, k);
-    hipLaunchKernelGGL(jmd::k_d_g2p<T>, pg, tb, 0, st, din, cur, nxt, (const T*)g, zero, MG, P,
-                       G, k);
-    cur = nxt;
+  // K_f(0): P2G of substep 0
+  hipLaunchKernelGGL((jmd::k_d_fwd<T, false, true>), pg, tb, 0, st, din, state_in,
+                     (const T*)nullptr, (const T*)nullptr, (T*)nullptr, fnew[0], grid(0),
+                     (!store && n > 1) ? ring + (size_t)MG : (T*)nullptr, MG, P, G, t, k);
+  for (int s = 1; s < n; ++s) {
     t = t + k.dt;  // mpm_sim.py:340 (fp32 in the reference)
+    // K_f(s): G2P of substep s-1, P2G of substep s
+    hipLaunchKernelGGL((jmd::k_d_fwd<T, true, true>), pg, tb, 0, st, din,
+                       (const T*)(s == 1 ? state_in : state(s - 1)), (const T*)fnew[(s - 1) % 2],
+                       (const T*)grid(s - 1), state(s), fnew[s % 2], grid(s),
+                       (!store && s + 1 < n) ? ring + (size_t)((s + 1) % 3) * MG : (T*)nullptr,
+                       MG, P, G, t, k);
   }
+  // K_f(n): G2P of substep n-1
+  hipLaunchKernelGGL((jmd::k_d_fwd<T, true, false>), pg, tb, 0, st, din,
+                     (const T*)(n == 1 ? state_in : state(n - 1)), (const T*)fnew[(n - 1) % 2],
+                     (const T*)grid(n - 1), state(n), (T*)nullptr, (T*)nullptr, (T*)nullptr,
+                     MG, P, G, t, k);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
 
 template <typename T>
 int bwd_block(jm_diff* d, const jm_diff_inputs* in, const T* traj, const T* grids,
-              double t0_in, int nsteps, const T* grad_out, T* grad_in, T* grad_amp, T* grad_pp,
-              hipStream_t st) {
+              double t0_in, int nsteps, const T* grad_out, T* grad_in, T* grad_amp,
+              double* grad_params, hipStream_t st) {
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
   const jmd::Consts<T>& k = consts_of<T>(d);
-  const int P = (int)d->P, G = d->G;
+  const int P = (int)d->P, G = d->G, n = nsteps;
   const size_t S = (size_t)P * 12;
   const int64_t MG = (int64_t)d->M * 3 * G;
   const dim3 pg(blocks_for(P, jmd::kDiffBlock)), tb(jmd::kDiffBlock);
-  T* agrid = static_cast<T*>(d->agrid);
-  T* gbuf = static_cast<T*>(d->gbuf);
-  std::vector<T> times(nsteps + 1);
+  T* ring = static_cast<T*>(d->agrid);
+  T* pb = static_cast<T*>(d->pbuf);
+  T* mb[2] = {pb + 8 * (size_t)P, pb + 16 * (size_t)P};     // m_s, by s % 2
+  T* aF[2] = {pb + 24 * (size_t)P, pb + 28 * (size_t)P};    // F part of a_s, by s % 2
+  auto agrid = [&](int s) { return ring + (size_t)(s % 3) * MG; };
+  std::vector<T> times(n + 1);
   times[0] = T(t0_in);
-  for (int s = 0; s < nsteps; ++s) times[s + 1] = times[s] + k.dt;
-  T* mid = gbuf + 2 * S;
+  for (int s = 0; s < n; ++s) times[s + 1] = times[s] + k.dt;
   hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st,
-                     agrid + (size_t)((nsteps - 1) % 2) * MG, MG);
-  const T* bout = grad_out;
-  for (int s = nsteps - 1; s >= 0; --s) {
-    const T* cur = traj + (size_t)s * S;
-    T* a = agrid + (size_t)(s % 2) * MG;
-    T* bin = s == 0 ? grad_in : gbuf + (size_t)(s % 2) * S;
-    T* zero = s > 0 ? agrid + (size_t)((s + 1) % 2) * MG : nullptr;
-    hipLaunchKernelGGL(jmd::k_d_g2p_bwd<T>, pg, tb, 0, st, din, cur, bout, mid,
-                       grids + (size_t)s * MG, a, P, G, k);
-    hipLaunchKernelGGL(jmd::k_d_p2g_bwd<T>, pg, tb, 0, st, din, cur, bout, (const T*)mid, bin,
-                       (const T*)a, zero, MG, P, G, times[s], k, grad_amp, grad_pp);
-    bout = bin;
+                     agrid(n - 1), MG);
+  // K_b(n): G2P adjoint of substep n-1, driven by grad_out
+  hipLaunchKernelGGL((jmd::k_d_bwd<T, false, true>), pg, tb, 0, st, din, (const T*)nullptr,
+                     traj + (size_t)(n - 1) * S, (const T*)nullptr, 4, (const T*)nullptr,
+                     (const T*)nullptr, grids + (size_t)(n - 1) * MG, agrid(n - 1), grad_out,
+                     mb[(n - 1) % 2], aF[n % 2], (T*)nullptr,
+                     n >= 2 ? agrid(n + 1) : (T*)nullptr, MG, P, G, times[n], k, grad_amp,
+                     grad_params);
+  for (int s = n - 1; s >= 1; --s) {
+    // K_b(s): P2G adjoint of substep s, G2P adjoint of substep s-1
+    hipLaunchKernelGGL((jmd::k_d_bwd<T, true, true>), pg, tb, 0, st, din, traj + (size_t)s * S,
+                       traj + (size_t)(s - 1) * S, (const T*)aF[(s + 1) % 2], 4,
+                       (const T*)mb[s % 2], (const T*)agrid(s), grids + (size_t)(s - 1) * MG,
+                       agrid(s - 1), (const T*)nullptr, mb[(s - 1) % 2], aF[s % 2],
+                       (T*)nullptr, s >= 2 ? agrid(s + 1) : (T*)nullptr, MG, P, G, times[s],
+                       k, grad_amp, grad_params);
   }
+  // K_b(0): P2G adjoint of substep 0, writing grad_in
+  hipLaunchKernelGGL((jmd::k_d_bwd<T, true, false>), pg, tb, 0, st, din, traj,
+                     (const T*)nullptr, (const T*)aF[1], 4, (const T*)mb[0],
+                     (const T*)agrid(0), (const T*)nullptr, (T*)nullptr, (const T*)nullptr,
+                     (T*)nullptr, (T*)nullptr, grad_in, (T*)nullptr, MG, P, G, times[0], k,
+                     grad_amp, grad_params);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
@@ -1269,11 +1295,11 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
   d->kf = diff_consts<float>(*cfg, nx, ny);
   d->kd = diff_consts<double>(*cfg, nx, ny);
   const size_t es = scalar_bits / 8;
-  const size_t gbytes = 2 * (size_t)d->M * 3 * d->G * es;
+  const size_t gbytes = 3 * (size_t)d->M * 3 * d->G * es;
   const size_t sbytes = (size_t)d->P * 12 * es;
   if (hipMalloc(&d->grid, gbytes) != hipSuccess || hipMalloc(&d->agrid, gbytes) != hipSuccess ||
       hipMalloc(&d->sbuf, 2 * sbytes) != hipSuccess ||
-      hipMalloc(&d->gbuf, 3 * sbytes) != hipSuccess) {
+      hipMalloc(&d->pbuf, 32 * (size_t)d->P * es) != hipSuccess) {
     delete d;
     return fail(JM_ERR_NOMEM, "hipMalloc(diff buffers)");
   }
@@ -1300,20 +1326,20 @@ int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const void* state_in, dou
 
 int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const void* grids,
                  double t0, int32_t nsteps, const void* grad_out, void* grad_in, void* grad_amp,
-                 void* grad_pp, void* stream) {
+                 double* grad_params, void* stream) {
   JM_TRY(diff_check(d, in));
-  if (!traj || !grids || !grad_out || !grad_in || !grad_amp || !grad_pp || nsteps < 1)
+  if (!traj || !grids || !grad_out || !grad_in || !grad_amp || !grad_params || nsteps < 1)
     return fail(JM_ERR_ARG, "bad backward arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d->bits == 64)
     return bwd_block<double>(d, in, static_cast<const double*>(traj),
                              static_cast<const double*>(grids), t0, nsteps,
                              static_cast<const double*>(grad_out), static_cast<double*>(grad_in),
-                             static_cast<double*>(grad_amp), static_cast<double*>(grad_pp), st);
+                             static_cast<double*>(grad_amp), grad_params, st);
   return bwd_block<float>(d, in, static_cast<const float*>(traj),
                           static_cast<const float*>(grids), t0, nsteps,
                           static_cast<const float*>(grad_out), static_cast<float*>(grad_in),
-                          static_cast<float*>(grad_amp), static_cast<float*>(grad_pp), st);
+                          static_cast<float*>(grad_amp), grad_params, st);
 }
 
 }  // extern "C"

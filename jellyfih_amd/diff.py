@@ -220,17 +220,16 @@ class DiffTank:
         grad_out = grad_out.to(self.dtype).contiguous()
         grad_in = torch.empty_like(state)
         gamp = torch.zeros(self.P, dtype=self.dtype, device=self.device)
-        gpp = torch.zeros((self.P, 4), dtype=self.dtype, device=self.device)
+        gparams = torch.zeros((self.M, 4), dtype=torch.float64, device=self.device)
         ins = self._inputs(amp_scale, params)
         call("jm_bwd_block", self._h, ctypes.byref(ins), _ptr(traj), _ptr(grids),
-             ctypes.c_double(t0), int(n), _ptr(grad_out), _ptr(grad_in), _ptr(gamp), _ptr(gpp),
-             _stream())
-        per_inst = self.instance_sums(gpp)
+             ctypes.c_double(t0), int(n), _ptr(grad_out), _ptr(grad_in), _ptr(gamp),
+             _ptr(gparams), _stream())
         gpar = torch.zeros((self.M, 5), dtype=self.dtype, device=self.device)
-        gpar[:, 4] = per_inst[:, 0]    # E_jelly
-        gpar[:, 0] = per_inst[:, 1]    # amplitude
-        gpar[:, 1] = per_inst[:, 2]    # contraction
-        gpar[:, 2] = per_inst[:, 3]    # freq
+        gpar[:, 4] = gparams[:, 0]    # E_jelly
+        gpar[:, 0] = gparams[:, 1]    # amplitude
+        gpar[:, 1] = gparams[:, 2]    # contraction
+        gpar[:, 2] = gparams[:, 3]    # freq
         return grad_in, gamp, gpar
 
     # -- autograd ---------------------------------------------------------------------
