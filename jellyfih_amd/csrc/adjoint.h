@@ -232,6 +232,35 @@ JMD_HD void clamp_sigma(int mat, const Svd<T>& sv, const Consts<T>& k, T& n0, T&
   }
 }
 
+// Water's J without the Jacobi step. With the polar rotation R of taichi's `_svd2d`
+// (c, s) = normalize(F00 + F11, F10 - F01) and S = R^T F, tr S = |(F00 + F11, F10 - F01)|
+// >= 0; when det F > 1e-6 |F|_F (> 0) both singular values exceed the 1e-6 floor, so the
+// clamped product is the unclamped one: S00 * S11 in the diagonal branch (|S01| < 1e-5),
+// det S = det F otherwise, equal to the SVD path up to rounding. Returns false when the
+// fast form does not apply (the general path then runs).
+template <typename T>
+struct WaterJ {
+  T J, c, s, S00, S11;
+  bool diag;
+};
+
+template <typename T>
+JMD_HD bool water_j(const Mat<T>& F, WaterJ<T>& w) {
+  const T det = F.a * F.d - F.b * F.c;
+  const T fro2 = F.a * F.a + F.b * F.b + F.c * F.c + F.d * F.d;
+  if (!(det > T(0) && det * det > T(1e-12) * fro2)) return false;
+  const T x = F.a + F.d, y = F.c - F.b;
+  const T scale = T(1) / sqrt(x * x + y * y);
+  w.c = x * scale;
+  w.s = y * scale;
+  const T S01 = w.c * F.b + w.s * F.d;
+  w.diag = fabs(S01) < T(1e-5);
+  w.S00 = w.c * F.a + w.s * F.c;
+  w.S11 = -w.s * F.b + w.c * F.d;
+  w.J = w.diag ? w.S00 * w.S11 : det;
+  return true;
+}
+
 // Forward constitutive update of one in-range particle: new F and the affine term
 // A = stress_scale * tau + mass * C (mpm_sim.py:189-254), with J the product of the clamped
 // singular values of taichi's SVD as in the reference. `beta` = activation *
@@ -243,6 +272,14 @@ JMD_HD void p2g_forward(int mat, const Mat<T>& F0, const Mat<T>& C, T d0, T d1, 
   Mat<T> F = mmul(M, F0);
   T mu, la;
   material(mat, mu_j, la_j, pdens, k, mu, la, mass);
+  WaterJ<T> wj;
+  if (mat == 0 && water_j(F, wj)) {
+    // mu = 0: tau = la J (J - 1) I, F <- sqrt(J) I
+    const T r = sqrt(wj.J), dg = (la * wj.J) * (wj.J - T(1));
+    Fn = {r, T(0), T(0), r};
+    A = madd(mscale(k.stress_scale, Mat<T>{dg, T(0), T(0), dg}), mscale(mass, C));
+    return;
+  }
   Svd<T> sv = svd2d(F);
   T n0, n1, g0, g1;
   clamp_sigma(mat, sv, k, n0, n1, g0, g1);
@@ -282,6 +319,31 @@ JMD_HD void p2g_backward(int mat, const Mat<T>& F0, const Mat<T>& C, T d0, T d1,
   Mat<T> F = mmul(M, F0);
   T mu, la, mass;
   material(mat, mu_j, la_j, pdens, k, mu, la, mass);
+  WaterJ<T> wj;
+  if (mat == 0 && water_j(F, wj)) {
+    // A = s la J (J - 1) I + mass C, Fn = sqrt(J) I
+    const Mat<T> taub = mscale(k.stress_scale, Ab);
+    Cb = madd(Cb, mscale(mass, Ab));
+    const T J = wj.J;
+    const T Jb = la * (T(2) * J - T(1)) * (taub.a + taub.d) + (Fnb.a + Fnb.d) / (T(2) * sqrt(J));
+    Mat<T> Fb;
+    if (wj.diag) {
+      // J = S00 S11 with S = R^T F: F_b += R S_b, R_b += F S_b^T, then the polar angle
+      const Mat<T> R = {wj.c, -wj.s, wj.s, wj.c};
+      const Mat<T> Sb = {Jb * wj.S11, T(0), T(0), Jb * wj.S00};
+      Fb = mmul(R, Sb);
+      const Mat<T> Rb = mmul_bt(F, Sb);
+      const Mat<T> RK = {R.b, -R.a, R.d, -R.c};
+      const T thb = mdot(Rb, RK);
+      const T x = F.a + F.d, y = F.c - F.b;
+      Fb = madd(Fb, mscale(thb / (x * x + y * y), Mat<T>{-y, -x, x, -y}));
+    } else {
+      Fb = mscale(Jb, mcof(F));  // d det F / dF
+    }
+    F0b = madd(F0b, mmul_at(M, Fb));
+    Cb = madd(Cb, mscale(k.dt, mmul_bt(Fb, F0)));
+    return;
+  }
   Svd<T> sv = svd2d(F);
   const Mat<T>& R = sv.R;
   T n0, n1, g0, g1;

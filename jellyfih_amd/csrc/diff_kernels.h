@@ -12,8 +12,11 @@
 // stencil nodes when the workgroup's particles are of one instance and close together
 // (particles packed instance by instance in spatial order, as tanks are filled), then add
 // the tile to the grid with one global atomic per touched node and plane; otherwise they
-// use global atomics directly. The tile is fp64 for both scalar types: gfx950's fp64 LDS
-// atomic add is much faster than its fp32 one, and the sums only gain precision. Every
+// use global atomics directly. Adjacent lanes of one DPP row whose particles share a base
+// cell add their contributions pairwise first, so only one lane of a pair issues atomics.
+// The fp32 P2G tile accumulates in 64-bit fixed point (exact); the fp64 P2G tile and the
+// adjoint tile (whose values have no fixed range) in fp64 (gfx950's fp64 LDS atomic add is
+// much faster than its fp32 one). Every
 // grid a kernel accumulates into starts zeroed: grids rotate through three buffers and
 // each kernel clears the one the kernel after next accumulates into (grid-stride), or a
 // fill kernel clears them at the start of a block.
@@ -26,7 +29,7 @@
 namespace jmd {
 
 constexpr int kDiffBlock = 256;   // threads per workgroup, one particle each
-constexpr int kTileNodes = 2048;  // forward LDS window/tile: 3 planes x 2048 nodes of fp64
+constexpr int kTileNodes = 1024;  // forward LDS window/tile: 3 planes x 1024 nodes of fp64 (24 KB)
 constexpr int kWinB = 1024;       // adjoint LDS window: 6 staged + 3 accumulated per node
 
 template <typename T>
@@ -139,6 +142,70 @@ struct GlobalEmit {
   }
 };
 
+// ---- cross-lane pre-sums and fixed point ------------------------------------------------
+
+// Pairs of adjacent lanes (within a DPP row of 16) whose particles share a base cell (keys
+// equal; lanes that scatter nothing carry unique keys): the first lane of each pair in a
+// run of equal keys adds its neighbour's contributions and alone issues the LDS atomics.
+// Called by every lane of the wave.
+struct LanePairs {
+  bool same_next, head;
+};
+
+__device__ __forceinline__ LanePairs lane_pairs(int key) {
+  const int lane = threadIdx.x & 63;
+  const int kn = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x101, 0xF, 0xF, false);  // lane+1
+  const int kp = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x111, 0xF, 0xF, false);  // lane-1
+  const unsigned long long heads = __ballot(kp != key);
+  const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int runstart = 63 - __clzll(heads & le);
+  return {kn == key, ((lane - runstart) & 1) == 0};
+}
+
+// The value held by lane + 1 of the same DPP row (0 past the row's end).
+__device__ __forceinline__ float next_lane(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double next_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x101, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x101, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// 64-bit fixed point for the fp32 kernels' P2G tile: c * 2^50 as a two's-complement int64
+// for |c| < 1, hi = floor(c 2^18), lo = frac(c 2^18) 2^32: exact up to the truncation of
+// lo, below one unit (finer than fp32 rounding of any contribution above 2^-26). A node of
+// a workgroup's tile receives at most 256 contributions (one per particle), so its sum
+// stays below 2^58 units: no overflow. Larger contributions (and NaN) go to the global
+// grid. The fp64 kernels accumulate in fp64 instead (fixed point would be coarser than
+// fp64 rounding).
+__device__ __forceinline__ unsigned long long to_fixed(float c) {
+  const float y = c * 262144.0f;                   // 2^18
+  const unsigned lo = (unsigned)(__builtin_amdgcn_fractf(y) * 4294967296.0f);
+  int hi;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(y));
+  return ((unsigned long long)(unsigned)hi << 32) | lo;
+}
+__device__ __forceinline__ float from_fixed(unsigned long long v) {
+  return float((double)(long long)v * (1.0 / 1125899906842624.0));  // 2^-50
+}
+
+// Adds a 3-plane LDS tile of fixed-point sums into the instance's grid planes.
+__device__ __forceinline__ void tile_flush_fixed(const unsigned long long* acc, const Tile& t,
+                                                 float* g, int nx, int G) {
+  const int n = t.tw * t.th;
+  for (int li = threadIdx.x; li < n; li += kDiffBlock) {
+    const int jj = li / t.tw;
+    const int node = (t.oy + jj) * nx + t.ox + (li - jj * t.tw);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = from_fixed(acc[c * n + li]);
+      if (v != 0.0f) unsafeAtomicAdd(g + (size_t)c * G + node, v);
+    }
+  }
+}
+
 // ---- forward ---------------------------------------------------------------------------
 //
 // One kernel per substep boundary: K_f(s) finishes substep s-1 (G2P, with the grid op per
@@ -223,26 +290,57 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
       as = in.amp_scale[p];
     }
     const int n = tl.tw * tl.th;
-    const GlobalEmit<T> direct{grid_cur + (size_t)m * 3 * G, k.nx, G};
+    T* gi = grid_cur + (size_t)m * 3 * G;
+    const GlobalEmit<T> direct{gi, k.nx, G};
+    // fp32: 64-bit fixed point in LDS (exact, order-independent sums), a contribution
+    // outside the fixed-point range (or NaN) going to the global grid directly; fp64: fp64
+    constexpr bool kFixed = sizeof(T) == 4;
+    unsigned long long* accf = reinterpret_cast<unsigned long long*>(lds);
+    const LanePairs pr = lane_pairs(active && tl.use ? b1 * k.nx + b0 + 1 : -(1 + (int)threadIdx.x));
     if (tl.use) {
-      for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) lds[i] = 0.0;
+      for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) {
+        if (kFixed)
+          accf[i] = 0ull;
+        else
+          lds[i] = 0.0;
+      }
       __syncthreads();
     }
     if (active)
       p2g_particle(s, mt, d0, d1, act.a * amp * as, mu, la, pdens, k,
                    [&](int i, int j, T c0, T c1, T c2) {
                      if (tl.use) {
-                       const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-                       atomicAdd(&lds[li], double(c0));
-                       atomicAdd(&lds[n + li], double(c1));
-                       atomicAdd(&lds[2 * n + li], double(c2));
+                       const T n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
+                       if (pr.same_next) {
+                         c0 += n0;
+                         c1 += n1;
+                         c2 += n2;
+                       }
+                       if (pr.head) {
+                         const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
+                         const T cs[3] = {c0, c1, c2};
+#pragma unroll
+                         for (int c = 0; c < 3; ++c) {
+                           if constexpr (kFixed) {
+                             if (fabsf(cs[c]) < 1.0f)
+                               atomicAdd(&accf[c * n + li], to_fixed(cs[c]));
+                             else
+                               unsafeAtomicAdd(gi + (size_t)c * G + j * k.nx + i, cs[c]);
+                           } else {
+                             atomicAdd(&lds[c * n + li], double(cs[c]));
+                           }
+                         }
+                       }
                      } else {
                        direct(i, j, c0, c1, c2);
                      }
                    });
     if (tl.use) {
       __syncthreads();
-      tile_flush(lds, tl, grid_cur + (size_t)tl.inst * 3 * G, k.nx, G);
+      if constexpr (kFixed)
+        tile_flush_fixed(accf, tl, grid_cur + (size_t)tl.inst * 3 * G, k.nx, G);
+      else
+        tile_flush(lds, tl, grid_cur + (size_t)tl.inst * 3 * G, k.nx, G);
     }
     if (valid) {
       T* fo = fnew_out + 4 * (size_t)p;
@@ -381,6 +479,7 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
       }
       __syncthreads();
     }
+    const LanePairs pr = lane_pairs(active && tl.use ? b1 * k.nx + b0 + 1 : -(1 + (int)threadIdx.x));
     if (valid && mt >= 0)
       g2p_backward_c(
           sp, a, k,
@@ -396,10 +495,18 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
           mo,
           [&](int i, int j, T c0, T c1, T c2) {
             if (tl.use) {
-              const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-              atomicAdd(&acc[li], double(c0));
-              atomicAdd(&acc[n + li], double(c1));
-              atomicAdd(&acc[2 * n + li], double(c2));
+              const T n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
+              if (pr.same_next) {
+                c0 += n0;
+                c1 += n1;
+                c2 += n2;
+              }
+              if (pr.head) {
+                const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
+                atomicAdd(&acc[li], double(c0));
+                atomicAdd(&acc[n + li], double(c1));
+                atomicAdd(&acc[2 * n + li], double(c2));
+              }
             } else {
               direct(i, j, c0, c1, c2);
             }
