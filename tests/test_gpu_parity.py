@@ -167,6 +167,42 @@ def test_reference_tank_1000_substeps(require_gpu):
     assert eng.get_time() == o32.t[0]
 
 
+def test_tall_tank_500_substeps(require_gpu):
+    """The tall configuration (evolve.py:33-36,136): 128x256 grid, domain height 2, 160K
+    slots, spawn (0.5, 0.60); one seeded tank (~126K live), 500 substeps, fused path
+    against the oracle in fp32 and fp64."""
+    from jellyfih_amd import phenotype
+    say("tall tank seed3, 128x256 H=2, 500 substeps, fused vs oracle f32/f64")
+    pos, mat, fib, g, info = reference_tank(3, n_slots=160000, domain_height=2.0,
+                                            spawn=phenotype.TALL_SPAWN)
+    eng = Engine(1, 160000, n_grid_y=256, domain_height=2.0)
+    eng.load_particles(0, pos, mat, fib)
+    con, frq = float(np.clip(g[9], 0.05, 0.6)), float(np.clip(g[10], 0.5, 2.0))
+    eng.set_param("contraction", con)
+    eng.set_param("freq", frq)
+    consts = oracle.reference_consts(n_grid_y=256, domain_height=2.0)
+    o32 = oracle.OracleSim(1, 160000, consts, dtype=np.float32)
+    o64 = oracle.OracleSim(1, 160000, consts, dtype=np.float64)
+    for o in (o32, o64):
+        o.load(0, pos, mat, fib)
+        o.contraction[:] = np.float32(con)
+        o.freq[:] = np.float32(frq)
+    eng.substep(500)
+    o32.run(500, nthreads=8)
+    o64.run(500, nthreads=8)
+    live = mat >= 0
+    assert int(live.sum()) == eng.live_count(0)
+    for n, a32, a64 in (("x", o32.x, o64.x), ("v", o32.v, o64.v), ("C", o32.C, o64.C),
+                        ("F", o32.F, o64.F)):
+        out = eng.read(n)[0][live]
+        e_gpu = float(np.max(np.abs(out - a64[0][live])))
+        e_f32 = float(np.max(np.abs(a32[0][live] - a64[0][live])))
+        say(f"{n}: |gpu-f64|={e_gpu:.3e} |oracle32-f64|={e_f32:.3e}")
+        assert e_gpu <= 2.0 * e_f32 + 1e-6, n
+    assert np.max(np.abs(eng.read("x")[0][live] - o64.x[0][live])) <= 1e-4
+    assert eng.get_time() == o32.t[0]
+
+
 def test_headless_matches_oracle(require_gpu):
     say("run_batch_headless vs oracle")
     cases = [reference_tank(s)[:3] for s in (1, 2)]
