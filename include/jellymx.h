@@ -162,6 +162,18 @@ int jm_sort_now(jm_handle* h);
  * jm_substep(h, n). No reference counterpart (the reference has no timing hooks). */
 int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms);
 
+/* ---- Tank building (host code, no GPU) -------------------------------------------------
+ * Replaces the k-d tree query of make_jelly.py:359-366 (fill_tank, step 3): keep[i] = 1 when
+ * every body point lies farther than `margin` from candidate i (Euclidean distance in
+ * double, compared with >), else 0. body (n_body,2) and cand (n_cand,2) are host arrays of
+ * double. */
+int jm_water_mask(const double* body, int64_t n_body, const double* cand, int64_t n_cand,
+                  double margin, uint8_t* keep);
+
+/* make_jelly.py:31-58 (_polygon_self_intersects): *out = 1 when two non-adjacent edges of
+ * the closed polygon poly (n,2, host double) cross properly, else 0. */
+int jm_polygon_self_intersects(const double* poly, int64_t n, int32_t* out);
+
 /* ---- Differentiable path -----------------------------------------------------------
  * No counterpart in the reference (it has no autodiff). A batched substep and its adjoint
  * over caller-owned device buffers (e.g. torch tensors), P particles in any order, in fp32

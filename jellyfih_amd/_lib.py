@@ -33,8 +33,9 @@ EXPORTS = (
     "jm_load_particles", "jm_set_instance_param", "jm_get_instance_param", "jm_set_time",
     "jm_get_time", "jm_substep", "jm_compute_stats", "jm_run_batch_headless", "jm_read_field",
     "jm_write_field", "jm_live_count", "jm_read_device_order", "jm_sort_now", "jm_timed_block", "jm_sync",
-    "jm_stream", "jm_last_error", "jm_diff_create", "jm_diff_destroy", "jm_fwd_block",
-    "jm_bwd_block",
+    "jm_stream", "jm_last_error", "jm_water_mask", "jm_polygon_self_intersects",
+    "jm_diff_create", "jm_diff_destroy",
+    "jm_fwd_block", "jm_bwd_block",
 )
 
 
@@ -119,6 +120,9 @@ def load() -> ctypes.CDLL:
         "jm_sort_now": [H],
         "jm_timed_block": [H, ctypes.c_int32, f32p, f32p],
         "jm_sync": [H],
+        "jm_water_mask": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                          ctypes.c_double, ctypes.c_void_p],
+        "jm_polygon_self_intersects": [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p],
         "jm_diff_create": [P(JmConfig), ctypes.c_int64, ctypes.c_int32, P(H)],
         "jm_diff_destroy": [H],
         "jm_fwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,

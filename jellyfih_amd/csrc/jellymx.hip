@@ -1097,6 +1097,118 @@ void* jm_stream(jm_handle* h) { return h ? (void*)h->stream : nullptr; }
 }  // extern "C"
 
 // -----------------------------------------------------------------------------------------
+// Tank building (host): the water candidates a body does not exclude (make_jelly.py:359-366).
+
+int jm_water_mask(const double* body, int64_t n_body, const double* cand, int64_t n_cand,
+                  double margin, uint8_t* keep) {
+#pragma clang fp contract(off)
+  if ((n_body > 0 && !body) || (n_cand > 0 && (!cand || !keep)) || n_body < 0 || n_cand < 0)
+    return fail(JM_ERR_ARG, "jm_water_mask: bad arguments");
+  if (n_body == 0 || !(margin > 0.0)) {
+    // no body, or every distance > margin unless it is not: test all pairs
+    for (int64_t i = 0; i < n_cand; ++i) {
+      bool k = true;
+      for (int64_t j = 0; j < n_body && k; ++j) {
+        const double dx = cand[2 * i] - body[2 * j], dy = cand[2 * i + 1] - body[2 * j + 1];
+        const double d2 = dx * dx;
+        k = std::sqrt(d2 + dy * dy) > margin;
+      }
+      keep[i] = k ? 1 : 0;
+    }
+    return JM_OK;
+  }
+  // Bucket the body by cells of side `margin`: a body point within `margin` of a candidate
+  // lies at most one cell away in each axis; two cells are searched, so rounding of the
+  // cell index cannot hide one. The distance is the Euclidean norm in double, summed in the
+  // order x then y and compared with `> margin`, as a k-d tree query followed by that
+  // comparison computes it.
+  double x0 = body[0], y0 = body[1], x1 = body[0], y1 = body[1];
+  for (int64_t j = 1; j < n_body; ++j) {
+    x0 = std::min(x0, body[2 * j]);
+    x1 = std::max(x1, body[2 * j]);
+    y0 = std::min(y0, body[2 * j + 1]);
+    y1 = std::max(y1, body[2 * j + 1]);
+  }
+  if (!(std::isfinite(x0) && std::isfinite(x1) && std::isfinite(y0) && std::isfinite(y1)))
+    return fail(JM_ERR_ARG, "jm_water_mask: non-finite body position");
+  const double h = margin;
+  const int64_t cx = std::min<int64_t>((int64_t)((x1 - x0) / h) + 1, 1 << 20);
+  const int64_t cy = std::min<int64_t>((int64_t)((y1 - y0) / h) + 1, 1 << 20);
+  if (cx * cy > (int64_t)1 << 26) return fail(JM_ERR_ARG, "jm_water_mask: body too sparse");
+  auto cell_of = [&](double v, double lo, int64_t n) -> int64_t {
+    const double f = std::floor((v - lo) / h);
+    return f < 0 ? -1 : (f >= (double)n ? n : (int64_t)f);
+  };
+  std::vector<int64_t> start(cx * cy + 1, 0), order(n_body);
+  for (int64_t j = 0; j < n_body; ++j)
+    ++start[cell_of(body[2 * j + 1], y0, cy) * cx + cell_of(body[2 * j], x0, cx) + 1];
+  for (int64_t c = 0; c < cx * cy; ++c) start[c + 1] += start[c];
+  {
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t j = 0; j < n_body; ++j)
+      order[fill[cell_of(body[2 * j + 1], y0, cy) * cx + cell_of(body[2 * j], x0, cx)]++] = j;
+  }
+  for (int64_t i = 0; i < n_cand; ++i) {
+    const double px = cand[2 * i], py = cand[2 * i + 1];
+    const int64_t ix = cell_of(px, x0, cx), iy = cell_of(py, y0, cy);
+    bool k = true;
+    for (int64_t yy = std::max<int64_t>(iy - 2, 0); yy <= std::min<int64_t>(iy + 2, cy - 1) && k; ++yy)
+      for (int64_t xx = std::max<int64_t>(ix - 2, 0); xx <= std::min<int64_t>(ix + 2, cx - 1) && k; ++xx) {
+        const int64_t c = yy * cx + xx;
+        for (int64_t q = start[c]; q < start[c + 1]; ++q) {
+          const int64_t j = order[q];
+          const double dx = px - body[2 * j], dy = py - body[2 * j + 1];
+          const double d2 = dx * dx;
+          if (!(std::sqrt(d2 + dy * dy) > margin)) {
+            k = false;
+            break;
+          }
+        }
+      }
+    keep[i] = k ? 1 : 0;
+  }
+  return JM_OK;
+}
+
+namespace {
+
+// Proper crossing of segments ab and cd (make_jelly.py:31-43): cross products in double,
+// parallel below 1e-12, crossing parameters strictly inside (0, 1).
+bool segments_cross(const double* a, const double* b, const double* c, const double* d) {
+#pragma clang fp contract(off)
+  const double r0 = b[0] - a[0], r1 = b[1] - a[1];
+  const double s0 = d[0] - c[0], s1 = d[1] - c[1];
+  const double rs_a = r0 * s1, rs_b = r1 * s0;
+  const double rxs = rs_a - rs_b;
+  if (std::fabs(rxs) < 1e-12) return false;
+  const double q0 = c[0] - a[0], q1 = c[1] - a[1];
+  const double qs_a = q0 * s1, qs_b = q1 * s0, qr_a = q0 * r1, qr_b = q1 * r0;
+  const double t = (qs_a - qs_b) / rxs;
+  const double u = (qr_a - qr_b) / rxs;
+  return 0 < t && t < 1 && 0 < u && u < 1;
+}
+
+}  // namespace
+
+// Whether a closed polygon has a crossing between non-adjacent edges (make_jelly.py:46-58).
+int jm_polygon_self_intersects(const double* poly, int64_t n, int32_t* out) {
+  if (!out || n < 0 || (n > 0 && !poly)) return fail(JM_ERR_ARG, "bad arguments");
+  *out = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double* a = poly + 2 * i;
+    const double* b = poly + 2 * ((i + 1) % n);
+    for (int64_t j = i + 2; j < n; ++j) {
+      if (i == 0 && j == n - 1) continue;  // the last edge is adjacent to the first
+      if (segments_cross(a, b, poly + 2 * j, poly + 2 * ((j + 1) % n))) {
+        *out = 1;
+        return JM_OK;
+      }
+    }
+  }
+  return JM_OK;
+}
+
+// -----------------------------------------------------------------------------------------
 // Differentiable path (no reference counterpart: the reference has no autodiff).
 
 struct jm_diff {

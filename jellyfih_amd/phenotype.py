@@ -44,33 +44,15 @@ def _bezier(p0, p1, p2, p3, t):
     return (1 - t) ** 3 * p0 + 3 * (1 - t) ** 2 * t * p1 + 3 * (1 - t) * t ** 2 * p2 + t ** 3 * p3
 
 
-def _cross(u, v):
-    return u[0] * v[1] - u[1] * v[0]
-
-
-def _segments_cross(a, b, c, d) -> bool:
-    """Proper crossing of segments ab and cd (make_jelly.py:31-43)."""
-    r, s = b - a, d - c
-    rxs = _cross(r, s)
-    if abs(rxs) < 1e-12:
-        return False
-    qp = c - a
-    t = _cross(qp, s) / rxs
-    u = _cross(qp, r) / rxs
-    return 0 < t < 1 and 0 < u < 1
-
-
 def polygon_self_intersects(poly: np.ndarray) -> bool:
-    """Any crossing between non-adjacent edges of a closed polygon (make_jelly.py:46-58)."""
-    n = len(poly)
-    for i in range(n):
-        a, b = poly[i], poly[(i + 1) % n]
-        for j in range(i + 2, n):
-            if i == 0 and j == n - 1:
-                continue
-            if _segments_cross(a, b, poly[j], poly[(j + 1) % n]):
-                return True
-    return False
+    """Any proper crossing between non-adjacent edges of a closed polygon
+    (make_jelly.py:31-58), computed by jm_polygon_self_intersects in libjellymx.so."""
+    from . import _lib
+    _lib.load()
+    poly = np.ascontiguousarray(poly, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(1, dtype=np.int32)
+    _lib.call("jm_polygon_self_intersects", poly.ctypes.data, len(poly), out.ctypes.data)
+    return bool(out[0])
 
 
 def _unit_normals(points: np.ndarray) -> np.ndarray:
@@ -232,6 +214,20 @@ def water_lattice(grid_res: int = 128, domain_height: float = 1.0) -> np.ndarray
     return np.vstack([wgx.ravel(), wgy.ravel()]).T
 
 
+def water_mask(body, cands, margin: float) -> np.ndarray:
+    """Candidates farther than `margin` from every body point: the nearest-neighbour
+    distance test of make_jelly.py:359-366 (a k-d tree query, then `distance > margin`),
+    computed by jm_water_mask in libjellymx.so (cell buckets, same distance arithmetic)."""
+    from . import _lib
+    _lib.load()
+    body = np.ascontiguousarray(body, dtype=np.float64).reshape(-1, 2)
+    cands = np.ascontiguousarray(cands, dtype=np.float64).reshape(-1, 2)
+    keep = np.empty(len(cands), dtype=np.uint8)
+    _lib.call("jm_water_mask", body.ctypes.data, len(body), cands.ctypes.data, len(cands),
+              float(margin), keep.ctypes.data)
+    return keep.astype(bool)
+
+
 def fill_tank(genome, max_particles: int, grid_res: int = 128, spawn_offset=None,
               water_margin: float = 0.005, with_payload: bool = True,
               domain_height: float = 1.0):
@@ -243,8 +239,7 @@ def fill_tank(genome, max_particles: int, grid_res: int = 128, spawn_offset=None
     n_robot = len(robot_pos)
     cands = water_lattice(grid_res, domain_height)
     if n_robot > 0:
-        dist, _ = cKDTree(robot_pos).query(cands, k=1)
-        water = cands[dist > water_margin]
+        water = cands[water_mask(robot_pos, cands, water_margin)]
     else:
         water = cands
     n_water = len(water)

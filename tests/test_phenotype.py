@@ -50,3 +50,64 @@ def test_self_intersection_checker():
     bowtie = np.array([[0, 0], [1, 1], [1, 0], [0, 1]], float)
     assert not phenotype.polygon_self_intersects(square)
     assert phenotype.polygon_self_intersects(bowtie)
+
+
+def test_water_mask_matches_kdtree():
+    """The native water exclusion equals the k-d tree test it replaces
+    (make_jelly.py:359-366), including candidates at exactly the margin."""
+    from scipy.spatial import cKDTree
+    from jellyfih_amd.phenotype import water_mask
+    rng = np.random.default_rng(3)
+    for n_body, spread in ((1, 0.2), (50, 0.05), (3000, 0.3)):
+        body = rng.uniform(0.35, 0.35 + spread, size=(n_body, 2))
+        cands = np.concatenate([rng.uniform(0.2, 0.8, size=(20000, 2)),
+                                body + [0.005, 0.0], body - [0.0, 0.005],
+                                body + rng.normal(size=body.shape) * 0.004])
+        ref = cKDTree(body).query(cands, k=1)[0] > 0.005
+        np.testing.assert_array_equal(water_mask(body, cands, 0.005), ref)
+    assert water_mask(np.zeros((0, 2)), cands[:10], 0.005).all()
+
+
+def _segments_cross_py(a, b, c, d):
+    """make_jelly.py:31-43 restated in Python (the check for the native version)."""
+    cross = lambda u, v: u[0] * v[1] - u[1] * v[0]
+    r, s = b - a, d - c
+    rxs = cross(r, s)
+    if abs(rxs) < 1e-12:
+        return False
+    qp = c - a
+    t = cross(qp, s) / rxs
+    u = cross(qp, r) / rxs
+    return 0 < t < 1 and 0 < u < 1
+
+
+def _self_intersects_py(poly):
+    """make_jelly.py:46-58 restated in Python."""
+    n = len(poly)
+    for i in range(n):
+        for j in range(i + 2, n):
+            if i == 0 and j == n - 1:
+                continue
+            if _segments_cross_py(poly[i], poly[(i + 1) % n], poly[j], poly[(j + 1) % n]):
+                return True
+    return False
+
+
+def test_polygon_self_intersects_matches_python():
+    from jellyfih_amd.phenotype import polygon_self_intersects
+    rng = np.random.default_rng(4)
+    hits = 0
+    for trial in range(300):
+        n = int(rng.integers(3, 40))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        rad = rng.uniform(0.5, 1.0, n) if trial % 2 else rng.uniform(0.0, 1.0, n)
+        poly = np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+        if trial % 3 == 0:
+            poly = rng.permutation(poly)          # mostly self-intersecting
+        ref = _self_intersects_py(poly)
+        hits += ref
+        assert polygon_self_intersects(poly) == ref, trial
+    assert 0 < hits < 300
+    square = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    bow = np.array([[0.0, 0.0], [1.0, 1.0], [1.0, 0.0], [0.0, 1.0]])
+    assert not polygon_self_intersects(square) and polygon_self_intersects(bow)
