@@ -238,6 +238,8 @@ def main() -> int:
                     help="run only the differentiable legs (no throughput line fields)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched tall-tank forward + backward")
+    ap.add_argument("--batched-steps", type=int, default=4000)
+    ap.add_argument("--batched-checkpoint", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -252,12 +254,14 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "no GPU visible"
     if args.only_batched:
-        print(json.dumps({"diff": {"batched": diff_batched()}}), flush=True)
+        print(json.dumps({"diff": {"batched": diff_batched(
+            steps=args.batched_steps, checkpoint=args.batched_checkpoint)}}), flush=True)
         return 0
     if args.only_diff:
         out = {"diff": diff_episode()}
         if args.diff_batched:
-            out["diff"]["batched"] = diff_batched()
+            out["diff"]["batched"] = diff_batched(steps=args.batched_steps,
+                                                  checkpoint=args.batched_checkpoint)
         print(json.dumps(out), flush=True)
         return 0
     M, N = args.instances, args.slots
@@ -335,7 +339,8 @@ def main() -> int:
         diff = diff_episode()
         log(f"diff {diff}")
     if args.diff_batched:
-        diff = dict(diff or {}, batched=diff_batched())
+        diff = dict(diff or {}, batched=diff_batched(steps=args.batched_steps,
+                                                     checkpoint=args.batched_checkpoint))
         log(f"diff batched {diff['batched']}")
 
     cpu = None

@@ -41,17 +41,26 @@ struct DiffInputs {
   const T* params;      // (M, 5): amp, contraction, freq, payload density, E_jelly
 };
 
+// The per-instance parameters a particle of material `mt` uses: jelly/muscle Lame
+// parameters from E_jelly (nu = 0.3), the actuation waveform and amplitude for muscle, the
+// payload density for payload (zeros otherwise: water, the majority, needs none of them).
 template <typename T>
-__device__ __forceinline__ void inst_params(const DiffInputs<T>& in, int m, T t,
+__device__ __forceinline__ void inst_params(const DiffInputs<T>& in, int m, int mt, T t,
                                             const Consts<T>& k, Act<T>& act, T& mu, T& la,
                                             T& amp, T& pdens) {
   const T* pr = in.params + 5 * m;
-  act = activation(t, pr[2], pr[1], k);
-  const T E = pr[4], nu = k.nu_jelly;
-  mu = E / (T(2) * (T(1) + nu));
-  la = E * nu / ((T(1) + nu) * (T(1) - T(2) * nu));
-  amp = pr[0];
-  pdens = pr[3];
+  act = {T(0), T(0), T(0)};
+  mu = la = amp = pdens = T(0);
+  if (mt == 1 || mt == 3) {
+    const T E = pr[4], nu = k.nu_jelly;
+    mu = E / (T(2) * (T(1) + nu));
+    la = E * nu / ((T(1) + nu) * (T(1) - T(2) * nu));
+  }
+  if (mt == 3) {
+    act = activation(t, pr[2], pr[1], k);
+    amp = pr[0];
+  }
+  if (mt == 2) pdens = pr[3];
 }
 
 template <typename T>
@@ -284,10 +293,12 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
     Act<T> act = {T(0), T(0), T(0)};
     T mu = T(0), la = T(0), amp = T(0), pdens = T(0), d0 = T(0), d1 = T(0), as = T(0);
     if (active) {
-      inst_params(in, m, t, k, act, mu, la, amp, pdens);
-      d0 = in.fiber[2 * p];
-      d1 = in.fiber[2 * p + 1];
-      as = in.amp_scale[p];
+      inst_params(in, m, mt, t, k, act, mu, la, amp, pdens);
+      if (mt == 3) {  // fibre and multiplier matter for muscle only
+        d0 = in.fiber[2 * p];
+        d1 = in.fiber[2 * p + 1];
+        as = in.amp_scale[p];
+      }
     }
     const int n = tl.tw * tl.th;
     T* gi = grid_cur + (size_t)m * 3 * G;
@@ -420,13 +431,17 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
         const PState<T> sc = load_state(st_cur + 12 * (size_t)p);
         Act<T> act;
         T mu, la, amp, pdens;
-        inst_params(in, m, t, k, act, mu, la, amp, pdens);
+        inst_params(in, m, mt, t, k, act, mu, la, amp, pdens);
         const T* ag = agrid_cur + (size_t)m * 3 * G;
         T mub = T(0), lab = T(0), betab = T(0);
-        const T as = in.amp_scale[p];
-        p2g_backward_particle(sc, mt, in.fiber[2 * p], in.fiber[2 * p + 1], act.a * amp * as,
-                              mu, la, pdens, k, ag, ag + G, ag + 2 * (size_t)G, Fnb, a, mub,
-                              lab, betab);
+        T as = T(0), d0 = T(0), d1 = T(0);
+        if (mt == 3) {  // fibre and multiplier matter for muscle only
+          as = in.amp_scale[p];
+          d0 = in.fiber[2 * p];
+          d1 = in.fiber[2 * p + 1];
+        }
+        p2g_backward_particle(sc, mt, d0, d1, act.a * amp * as, mu, la, pdens, k, ag, ag + G,
+                              ag + 2 * (size_t)G, Fnb, a, mub, lab, betab);
         if (mt == 1 || mt == 3) {
           const T nu = k.nu_jelly;
           gp4[0] = double(mub / (T(2) * (T(1) + nu)) +
