@@ -73,6 +73,7 @@ struct jm_handle {
   int* nchunks_d = nullptr;
   float* stats_d = nullptr;   // M * 6
   float* staging = nullptr;   // M * 2 * G floats (grid velocity readout)
+  float* damp = nullptr;      // nx + ny floats: boundary damping factors (k.damp)
 
   // host mirrors
   std::vector<float> h_param[5];  // amp, contraction, freq, pdens, E_jelly
@@ -109,12 +110,12 @@ struct jm_handle {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {st.base, tmp.base, sortbuf, hist, acc, t_buf, params, n_live_d,
-                    chunks_d, nchunks_d, stats_d, staging};
+                    chunks_d, nchunks_d, stats_d, staging, damp};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     st.base = tmp.base = nullptr;
     sortbuf = hist = nullptr;
-    acc = t_buf = params = stats_d = staging = nullptr;
+    acc = t_buf = params = stats_d = staging = damp = nullptr;
     n_live_d = nullptr;
     chunks_d = nullptr;
     nchunks_d = nullptr;
@@ -718,8 +719,15 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
             dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8) &&
             dm((void**)&h->params, (size_t)6 * M * 4) && dm((void**)&h->n_live_d, M * 4) &&
             dm((void**)&h->nchunks_d, M * 4) && dm((void**)&h->stats_d, (size_t)M * 6 * 4) &&
-            dm((void**)&h->staging, (size_t)M * 2 * h->G * 4);
+            dm((void**)&h->staging, (size_t)M * 2 * h->G * 4) &&
+            dm((void**)&h->damp, (size_t)(nx + ny) * 4);
   if (!ok) return cleanup(fail(JM_ERR_NOMEM, "hipMalloc(engine buffers)"));
+  std::vector<float> damp(nx + ny);
+  for (int i = 0; i < nx; ++i) damp[i] = jm::damping_factor(i, nx, h->k.damp_x);
+  for (int j = 0; j < ny; ++j) damp[nx + j] = jm::damping_factor(j, ny, h->k.damp_y);
+  if (hipMemcpy(h->damp, damp.data(), damp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(JM_ERR_HIP, "damping table upload"));
+  h->k.damp = h->damp;
   if (hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * 4, h->stream) != hipSuccess ||
       hipMemsetAsync(h->t_buf, 0, 8, h->stream) != hipSuccess ||
       hipMemsetAsync(h->stats_d, 0, (size_t)M * 6 * 4, h->stream) != hipSuccess ||

@@ -248,28 +248,35 @@ constexpr int kBoxEmpty = 1 << 29;
 // P2G accumulates in LDS as 64-bit fixed point with 50 fractional bits: integer LDS
 // atomics run at full rate on gfx950 where fp32 LDS atomics do not, and integer sums are
 // exact and independent of the order of the contributions. A contribution converts with
-// an error below 2^-50 (truncation of c * 2^50). A window gathers at most 4096 particles,
+// an error of at most 2^-51 (c * 2^50 rounded to the nearest integer). A window gathers at most 4096 particles,
 // so with |c| < 1 every node sum is below 2^12 = 2^62 / 2^50 and fits in int64 (partial
 // sums may wrap; two's-complement addition still yields the exact final sum).
 // Particles with a contribution outside that range (or NaN) scatter straight to the global
 // fp32 grid instead, so non-finite values propagate as they do in fp32.
 constexpr double kFixInv = 1.0 / 1125899906842624.0;   // 2^-50
-constexpr float kFixPre = 262144.0f;                    // 2^18
+constexpr float kFixPre = 1125899906842624.0f;          // 2^50
 constexpr float kFixMax = 1.0f;
 
-// c * 2^50 as a two's-complement int64, split as hi * 2^32 + lo with hi = floor(c * 2^18)
-// and lo = frac(c * 2^18) * 2^32: scaling by powers of two and taking the fraction are
-// exact for |c| < 1, and the truncation of lo drops less than one unit of 2^-50 (for a
-// tiny negative y, v_fract's clamp below 1.0 costs at most 256 units, 2.3e-13). Four VALU
-// operations (fract, mul, cvt_u32, cvt_flr_i32) instead of a generic float-to-int64
-// conversion; the compiler does not fuse floor + convert into v_cvt_flr_i32_f32 by itself.
-// y = c * 2^18 (the caller pre-scales its per-particle factors, which is exact): the
-// fixed-point value c * 2^50 = hi * 2^32 + lo with hi = floor(y), lo = frac(y) * 2^32.
+// y = c * 2^50 (|y| < 2^51; the caller pre-scales its per-particle factors by 2^50, which
+// is exact) rounded to the nearest integer, as two's-complement int64 bits, in three VALU
+// operations: y + 1.5 * 2^52 in double is exact up to that rounding and lands in
+// [2^52, 2^53), where the 52 mantissa bits hold y + 2^51; the low word is then y's low word
+// and the high word, minus the exponent and the 2^51 bit (0x43380000), is y's high word.
+__device__ __forceinline__ unsigned long long fixed_bits(float y) {
+  const unsigned long long b = __double_as_longlong((double)y + 6755399441055744.0);
+  const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
+  return ((unsigned long long)hi << 32) | (unsigned)b;
+}
+
 __device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float y) {
-  const unsigned lo = (unsigned)(__builtin_amdgcn_fractf(y) * 4294967296.0f);
-  int hi;
-  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(y));
-  atomicAdd(slot, ((unsigned long long)(unsigned)hi << 32) | lo);
+  atomicAdd(slot, fixed_bits(y));
+}
+
+// Row of node q of a window ww nodes wide, for q < 2^21: (q + 0.5) / ww lies at least
+// 0.5 / ww from an integer, and the product with the approximate reciprocal (relative
+// error below 2^-22) is off by less than that.
+__device__ __forceinline__ int window_row(int q, float rcp_ww) {
+  return (int)(((float)q + 0.5f) * rcp_ww);
 }
 
 // Rare path: a particle whose contributions leave the fixed-point range scatters its nine
@@ -454,8 +461,9 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     float2* lv = reinterpret_cast<float2*>(lds);
     if (use_lds) {
       const int nodes = ww * wh;
+      const float rcp_ww = __builtin_amdgcn_rcpf((float)ww);
       for (int q = tid; q < nodes; q += BLOCK) {
-        const int jj = q / ww, ii = q - jj * ww;
+        const int jj = window_row(q, rcp_ww), ii = q - jj * ww;
         const int i = mn0 + ii, j = mn1 + jj;
         const int node = j * nx + i;
         lv[q] = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
@@ -519,8 +527,13 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
 
   if constexpr (DO_P2G) {
     const int mi = m;
-    const float act = activation(a.t_buf[a.t_rd], a.ip.freq[mi], a.ip.contraction[mi], k);
-    const float amp_act = a.ip.amp[mi] * act;
+    // The activation is uniform over the workgroup: the first wave evaluates it and the
+    // others read it after block_bbox's barrier.
+    __shared__ float s_act;
+    if (tid < 64) {
+      const float act = activation(a.t_buf[a.t_rd], a.ip.freq[mi], a.ip.contraction[mi], k);
+      if (tid == 0) s_act = act;
+    }
     const float mu_j = a.ip.mu_j[mi], la_j = a.ip.la_j[mi], pdens = a.ip.pdens[mi];
 
     int mn0 = kBoxEmpty, mx0 = -kBoxEmpty, mn1 = kBoxEmpty, mx1 = -kBoxEmpty;
@@ -538,10 +551,12 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       mx1 = max(mx1, b1);
     }
     block_bbox<BLOCK>(mn0, mx0, mn1, mx1, red);
+    const float amp_act = a.ip.amp[mi] * s_act;
     const int ww = mx0 - mn0 + 3, wh = mx1 - mn1 + 3;
     const bool any = mn0 <= mx0;
     const bool use_lds = any && ww * wh <= WMAX;
     const int nodes = use_lds ? ww * wh : 0;
+    const float rcp_ww = __builtin_amdgcn_rcpf((float)ww);
     for (int q = tid; q < nodes; q += BLOCK) {
       lds64[q] = 0ull;
       lds64[WMAX + q] = 0ull;
@@ -605,23 +620,37 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
           scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
                          {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
                          {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
+          // done: its neighbour lane adds this lane's (now zero, finite) terms times 0
+#pragma unroll
+          for (int ii = 0; ii < 3; ++ii) {
+            P0[ii] = Q0[ii] = P1[ii] = Q1[ii] = Mi[ii] = 0.f;
+            wy[ii] = ey[ii] = 0.f;
+          }
         }
       }
       // Cell-sorted particles of one base cell sit in adjacent lanes (water: 4 per cell).
-      // Pairs of adjacent lanes with the same cell, within a DPP row of 16 lanes, add their
-      // 27 contributions across lanes (row_shl:1), so the first lane of every pair in a run
-      // holds the pair's sums and alone issues the 27 64-bit LDS atomics: half the atomics
-      // of one set per particle, in a fixed order (deterministic). Lanes outside the
-      // fixed-point LDS path carry unique keys and add nothing. (A second step over groups
-      // of 4 halves the atomics again but costs more VALU than it saves: measured 49.5 us
-      // against 48.0 us per substep at 32 x 80K.)
+      // Runs of lanes with the same cell, within a DPP row of 16 lanes, add their 27
+      // contributions across lanes in two steps (row_shl:1, then row_shl:2), so the first
+      // lane of every group of 4 in a run holds the group's sums and alone issues the 27
+      // 64-bit LDS atomics: a quarter of the atomic traffic of one set per particle, and no
+      // two lanes of a water cell hitting one address, in a fixed order (deterministic).
+      // A neighbour's term enters as t * f with f = 1 when it has the same cell and 0
+      // otherwise (one FMA, exact either way: every lane's terms are finite, since
+      // out-of-range lanes carry zeros). After the first step, lane l + 2 holds lanes
+      // l + 2 and l + 3 when they share its cell; lane l takes it only when l + 1 and l + 2
+      // both share l's cell, i.e. l + 2 is in l's run (between sorts a cell can recur
+      // after a gap, A B A A, where the second A run emits by itself). Lanes outside the
+      // fixed-point LDS path carry unique keys and add nothing. Measured at 32 x 80K:
+      // 44.3 us per substep against 45.7 us with pairs only.
       const int key = fixed_ok ? (b1 * nx + b0 + 1) : -(1 + lane);
       const int k1 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x101, 0xF, 0xF, false);
+      const int k2 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x102, 0xF, 0xF, false);
       const int kp = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x111, 0xF, 0xF, false);
-      const bool same1 = k1 == key;
+      const float f1 = k1 == key ? 1.0f : 0.0f;
+      const float f2 = (k1 == key && k2 == key) ? 1.0f : 0.0f;
       const unsigned long long heads = __ballot(kp != key);
       const int runstart = 63 - __clzll(heads & lanes_le);
-      const bool emit = fixed_ok && (((lane - runstart) & 1) == 0);
+      const bool emit = fixed_ok && (((lane - runstart) & 3) == 0);
       const int q0 = (b1 - mn1) * ww + (b0 - mn0);
 #pragma unroll
       for (int jj = 0; jj < 3; ++jj) {
@@ -631,9 +660,13 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
                         wy[jj] * Mi[ii]};
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
-            const float t1 = __int_as_float(__builtin_amdgcn_update_dpp(
-                0, __float_as_int(c[e]), 0x101, 0xF, 0xF, false));
-            c[e] = same1 ? c[e] + t1 : c[e];
+            // row_shl with bound_ctrl: lanes past the end of their row read 0
+            const float t1 = __int_as_float(
+                __builtin_amdgcn_mov_dpp(__float_as_int(c[e]), 0x101, 0xF, 0xF, true));
+            c[e] = __builtin_fmaf(t1, f1, c[e]);
+            const float t2 = __int_as_float(
+                __builtin_amdgcn_mov_dpp(__float_as_int(c[e]), 0x102, 0xF, 0xF, true));
+            c[e] = __builtin_fmaf(t2, f2, c[e]);
           }
           if (emit) {
             const int q = q0 + jj * ww + ii;
@@ -646,7 +679,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     }
     __syncthreads();
     for (int q = tid; q < nodes; q += BLOCK) {
-      const int jj = q / ww, ii = q - jj * ww;
+      const int jj = window_row(q, rcp_ww), ii = q - jj * ww;
       const int node = (mn1 + jj) * nx + (mn0 + ii);
       const float c0 = from_fixed(lds64[q]), c1 = from_fixed(lds64[WMAX + q]),
                   cm = from_fixed(lds64[2 * WMAX + q]);

@@ -16,6 +16,9 @@ namespace jm {
 // the host in double and rounded to fp32 once.
 struct PhysConsts {
   int nx, ny, damp_x, damp_y;
+  // Boundary damping factor per node column (nx floats) then per node row (ny floats),
+  // filled on the host from damping_factor(): the grid op's damp = damp[i] * damp[nx + j].
+  const float* damp;
   float dt, dx, inv_dx, p_mass, gravity_dt, stress_scale, apic_scale;
   float mu_base, la_base, mu_pay, la_pay, water_la, act_freq;
   float x_lo, x_hi, y_hi, sig_floor, pay_lo, pay_hi, pi;
@@ -218,18 +221,26 @@ __device__ __forceinline__ P2GOut p2g_update(int mat, const Mat2& F0, const Mat2
   return o;
 }
 
+// Damping factor of mpm_sim.py:287-294 along one axis of n nodes with a band of d: the
+// reference multiplies damp = 1 by the x factor, then the y factor; the two x (or y)
+// conditions never hold together (d = n/20), so damp = fx(i) * fy(j) with each factor
+// computed exactly as the reference's fp32 expression. Host and device share this code.
+__host__ __device__ inline float damping_factor(int i, int n, int d) {
+#pragma clang fp contract(off)
+  float f = 1.0f;
+  if (i < d) f = f * (0.95f + 0.05f * (float)i / (float)d);
+  if (i > n - d) f = f * (0.95f + 0.05f * (float)(n - i) / (float)d);
+  return f;
+}
+
 // Grid update of one node (mpm_sim.py:274-309): normalise by mass, boundary damping band
-// of n/20 cells on all four sides, then the wall clamps. Nodes with no mass keep their
-// (zero) momentum, as in the reference.
+// of n/20 cells on all four sides (factors from the table in k.damp), then the wall
+// clamps. Nodes with no mass keep their (zero) momentum, as in the reference.
 __device__ __forceinline__ float2 grid_node(float px, float py, float m, int i, int j,
                                             const PhysConsts& k) {
   if (!(m > 0.0f)) return make_float2(px, py);
   float g0 = px / m, g1 = py / m;
-  float damp = 1.0f;
-  if (i < k.damp_x) damp = damp * (0.95f + 0.05f * (float)i / (float)k.damp_x);
-  if (i > k.nx - k.damp_x) damp = damp * (0.95f + 0.05f * (float)(k.nx - i) / (float)k.damp_x);
-  if (j < k.damp_y) damp = damp * (0.95f + 0.05f * (float)j / (float)k.damp_y);
-  if (j > k.ny - k.damp_y) damp = damp * (0.95f + 0.05f * (float)(k.ny - j) / (float)k.damp_y);
+  const float damp = k.damp[i] * k.damp[k.nx + j];
   g0 = g0 * damp;
   g1 = g1 * damp;
   if (i < 3 && g0 < 0.0f) g0 = 0.0f;

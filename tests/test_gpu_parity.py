@@ -132,6 +132,50 @@ def test_sort_permutation_bitexact(require_gpu):
         assert np.all(np.diff(keys) >= 0)
 
 
+def test_unsorted_chunks_grid(require_gpu):
+    """Grid of one substep from a state whose particles have left their sorted cells
+    (no sort for 300 substeps of fast random motion), so that cells recur with gaps along
+    the chunk order: the fused path's cross-lane pre-sums must count each particle once.
+    The oracles (fp32 and fp64) start from the engine's state. Grid mass has no
+    cancellation: within 1e-5 of the fp32 oracle (a particle counted twice or dropped
+    moves a node by its whole weight). Momentum of this state is a sum of large stress
+    terms that cancel, so its error against the fp64 oracle is bounded by twice the fp32
+    oracle's own error, as for the long runs."""
+    say("unsorted chunks: 300 substeps without sort, then one substep vs oracle f32/f64")
+    cases = ragged_cases()
+    eng, orc = make_pair(cases)
+    perturb(eng, orc, seed=8, sv=0.5, sc=0.2, sf=0.01)
+    eng.set_tuning(sort_interval=1000000)
+    eng.substep(300)
+    orc64 = oracle.OracleSim(len(cases), cases[0][0].shape[0], dtype=np.float64)
+    for m, (pos, mat, fib) in enumerate(cases):
+        orc64.load(m, pos, mat, fib)
+    for name in ("x", "v", "C", "F", "Jp"):
+        val = eng.read(name)
+        for o in (orc, orc64):
+            getattr(o, name)[...] = val
+    for o in (orc, orc64):
+        o.t[0] = eng.get_time()
+    # the chunk order really is unsorted now
+    c = oracle.reference_consts()
+    _, x0 = eng.device_order(0)
+    keys = oracle.cell_keys(c, x0)
+    say(f"descending key steps in instance 0: {int(np.sum(np.diff(keys) < 0))}")
+    assert np.sum(np.diff(keys) < 0) > 100
+    eng.substep(1)
+    orc.run(1)
+    orc64.run(1)
+    gm, gv = eng.read("grid_m"), eng.read("grid_v")
+    assert relmax(gm, orc.grid_m) <= 1e-5, relmax(gm, orc.grid_m)
+    mom = gv.astype(np.float64) * gm[..., None]
+    mom64 = orc64.grid_v * orc64.grid_m[..., None]
+    mom32 = orc.grid_v.astype(np.float64) * orc.grid_m[..., None]
+    e_gpu, e_32 = relmax(mom, mom64), relmax(mom32, mom64)
+    say(f"momentum: |gpu-f64| {e_gpu:.3e}, |oracle32-f64| {e_32:.3e} (relative to max)")
+    assert e_gpu <= 2.0 * e_32 + 1e-6
+    assert np.max(np.abs(eng.read("x") - orc64.x)) <= 2e-7
+
+
 def test_reference_tank_1000_substeps(require_gpu):
     """cfg1: one seeded reference tank (~62K live of 80K slots), 1000 substeps."""
     say("reference tank seed0, 1000 substeps, fused vs oracle f32/f64")
