@@ -95,7 +95,9 @@ typedef struct jm_tuning {
   int32_t use_graphs;       /* 1: replay captured HIP graphs per block */
   int32_t xcd_affinity;     /* 1: keep each instance's workgroups on one XCD (needs M >= 8) */
   int32_t lazy_vc;          /* 1: keep v, C in registers across fused substeps */
-  int32_t reserved;
+  int32_t sort_strip;       /* cell order of the sort: strips of this many cell rows, the
+                               columns of alternate strips in opposite directions, rows
+                               within a column (default 8); 0: row-major cells */
 } jm_tuning;
 
 typedef struct jm_handle jm_handle;

@@ -48,7 +48,7 @@ class JmConfig(ctypes.Structure):
         ("n_grid_y", ctypes.c_int32),
         ("track_glow", ctypes.c_int32),
         ("path", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sort_strip", ctypes.c_int32),
         ("domain_height", ctypes.c_double),
     ]
 
@@ -62,11 +62,11 @@ class JmTuning(ctypes.Structure):
         ("use_graphs", ctypes.c_int32),
         ("xcd_affinity", ctypes.c_int32),
         ("lazy_vc", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sort_strip", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class JmDiffInputs(ctypes.Structure):

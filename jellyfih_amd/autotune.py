@@ -1,6 +1,7 @@
 """Autotuner for the fused path: picks workgroup size, particles per thread (the chunk of
 cell-sorted particles one workgroup owns, which sets the LDS grid-window shape), the sort
-interval K, the graph block length, and the XCD-affinity / lazy-v,C switches.
+interval K and cell order (strip height), the graph block length, and the XCD-affinity /
+lazy-v,C switches.
 
 It measures each candidate on the caller's own workload (an Engine already loaded with it)
 with HIP-event-timed blocks, keeps the fastest, and can write it to ``tuning.json`` beside
@@ -24,6 +25,7 @@ TUNING_PATH = os.path.join(HERE, "tuning.json")
 SHAPES = [(256, 4), (256, 8), (512, 4), (512, 8), (128, 8), (128, 16), (256, 16), (256, 2),
           (512, 2)]
 SORT_INTERVALS = [50, 100, 200, 500, 1000]
+SORT_STRIPS = [0, 4, 8, 16]
 BLOCKS = [50, 100, 200]
 
 
@@ -64,6 +66,8 @@ def tune(eng, live: int, substeps: int = 1000, log=print) -> dict:
         trial(threads=t, ppt=p)
     for k in SORT_INTERVALS:
         trial(sort_interval=k)
+    for st in SORT_STRIPS:
+        trial(sort_strip=st)
     for b in BLOCKS:
         trial(block_substeps=b)
     for flag in ("lazy_vc", "xcd_affinity"):

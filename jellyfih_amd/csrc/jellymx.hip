@@ -349,6 +349,15 @@ int commit(jm_handle* h) {
   return JM_OK;
 }
 
+// Radix bits (a multiple of 8) that cover every sort key of cell_key: row-major keys are
+// below nx * ny, strip keys below (strips) * nx * strip.
+int sort_key_bits(int nx, int ny, int strip) {
+  const int64_t rows = strip > 0 ? (int64_t)(ny + strip - 1) / strip * strip : ny;
+  int bits = 8;
+  while ((1LL << bits) < rows * nx) bits += 8;
+  return bits;
+}
+
 jm::SortArgs sort_args(jm_handle* h) {
   jm::SortArgs a;
   const size_t n = (size_t)h->M * h->cap;
@@ -364,6 +373,7 @@ jm::SortArgs sort_args(jm_handle* h) {
   a.cap = h->cap;
   a.M = h->M;
   a.tmax = h->tmax;
+  a.strip = h->tun.sort_strip;
   a.k = h->k;
   return a;
 }
@@ -683,7 +693,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   h->G = nx * ny;
   h->cap = cfg->n_particles;
   h->k = make_consts(*cfg, nx, ny);
-  h->tun = {100, 100, 256, 4, 1, 1, 1, 0};
+  h->tun = {100, 100, 256, 4, 1, 1, 1, 8};
   const int M = h->M;
   h->h_param[JM_PARAM_ACTUATION].assign(M, 500.0f);       // mpm_sim.py:146
   h->h_param[JM_PARAM_CONTRACTION].assign(M, 0.20f);      // :148
@@ -712,8 +722,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
     return hipMalloc(p, std::max<size_t>(bytes, 256)) == hipSuccess;
   };
   h->tmax = (h->cap + jm::kSortTile - 1) / jm::kSortTile;
-  h->key_bits = 8;
-  while ((1LL << h->key_bits) < (int64_t)h->G) h->key_bits += 8;
+  h->key_bits = sort_key_bits(nx, ny, h->tun.sort_strip);
   bool ok = dm((void**)&h->sortbuf, n * 5 * 4) &&
             dm((void**)&h->hist, (size_t)M * 256 * h->tmax * 4) &&
             dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8) &&
@@ -754,10 +763,14 @@ int jm_set_tuning(jm_handle* h, const jm_tuning* t) {
   if (t->sort_interval < 1) return fail(JM_ERR_ARG, "sort_interval must be >= 1");
   if (!tuning_supported(t->threads, t->ppt))
     return fail(JM_ERR_ARG, "unsupported (threads, ppt) combination");
+  if (t->sort_strip < 0 || t->sort_strip > h->ny)
+    return fail(JM_ERR_ARG, "sort_strip must be in [0, grid rows]");
   JM_TRY(jm_sync(h));
   const bool relayout = t->threads * t->ppt != h->tun.threads * h->tun.ppt ||
                         t->xcd_affinity != h->tun.xcd_affinity;
+  if (t->sort_strip != h->tun.sort_strip) h->since_sort = t->sort_interval;  // re-sort next
   h->tun = *t;
+  h->key_bits = sort_key_bits(h->nx, h->ny, t->sort_strip);
   h->drop_graphs();
   if (relayout && !h->layout_dirty) JM_TRY(build_chunks(h));
   return JM_OK;

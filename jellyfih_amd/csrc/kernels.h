@@ -279,6 +279,26 @@ __device__ __forceinline__ int window_row(int q, float rcp_ww) {
   return (int)(((float)q + 0.5f) * rcp_ww);
 }
 
+// The two cross-lane pre-sum steps on three values: c += (c of lane + 1) * f1, then
+// c += (c of lane + 2) * f2, within DPP rows of 16 lanes (bound_ctrl: a lane past the end
+// of its row reads 0). One v_fmac_f32_dpp per value and step (the compiler keeps a
+// separate v_mov_b32_dpp before each FMA). Runs with every lane active. The leading
+// s_nop 1 gives the two wait states a DPP read of a VGPR needs after a VALU write of it
+// (the three values are written just before); inside the block each DPP source was
+// written at least two instructions earlier.
+__device__ __forceinline__ void presum3(float& c0, float& c1, float& c2, float f1, float f2) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %1, %1, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %2, %2, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %0, %0, %4 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %1, %1, %4 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %2, %2, %4 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(c0), "+v"(c1), "+v"(c2)
+      : "v"(f1), "v"(f2));
+}
+
 // Rare path: a particle whose contributions leave the fixed-point range scatters its nine
 // nodes with fp32 global atomics. Node (i, j) receives g_j P0_i + e_j Q0_i,
 // g_j P1_i + e_j Q1_i, g_j M_i.
@@ -386,9 +406,13 @@ __device__ __forceinline__ void g2p_global(const float* gp, int G, const PhysCon
 
 template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
 __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
-  // Grid-window capacity (nodes): a chunk of P sorted particles spans about P/512 + 2 rows
-  // of 128 cells, plus the 2-node stencil halo and drift since the last sort.
-  constexpr int WMAX = (BLOCK * PPT <= 1024) ? 1024 : (BLOCK * PPT <= 2048) ? 1536 : 2560;
+  // Grid-window capacity (nodes): a chunk of P sorted particles (4 per cell in water)
+  // covers about P/4 cells (P/4/S columns of an S-row strip, or P/512 + 2 rows of 128 in
+  // row-major order), plus the 2-node stencil halo and drift since the last sort.
+  // 1280 nodes (30 KB of LDS) for 1024-particle chunks: five workgroups per CU, as many as
+  // the kernel's registers allow, with twice the typical window (~400-650 nodes) as
+  // headroom before a chunk falls back to the slow global path.
+  constexpr int WMAX = (BLOCK * PPT <= 1024) ? 1280 : (BLOCK * PPT <= 2048) ? 1536 : 2560;
   // One LDS buffer: the gather window as float2, then the scatter window as three planes of
   // 64-bit fixed-point accumulators.
   __shared__ unsigned long long lds64[3 * WMAX];
@@ -658,16 +682,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
         for (int ii = 0; ii < 3; ++ii) {
           float c[3] = {wy[jj] * P0[ii] + ey[jj] * Q0[ii], wy[jj] * P1[ii] + ey[jj] * Q1[ii],
                         wy[jj] * Mi[ii]};
-#pragma unroll
-          for (int e = 0; e < 3; ++e) {
-            // row_shl with bound_ctrl: lanes past the end of their row read 0
-            const float t1 = __int_as_float(
-                __builtin_amdgcn_mov_dpp(__float_as_int(c[e]), 0x101, 0xF, 0xF, true));
-            c[e] = __builtin_fmaf(t1, f1, c[e]);
-            const float t2 = __int_as_float(
-                __builtin_amdgcn_mov_dpp(__float_as_int(c[e]), 0x102, 0xF, 0xF, true));
-            c[e] = __builtin_fmaf(t2, f2, c[e]);
-          }
+          presum3(c[0], c[1], c[2], f1, f2);
           if (emit) {
             const int q = q0 + jj * ww + ii;
             add_fixed_scaled(&lds64[q], c[0]);
@@ -737,17 +752,29 @@ struct SortArgs {
   int cap;
   int M;
   int tmax;         // tiles per instance segment (cap / 1024, rounded up)
+  int strip;        // cell order: strips of `strip` rows (0: row-major), see cell_key
   PhysConsts k;
 };
 
-__device__ __forceinline__ int cell_key(float2 x, const PhysConsts& k) {
+// Sort key of a particle's P2G base cell (clamped into the grid). strip = 0: row-major
+// cell index. strip = S > 0: the grid is cut into strips of S cell rows; strips follow
+// each other upwards, the columns of even strips run left to right and those of odd
+// strips right to left, and the S cells of a column run upwards. A chunk of consecutive
+// sorted particles then covers a compact block of about S rows (its LDS window is about
+// half the nodes of a row-major chunk's, which spans full rows), and the chunk that turns
+// at the end of a strip continues right above. Every order keeps the particles of one
+// cell adjacent.
+__device__ __forceinline__ int cell_key(float2 x, const PhysConsts& k, int strip) {
   int b0, b1;
   float f0, f1;
   base_frac(x.x, k.inv_dx, b0, f0);
   base_frac(x.y, k.inv_dx, b1, f1);
   b0 = min(max(b0, 0), k.nx - 1);
   b1 = min(max(b1, 0), k.ny - 1);
-  return b1 * k.nx + b0;
+  if (strip <= 0) return b1 * k.nx + b0;
+  const int s = b1 / strip, r = b1 - s * strip;
+  const int xs = (s & 1) ? (k.nx - 1 - b0) : b0;
+  return (s * k.nx + xs) * strip + r;
 }
 
 // Keys of the current order, identity values, and a copy of the state to gather from.
@@ -757,7 +784,7 @@ __global__ void __launch_bounds__(256) k_sort_keys(SortArgs a) {
   if (local >= a.n_live[m]) return;
   const size_t i = (size_t)m * a.cap + local;
   const float2 x = a.s.x[i];
-  a.key_a[i] = cell_key(x, a.k);
+  a.key_a[i] = cell_key(x, a.k, a.strip);
   a.val_a[i] = local;
   a.tmp.x[i] = x;
   a.tmp.v[i] = a.s.v[i];

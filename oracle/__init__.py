@@ -182,20 +182,27 @@ def base_cells(consts: OracleConsts, x: np.ndarray) -> np.ndarray:
     return out
 
 
-def cell_keys(consts: OracleConsts, x: np.ndarray) -> np.ndarray:
-    """Sort key of each particle: row-major index of its P2G base cell, clamped into the grid."""
+def cell_keys(consts: OracleConsts, x: np.ndarray, strip: int = 8) -> np.ndarray:
+    """Sort key of each particle's P2G base cell, clamped into the grid (the engine's sort
+    order; the reference does not sort). ``strip`` = 0: row-major cell index. ``strip`` =
+    S > 0: strips of S cell rows, one after the other upwards; columns left to right in
+    even strips and right to left in odd ones; the S cells of a column upwards."""
     b = base_cells(consts, x)
     bx = np.clip(b[:, 0], 0, consts.nx - 1).astype(np.int64)
     by = np.clip(b[:, 1], 0, consts.ny - 1).astype(np.int64)
-    return (by * consts.nx + bx).astype(np.int32)
+    if strip <= 0:
+        return (by * consts.nx + bx).astype(np.int32)
+    s, r = by // strip, by % strip
+    xs = np.where(s % 2 == 1, consts.nx - 1 - bx, bx)
+    return ((s * consts.nx + xs) * strip + r).astype(np.int32)
 
 
-def stable_cell_sort(consts: OracleConsts, x: np.ndarray) -> np.ndarray:
-    """Permutation that stably sorts one instance's particles by P2G base cell.
+def stable_cell_sort(consts: OracleConsts, x: np.ndarray, strip: int = 8) -> np.ndarray:
+    """Permutation that stably sorts one instance's particles by ``cell_keys``.
 
     ``perm[i]`` is the current index of the particle that goes to position ``i``; ties
     keep their current relative order (a stable counting sort)."""
-    return np.argsort(cell_keys(consts, x), kind="stable").astype(np.int32)
+    return np.argsort(cell_keys(consts, x, strip), kind="stable").astype(np.int32)
 
 
 class OracleSim:

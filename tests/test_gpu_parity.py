@@ -114,21 +114,23 @@ def test_ragged_batch_200_substeps(require_gpu, path):
         assert eng.live_count(m) == int((orc.material[m] >= 0).sum())
 
 
-def test_sort_permutation_bitexact(require_gpu):
-    say("sort permutation vs stable counting sort")
+@pytest.mark.parametrize("strip", [8, 0, 5])
+def test_sort_permutation_bitexact(require_gpu, strip):
+    say(f"sort permutation vs stable counting sort, strip {strip}")
     eng, orc = make_pair(ragged_cases())
     perturb(eng, orc, seed=3, sv=0.5)
-    eng.set_tuning(sort_interval=1000000)   # no automatic sort inside the steps below
+    # no automatic sort inside the steps below (after the one a new order asks for)
+    eng.set_tuning(sort_interval=1000000, sort_strip=strip)
     eng.substep(50)
     c = oracle.reference_consts()
     for m in range(3):
         slot0, x0 = eng.device_order(m)
         eng.sort_now()
         slot1, x1 = eng.device_order(m)
-        perm = oracle.stable_cell_sort(c, x0)
+        perm = oracle.stable_cell_sort(c, x0, strip)
         np.testing.assert_array_equal(slot1, slot0[perm])
         np.testing.assert_array_equal(x1, x0[perm])
-        keys = oracle.cell_keys(c, x1)
+        keys = oracle.cell_keys(c, x1, strip)
         assert np.all(np.diff(keys) >= 0)
 
 

@@ -333,13 +333,38 @@ def test_compute_fitness_closed_form():
     assert oracle.compute_fitness(res[2:], muscles[2:], None) == [100.0, 100.0]
 
 
-def test_stable_cell_sort_is_a_stable_counting_sort(c):
+def _walk_order(nx, ny, strip):
+    """Rank of every cell (bx, by) in the documented sort order, built by walking it."""
+    rank, n = {}, 0
+    if strip <= 0:
+        for by in range(ny):
+            for bx in range(nx):
+                rank[bx, by] = n
+                n += 1
+        return rank
+    for s in range((ny + strip - 1) // strip):
+        cols = range(nx) if s % 2 == 0 else range(nx - 1, -1, -1)
+        for bx in cols:
+            for by in range(s * strip, min((s + 1) * strip, ny)):
+                rank[bx, by] = n
+                n += 1
+    return rank
+
+
+@pytest.mark.parametrize("strip", [0, 8, 5])
+def test_stable_cell_sort_is_a_stable_counting_sort(c, strip):
     rng = np.random.default_rng(4)
     x = rng.uniform(0.0, 1.0, size=(3000, 2)).astype(np.float32)
     x[::7] = x[3]                                      # many ties
-    keys = oracle.cell_keys(c, x)
-    perm = oracle.stable_cell_sort(c, x)
+    keys = oracle.cell_keys(c, x, strip)
+    perm = oracle.stable_cell_sort(c, x, strip)
     brute_perm = sorted(range(len(x)), key=lambda i: (int(keys[i]), i))
     np.testing.assert_array_equal(perm, brute_perm)
     b = np.clip(np.trunc(x.astype(np.float64) * 128 - 0.5), 0, 127).astype(int)
-    np.testing.assert_array_equal(keys, b[:, 1] * 128 + b[:, 0])
+    rank = _walk_order(128, 128, strip)
+    order = np.array([rank[bx, by] for bx, by in b])
+    if strip in (0, 8):   # 128 rows divide into strips: the keys are exactly the ranks
+        np.testing.assert_array_equal(keys, order)
+    else:                 # a short last strip leaves gaps in the keys, not in their order
+        np.testing.assert_array_equal(np.argsort(keys, kind="stable"),
+                                      np.argsort(order, kind="stable"))
