@@ -189,19 +189,20 @@ __device__ __forceinline__ double next_lane(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// 64-bit fixed point for the fp32 kernels' P2G tile: c * 2^50 as a two's-complement int64
-// for |c| < 1, hi = floor(c 2^18), lo = frac(c 2^18) 2^32: exact up to the truncation of
-// lo, below one unit (finer than fp32 rounding of any contribution above 2^-26). A node of
-// a workgroup's tile receives at most 256 contributions (one per particle), so its sum
-// stays below 2^58 units: no overflow. Larger contributions (and NaN) go to the global
-// grid. The fp64 kernels accumulate in fp64 instead (fixed point would be coarser than
-// fp64 rounding).
+// 64-bit fixed point for the fp32 kernels' P2G tile: c * 2^50 rounded to the nearest
+// integer, as two's-complement int64 bits, for |c| < 1 (an error of at most 2^-51, finer
+// than fp32 rounding of any contribution above 2^-27). c * 2^50 + 1.5 * 2^52 in double
+// (one FMA, exact but for that rounding) lies in [2^52, 2^53), where the mantissa holds
+// c * 2^50 + 2^51: the low word is the integer's low word, the high word minus the
+// exponent and the 2^51 bit (0x43380000) its high word. A node of a workgroup's tile
+// receives at most 256 contributions (one per particle), so its sum stays below 2^58
+// units: no overflow. Larger contributions (and NaN) go to the global grid. The fp64
+// kernels accumulate in fp64 instead (fixed point would be coarser than fp64 rounding).
 __device__ __forceinline__ unsigned long long to_fixed(float c) {
-  const float y = c * 262144.0f;                   // 2^18
-  const unsigned lo = (unsigned)(__builtin_amdgcn_fractf(y) * 4294967296.0f);
-  int hi;
-  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(y));
-  return ((unsigned long long)(unsigned)hi << 32) | lo;
+  const unsigned long long b =
+      __double_as_longlong(__fma_rn((double)c, 1125899906842624.0, 6755399441055744.0));
+  const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
+  return ((unsigned long long)hi << 32) | (unsigned)b;
 }
 __device__ __forceinline__ float from_fixed(unsigned long long v) {
   return float((double)(long long)v * (1.0 / 1125899906842624.0));  // 2^-50

@@ -606,10 +606,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
           a.s.C[gi] = C[p];
         }
       }
-      float P0[3] = {0.f, 0.f, 0.f}, Q0[3] = {0.f, 0.f, 0.f}, P1[3] = {0.f, 0.f, 0.f},
-            Q1[3] = {0.f, 0.f, 0.f}, Mi[3] = {0.f, 0.f, 0.f}, ey[3] = {0.f, 0.f, 0.f},
-            wy[3] = {0.f, 0.f, 0.f};
-      bool fixed_ok = false;
+      // Particle update for in-range particles; every other lane scatters zeros, computed
+      // from finite stand-ins (mass 0, affine 0, v 0, and its base cell's weights) rather
+      // than by clearing the 21 per-particle factors.
+      float mass = 0.f;
+      Mat2 aff = {0.f, 0.f, 0.f, 0.f};
       if (inr) {
         float2 fib = make_float2(0.f, 1.f);
         if (mt[p] == 3) fib = a.s.fiber[gi];
@@ -618,38 +619,40 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
                               pdens, k);
         a.s.F[gi] = as_f4(o.F);
         if (o.jp_changes) a.s.Jp[gi] = (a.s.Jp[gi] * o.q0) * o.q1;
-        const float mv0 = o.mass * v[p].x, mv1 = o.mass * v[p].y;
-        float wx[3];
-        bspline(fx0, wx);
-        bspline(fx1, wy);
-        const float bnd = 1.5f * k.dx;
-        const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(o.affine.a00) + fabsf(o.affine.a01)),
-                                      fabsf(mv1) + bnd * (fabsf(o.affine.a10) + fabsf(o.affine.a11))),
-                                o.mass);
-        fixed_ok = use_lds && lim < kFixMax;
-        const float sc = fixed_ok ? kFixPre : 1.0f;
+        mass = o.mass;
+        aff = o.affine;
+      }
+      const float mv0 = mass * (inr ? v[p].x : 0.f), mv1 = mass * (inr ? v[p].y : 0.f);
+      float wx[3], wy[3], ey[3], P0[3], Q0[3], P1[3], Q1[3], Mi[3];
+      bspline(fx0, wx);
+      bspline(fx1, wy);
+      const float bnd = 1.5f * k.dx;
+      const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(aff.a00) + fabsf(aff.a01)),
+                                    fabsf(mv1) + bnd * (fabsf(aff.a10) + fabsf(aff.a11))),
+                              mass);
+      const bool fixed_ok = inr && use_lds && lim < kFixMax;
+      const float sc = fixed_ok ? kFixPre : (inr ? 1.0f : 0.0f);
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const float ai = wx[ii] * sc;
+        const float bi = ai * (((float)ii - fx0) * k.dx);
+        P0[ii] = ai * mv0 + bi * aff.a00;
+        Q0[ii] = ai * aff.a01;
+        P1[ii] = ai * mv1 + bi * aff.a10;
+        Q1[ii] = ai * aff.a11;
+        Mi[ii] = ai * mass;
+      }
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * (((float)jj - fx1) * k.dx);
+      if (inr && !fixed_ok) {
+        scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
+                       {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
+                       {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
+        // done: its neighbour lane adds this lane's (now zero, finite) terms times 0
 #pragma unroll
         for (int ii = 0; ii < 3; ++ii) {
-          const float ai = wx[ii] * sc;
-          const float bi = ai * (((float)ii - fx0) * k.dx);
-          P0[ii] = ai * mv0 + bi * o.affine.a00;
-          Q0[ii] = ai * o.affine.a01;
-          P1[ii] = ai * mv1 + bi * o.affine.a10;
-          Q1[ii] = ai * o.affine.a11;
-          Mi[ii] = ai * o.mass;
-        }
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * (((float)jj - fx1) * k.dx);
-        if (!fixed_ok) {
-          scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
-                         {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
-                         {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
-          // done: its neighbour lane adds this lane's (now zero, finite) terms times 0
-#pragma unroll
-          for (int ii = 0; ii < 3; ++ii) {
-            P0[ii] = Q0[ii] = P1[ii] = Q1[ii] = Mi[ii] = 0.f;
-            wy[ii] = ey[ii] = 0.f;
-          }
+          P0[ii] = Q0[ii] = P1[ii] = Q1[ii] = Mi[ii] = 0.f;
+          wy[ii] = ey[ii] = 0.f;
         }
       }
       // Cell-sorted particles of one base cell sit in adjacent lanes (water: 4 per cell).

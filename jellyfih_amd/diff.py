@@ -59,6 +59,25 @@ def clock(t0: float, n: int, dt=None, fp64: bool = False) -> float:
     return float(t)
 
 
+def packing_order(pos, mat, order: str, nx: int, ny: int):
+    """(instance, slot) of every live particle (``mat >= 0``) in packing order: instance
+    by instance, and within an instance by slot (``order="slot"``) or by grid cell in
+    snaking strips of 8 cell rows, ties by slot (``order="cell"``)."""
+    N = mat.shape[1]
+    inst, slot = np.divmod(np.flatnonzero(mat >= 0), N)
+    if order == "slot":
+        return inst, slot
+    if order != "cell":
+        raise ValueError("order must be 'cell' or 'slot'")
+    x = np.asarray(pos, np.float32)[inst, slot]
+    b = np.trunc(x * np.float32(nx) - np.float32(0.5)).astype(np.int64)
+    bx, by = np.clip(b[:, 0], 0, nx - 1), np.clip(b[:, 1], 0, ny - 1)
+    s, r = by // 8, by % 8
+    key = (s * nx + np.where(s % 2 == 1, nx - 1 - bx, bx)) * 8 + r
+    perm = np.lexsort((slot, key, inst))
+    return inst[perm], slot[perm]
+
+
 class DiffTank:
     """A batch of ``n_instances`` tanks packed as one particle array, with the device grids
     and scratch the forward and adjoint substeps need.
@@ -123,20 +142,29 @@ class DiffTank:
         self._L = load()
 
     @classmethod
-    def from_tanks(cls, pos, mat, fiber, **kw):
+    def from_tanks(cls, pos, mat, fiber, order: str = "cell", **kw):
         """Pack reference-layout tanks (``pos (M,N,2)``, ``mat (M,N)`` with -1 for dead
         slots, ``fiber (M,N,2)``) into a DiffTank and its initial state (v = 0, C = 0,
-        F = I, as ``load_particles`` sets them, ``mpm_sim.py:715-736``)."""
+        F = I, as ``load_particles`` sets them, ``mpm_sim.py:715-736``).
+
+        ``order="cell"`` packs each instance's live particles sorted by grid cell, in
+        snaking strips of 8 cell rows (the fused engine's sort order): a workgroup's
+        particles then touch a compact window of nodes and a cell's particles sit in
+        adjacent lanes, which the kernels' LDS tiles and cross-lane pre-sums exploit.
+        ``order="slot"`` keeps slot order. ``tank.slots`` (P, 2) gives the (instance,
+        slot) of every packed particle either way."""
         pos = np.asarray(pos, np.float32)
         mat = np.asarray(mat, np.int32)
         fiber = np.asarray(fiber, np.float32)
         dtype = kw.get("dtype", torch.float32)
         M = mat.shape[0]
-        live = mat >= 0
-        inst = np.repeat(np.arange(M, dtype=np.int32)[:, None], mat.shape[1], 1)[live]
-        tank = cls(inst, mat[live], fiber[live], M, **kw)
-        state = np.zeros((int(live.sum()), STATE_COLS), np.float32)
-        state[:, 0:2] = pos[live]
+        nx = 128 * int(kw.get("quality", 1))
+        inst_all, slot_all = packing_order(pos, mat, order, nx, int(kw.get("n_grid_y") or nx))
+        tank = cls(inst_all.astype(np.int32), mat[inst_all, slot_all],
+                   fiber[inst_all, slot_all], M, **kw)
+        tank.slots = np.stack([inst_all, slot_all], 1)
+        state = np.zeros((inst_all.size, STATE_COLS), np.float32)
+        state[:, 0:2] = pos[inst_all, slot_all]
         state[:, 8] = 1.0
         state[:, 11] = 1.0
         return tank, torch.from_numpy(state).to(tank.device, dtype)

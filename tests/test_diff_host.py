@@ -45,3 +45,24 @@ def test_clock_is_fp32_accumulation():
     3.0033383 (not 3.0), and blocks compose."""
     assert np.float32(clock(0.0, 150000)) == np.float32(3.0033383)
     assert clock(clock(0.0, 700), 1300) == clock(0.0, 2000)
+
+
+def test_packing_order_matches_engine_sort_order():
+    """``order="cell"`` packs each instance like the engine's stable cell sort (strips of
+    8 rows, the oracle's ``stable_cell_sort``); ``order="slot"`` keeps slot order."""
+    import oracle
+    from jellyfih_amd.diff import packing_order
+    rng = np.random.default_rng(3)
+    M, N = 3, 500
+    pos = rng.uniform(0.0, 1.0, size=(M, N, 2)).astype(np.float32)
+    mat = rng.integers(-1, 4, size=(M, N)).astype(np.int32)
+    inst, slot = packing_order(pos, mat, "slot", 128, 128)
+    assert np.all(mat[inst, slot] >= 0) and inst.size == int((mat >= 0).sum())
+    assert np.all(np.diff(inst * N + slot) > 0)
+    inst, slot = packing_order(pos, mat, "cell", 128, 128)
+    c = oracle.reference_consts()
+    for m in range(M):
+        live = np.flatnonzero(mat[m] >= 0)
+        expect = live[oracle.stable_cell_sort(c, pos[m, live], 8)]
+        np.testing.assert_array_equal(slot[inst == m], expect)
+    assert np.all(np.diff(inst) >= 0)
