@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JM_ABI_VERSION 1
+#define JM_ABI_VERSION 2
 
 typedef enum jm_status {
   JM_OK = 0,
@@ -98,6 +98,10 @@ typedef struct jm_tuning {
   int32_t sort_strip;       /* cell order of the sort: strips of this many cell rows, the
                                columns of alternate strips in opposite directions, rows
                                within a column (default 8); 0: row-major cells */
+  int32_t instance_groups;  /* fused path: instances split into this many groups (1..4,
+                               default 2) whose substep kernels run concurrently on their
+                               own streams, so one group's launch fills the CUs that the
+                               other's tail leaves idle */
 } jm_tuning;
 
 typedef struct jm_handle jm_handle;

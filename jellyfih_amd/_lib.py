@@ -12,7 +12,7 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libjellymx.so")
 
-JM_ABI_VERSION = 1
+JM_ABI_VERSION = 2
 
 # jm_status
 JM_OK, JM_ERR_ARG, JM_ERR_HIP, JM_ERR_STATE, JM_ERR_NOMEM = range(5)
@@ -48,7 +48,7 @@ class JmConfig(ctypes.Structure):
         ("n_grid_y", ctypes.c_int32),
         ("track_glow", ctypes.c_int32),
         ("path", ctypes.c_int32),
-        ("sort_strip", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("domain_height", ctypes.c_double),
     ]
 
@@ -63,6 +63,7 @@ class JmTuning(ctypes.Structure):
         ("xcd_affinity", ctypes.c_int32),
         ("lazy_vc", ctypes.c_int32),
         ("sort_strip", ctypes.c_int32),
+        ("instance_groups", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

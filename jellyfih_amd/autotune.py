@@ -1,7 +1,7 @@
 """Autotuner for the fused path: picks workgroup size, particles per thread (the chunk of
 cell-sorted particles one workgroup owns, which sets the LDS grid-window shape), the sort
-interval K and cell order (strip height), the graph block length, and the XCD-affinity /
-lazy-v,C switches.
+interval K and cell order (strip height), the graph block length, the number of
+concurrently launched instance groups, and the XCD-affinity / lazy-v,C switches.
 
 It measures each candidate on the caller's own workload (an Engine already loaded with it)
 with HIP-event-timed blocks, keeps the fastest, and can write it to ``tuning.json`` beside
@@ -70,6 +70,8 @@ def tune(eng, live: int, substeps: int = 1000, log=print) -> dict:
         trial(sort_strip=st)
     for b in BLOCKS:
         trial(block_substeps=b)
+    for ng in (1, 2, 3):
+        trial(instance_groups=ng)
     for flag in ("lazy_vc", "xcd_affinity"):
         trial(**{flag: 1 - best[flag]})
     log(f"best {best} -> {best_rate:.3e}")

@@ -44,6 +44,15 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kNumXcd = 8;
+constexpr int kMaxGroups = 4;  // concurrent instance groups of the fused path
+
+// One instance group's chunk table: its chunks are chunks_d[off, off + cnt), laid out for
+// its own launch (blockIdx -> chunk, XCD-affine when the group has >= 8 instances).
+struct GroupChunks {
+  int off = 0, cnt = 0, grid_blocks = 0;
+  bool affinity = false;
+  int xcd_off[kNumXcd] = {0}, xcd_cnt[kNumXcd] = {0};
+};
 
 struct DevState {
   jm::State s;
@@ -66,7 +75,7 @@ struct jm_handle {
   int tmax = 0;            // radix tiles per instance segment
   int key_bits = 0;        // bits of the largest cell key
   float* acc = nullptr;       // 3 grid buffers of M*3*G floats
-  float* t_buf = nullptr;     // 2 floats
+  float* t_buf = nullptr;     // 2 floats per instance group (clock ping-pong), group 0 first
   float* params = nullptr;    // 6 * M floats: amp, contraction, freq, pdens, mu_j, la_j
   int* n_live_d = nullptr;
   int4* chunks_d = nullptr;
@@ -87,9 +96,13 @@ struct jm_handle {
   // chunk tables
   int chunk_size = 0;
   int total_chunks = 0;
-  int grid_blocks = 0;
-  bool affinity = false;
-  int xcd_off[kNumXcd] = {0}, xcd_cnt[kNumXcd] = {0};
+  int n_groups = 1;                 // instance groups of the current chunk tables
+  GroupChunks grp[kMaxGroups];
+  // Group g > 0 runs on gstream[g]; group 0 on `stream`, which forks to the others
+  // (fork_ev) and joins them (gdone) around every fused block.
+  hipStream_t gstream[kMaxGroups] = {nullptr};
+  hipEvent_t gdone[kMaxGroups] = {nullptr};
+  hipEvent_t fork_ev = nullptr;
 
   // rotation / bookkeeping
   int acc_base = 0;
@@ -119,6 +132,14 @@ struct jm_handle {
     n_live_d = nullptr;
     chunks_d = nullptr;
     nchunks_d = nullptr;
+    for (int g = 1; g < kMaxGroups; ++g) {
+      if (gdone[g]) (void)hipEventDestroy(gdone[g]);
+      if (gstream[g]) (void)hipStreamDestroy(gstream[g]);
+      gdone[g] = nullptr;
+      gstream[g] = nullptr;
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    fork_ev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -226,59 +247,75 @@ int upload_params(jm_handle* h) {
   return JM_OK;
 }
 
-// Assign chunks of `chunk_size` sorted particles to workgroups. With XCD affinity every
-// chunk of one instance runs on the same XCD (workgroup i is dispatched to XCD i % 8), so
-// its grid's atomics and gathers stay in that XCD's L2; instances are spread over XCDs
-// longest-first to balance the chunk counts.
+// Assign chunks of `chunk_size` sorted particles to workgroups. Instances are split into
+// `instance_groups` groups balanced by chunk count (longest first); each group is one
+// launch per substep, and the groups' launches run concurrently. Within a group, with XCD
+// affinity every chunk of one instance runs on the same XCD (workgroup i is dispatched to
+// XCD i % 8), so its grid's atomics and gathers stay in that XCD's L2; the group's
+// instances are spread over XCDs longest-first to balance the chunk counts.
 int build_chunks(jm_handle* h) {
   const int M = h->M;
   const int cs = h->tun.threads * h->tun.ppt;
   h->chunk_size = cs;
   std::vector<int> nch(M);
   for (int m = 0; m < M; ++m) nch[m] = (h->n_live[m] + cs - 1) / cs;
-  h->affinity = h->tun.xcd_affinity && M >= kNumXcd;
-  std::vector<std::vector<int4>> per_xcd(kNumXcd);
-  std::vector<int4> flat;
-  std::vector<int> order(M);
-  std::iota(order.begin(), order.end(), 0);
+  std::vector<int> order;  // instances with live particles, longest first
+  for (int m = 0; m < M; ++m)
+    if (nch[m] > 0) order.push_back(m);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nch[a] > nch[b]; });
-  std::vector<int64_t> load(kNumXcd, 0);
-  for (int m : order) {
-    std::vector<int4> mine;
-    for (int c = 0; c < nch[m]; ++c) {
-      int start = m * h->cap + c * cs;
-      int count = std::min(cs, h->n_live[m] - c * cs);
-      mine.push_back(make_int4(m, start, count, c));
-    }
-    if (h->affinity) {
-      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      load[x] += nch[m];
-      per_xcd[x].insert(per_xcd[x].end(), mine.begin(), mine.end());
-    } else {
-      flat.insert(flat.end(), mine.begin(), mine.end());
+  const int ng = std::max(1, std::min<int>(h->tun.instance_groups, (int)order.size()));
+  std::vector<std::vector<int>> members(ng);
+  {
+    std::vector<int64_t> load(ng, 0);
+    for (int m : order) {
+      const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[g] += nch[m];
+      members[g].push_back(m);  // longest first within the group too
     }
   }
-  std::vector<int4> all;
-  if (h->affinity) {
-    int mx = 0;
-    for (int x = 0; x < kNumXcd; ++x) {
-      h->xcd_off[x] = (int)all.size();
-      h->xcd_cnt[x] = (int)per_xcd[x].size();
-      mx = std::max(mx, h->xcd_cnt[x]);
-      all.insert(all.end(), per_xcd[x].begin(), per_xcd[x].end());
+  auto chunks_of = [&](int m) {
+    std::vector<int4> mine;
+    for (int c = 0; c < nch[m]; ++c) {
+      const int start = m * h->cap + c * cs;
+      const int count = std::min(cs, h->n_live[m] - c * cs);
+      mine.push_back(make_int4(m, start, count, c));
     }
-    h->grid_blocks = kNumXcd * mx;
-  } else {
-    // keep instance order for the non-affine mapping
-    all.clear();
-    for (int m = 0; m < M; ++m)
-      for (int c = 0; c < nch[m]; ++c) {
-        int start = m * h->cap + c * cs;
-        int count = std::min(cs, h->n_live[m] - c * cs);
-        all.push_back(make_int4(m, start, count, c));
+    return mine;
+  };
+  std::vector<int4> all;
+  h->n_groups = ng;
+  for (int g = 0; g < kMaxGroups; ++g) h->grp[g] = GroupChunks{};
+  for (int g = 0; g < ng; ++g) {
+    GroupChunks& gc = h->grp[g];
+    gc.off = (int)all.size();
+    gc.affinity = h->tun.xcd_affinity && (int)members[g].size() >= kNumXcd;
+    if (gc.affinity) {
+      std::vector<std::vector<int4>> per_xcd(kNumXcd);
+      std::vector<int64_t> load(kNumXcd, 0);
+      for (int m : members[g]) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += nch[m];
+        std::vector<int4> mine = chunks_of(m);
+        per_xcd[x].insert(per_xcd[x].end(), mine.begin(), mine.end());
       }
-    for (int x = 0; x < kNumXcd; ++x) h->xcd_off[x] = h->xcd_cnt[x] = 0;
-    h->grid_blocks = (int)all.size();
+      int mx = 0;
+      for (int x = 0; x < kNumXcd; ++x) {
+        gc.xcd_off[x] = (int)all.size() - gc.off;
+        gc.xcd_cnt[x] = (int)per_xcd[x].size();
+        mx = std::max(mx, gc.xcd_cnt[x]);
+        all.insert(all.end(), per_xcd[x].begin(), per_xcd[x].end());
+      }
+      gc.grid_blocks = kNumXcd * mx;
+    } else {
+      std::vector<int> ms = members[g];
+      std::sort(ms.begin(), ms.end());  // instance order for the non-affine mapping
+      for (int m : ms) {
+        std::vector<int4> mine = chunks_of(m);
+        all.insert(all.end(), mine.begin(), mine.end());
+      }
+      gc.grid_blocks = (int)all.size() - gc.off;
+    }
+    gc.cnt = (int)all.size() - gc.off;
   }
   h->total_chunks = (int)all.size();
   if (h->chunks_d) (void)hipFree(h->chunks_d);
@@ -290,6 +327,11 @@ int build_chunks(jm_handle* h) {
   }
   HIP_TRY(hipMemcpyAsync(h->nchunks_d, nch.data(), M * sizeof(int), hipMemcpyHostToDevice,
                          h->stream));
+  // Every group runs every block, so the groups' clocks stay equal from here on; a new
+  // grouping starts them all from group 0's clock.
+  for (int g = 1; g < ng; ++g)
+    HIP_TRY(hipMemcpyAsync(h->t_buf + 2 * g, h->t_buf, 2 * sizeof(float),
+                           hipMemcpyDeviceToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return JM_OK;
 }
@@ -402,20 +444,22 @@ int enqueue_sort(jm_handle* h) {
   return JM_OK;
 }
 
-jm::FusedArgs fused_args(jm_handle* h) {
+// Kernel arguments of instance group g: its chunk table and its own clock pair.
+jm::FusedArgs fused_args(jm_handle* h, int g) {
+  const GroupChunks& gc = h->grp[g];
   jm::FusedArgs a;
   std::memset(&a, 0, sizeof(a));
   a.s = h->st.s;
   a.ip = inst_params(h);
-  a.t_buf = h->t_buf;
-  a.chunks = h->chunks_d;
+  a.t_buf = h->t_buf + 2 * g;
+  a.chunks = h->chunks_d + gc.off;
   a.nchunks = h->nchunks_d;
   for (int x = 0; x < kNumXcd; ++x) {
-    a.xcd_off[x] = h->xcd_off[x];
-    a.xcd_cnt[x] = h->xcd_cnt[x];
+    a.xcd_off[x] = gc.xcd_off[x];
+    a.xcd_cnt[x] = gc.xcd_cnt[x];
   }
-  a.affinity = h->affinity ? 1 : 0;
-  a.total_chunks = h->total_chunks;
+  a.affinity = gc.affinity ? 1 : 0;
+  a.total_chunks = gc.cnt;
   a.G = h->G;
   a.lazy_vc = h->tun.lazy_vc;
   a.glow = h->cfg.track_glow;
@@ -423,15 +467,18 @@ jm::FusedArgs fused_args(jm_handle* h) {
   return a;
 }
 
+hipStream_t group_stream(jm_handle* h, int g) { return g == 0 ? h->stream : h->gstream[g]; }
+
 template <int B, int P>
-void launch_fused_t(jm_handle* h, bool g2p, bool p2g, const jm::FusedArgs& a) {
-  dim3 grid(std::max(1, h->grid_blocks)), block(B);
+void launch_fused_t(jm_handle* h, int g, bool g2p, bool p2g, const jm::FusedArgs& a) {
+  dim3 grid(std::max(1, h->grp[g].grid_blocks)), block(B);
+  hipStream_t st = group_stream(h, g);
   if (g2p && p2g)
-    hipLaunchKernelGGL((jm::k_fused<B, P, true, true>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((jm::k_fused<B, P, true, true>), grid, block, 0, st, a);
   else if (g2p)
-    hipLaunchKernelGGL((jm::k_fused<B, P, true, false>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((jm::k_fused<B, P, true, false>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((jm::k_fused<B, P, false, true>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL((jm::k_fused<B, P, false, true>), grid, block, 0, st, a);
 }
 
 bool tuning_supported(int threads, int ppt) {
@@ -442,10 +489,10 @@ bool tuning_supported(int threads, int ppt) {
   return false;
 }
 
-void launch_fused(jm_handle* h, bool g2p, bool p2g, const jm::FusedArgs& a) {
+void launch_fused(jm_handle* h, int g, bool g2p, bool p2g, const jm::FusedArgs& a) {
   const int T = h->tun.threads, P = h->tun.ppt;
 #define JM_CASE(TT, PP) \
-  if (T == TT && P == PP) return launch_fused_t<TT, PP>(h, g2p, p2g, a);
+  if (T == TT && P == PP) return launch_fused_t<TT, PP>(h, g, g2p, p2g, a);
   JM_CASE(128, 8)
   JM_CASE(128, 16)
   JM_CASE(256, 2)
@@ -460,12 +507,14 @@ void launch_fused(jm_handle* h, bool g2p, bool p2g, const jm::FusedArgs& a) {
 
 float* grid_buf(jm_handle* h, int b) { return h->acc + (size_t)b * h->M * 3 * h->G; }
 
-// Kernel kk (0..n) of a block of n fused substeps with grid-buffer base b: it gathers
-// from buffer (b+kk-1)%3, scatters into (b+kk)%3 and zeroes (b+kk+1)%3 — the next kernel's
-// scatter target, last read two kernels ago. Kernel 0 only scatters (P2G of substep 0),
-// kernel n only gathers (G2P of substep n-1). The clock ping-pongs between t_buf[0] and
-// t_buf[1] and is back in t_buf[0] after the block.
-void enqueue_fused_kernel(jm_handle* h, jm::FusedArgs& a, int kk, int n, int b) {
+// Kernel kk (0..n) of a block of n fused substeps with grid-buffer base b, for instance
+// group g: it gathers from buffer (b+kk-1)%3, scatters into (b+kk)%3 and zeroes (b+kk+1)%3
+// — the next kernel's scatter target, last read two kernels ago (each workgroup touches only
+// its own instance's planes, so the groups never share memory). Kernel 0 only scatters
+// (P2G of substep 0), kernel n only gathers (G2P of substep n-1). The group's clock
+// ping-pongs between its t_buf[0] and t_buf[1] and is back in t_buf[0] after the block;
+// every group advances its clock by the same fp32 additions, so the clocks stay equal.
+void enqueue_fused_kernel(jm_handle* h, int g, jm::FusedArgs& a, int kk, int n, int b) {
   const bool g2p = kk > 0, p2g = kk < n;
   a.acc_r = g2p ? grid_buf(h, (b + kk - 1) % 3) : nullptr;
   a.acc_w = p2g ? grid_buf(h, (b + kk) % 3) : nullptr;
@@ -483,15 +532,32 @@ void enqueue_fused_kernel(jm_handle* h, jm::FusedArgs& a, int kk, int n, int b) 
     a.t_wr = -1;
     a.t_adv = 0;
   }
-  launch_fused(h, g2p, p2g, a);
+  launch_fused(h, g, g2p, p2g, a);
 }
 
-int enqueue_fused_block(jm_handle* h, int n, int b, bool sort) {
-  if (sort) JM_TRY(enqueue_sort(h));
-  if (h->total_chunks == 0) return JM_OK;
-  jm::FusedArgs a = fused_args(h);
-  for (int kk = 0; kk <= n; ++kk) enqueue_fused_kernel(h, a, kk, n, b);
+// Kernels kk0..kk1 of a block for group g, on the group's stream.
+int enqueue_group_kernels(jm_handle* h, int g, int kk0, int kk1, int n, int b) {
+  if (h->grp[g].cnt == 0) return JM_OK;
+  jm::FusedArgs a = fused_args(h, g);
+  for (int kk = kk0; kk <= kk1; ++kk) enqueue_fused_kernel(h, g, a, kk, n, b);
   HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+// Groups g > 0 start after the work already on the main stream.
+int fork_groups(jm_handle* h) {
+  if (h->n_groups < 2) return JM_OK;
+  HIP_TRY(hipEventRecord(h->fork_ev, h->stream));
+  for (int g = 1; g < h->n_groups; ++g) HIP_TRY(hipStreamWaitEvent(h->gstream[g], h->fork_ev, 0));
+  return JM_OK;
+}
+
+// The main stream continues after every group's work.
+int join_groups(jm_handle* h) {
+  for (int g = 1; g < h->n_groups; ++g) {
+    HIP_TRY(hipEventRecord(h->gdone[g], h->gstream[g]));
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->gdone[g], 0));
+  }
   return JM_OK;
 }
 
@@ -503,36 +569,58 @@ void finish_block(jm_handle* h, int n, int b, bool sort) {
   h->grid_is_velocity = false;
 }
 
+// Captured graph of group g's n + 1 kernels of a block with grid-buffer base b.
+int group_graph(jm_handle* h, int g, int n, int b, hipGraphExec_t* out) {
+  auto key = std::make_tuple(n, b, g);
+  auto it = h->graphs.find(key);
+  if (it == h->graphs.end()) {
+    hipStream_t st = group_stream(h, g);
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_group_kernels(h, g, 0, n, n, b);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &gr);
+    if (rc != JM_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return fail(JM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess)
+      return fail(JM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    it = h->graphs.emplace(key, ex).first;
+  }
+  *out = it->second;
+  return JM_OK;
+}
+
+// Blocks of up to block_substeps substeps: the sort (every sort_interval substeps) on the
+// main stream, then each instance group's n + 1 kernels on its own stream, concurrently,
+// joined back into the main stream. With graphs, each group's kernels of a block are one
+// captured graph per (n, grid-buffer base, group), captured before the block is enqueued.
 int run_fused(jm_handle* h, int64_t steps) {
   int64_t left = steps;
   while (left > 0) {
     const int n = (int)std::min<int64_t>(left, h->tun.block_substeps);
     const bool sort = h->since_sort >= h->tun.sort_interval;
     const int b = h->acc_base;
-    if (h->tun.use_graphs) {
-      auto key = std::make_tuple(n, b, sort ? 1 : 0);
-      auto it = h->graphs.find(key);
-      if (it == h->graphs.end()) {
-        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_fused_block(h, n, b, sort);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(h->stream, &g);
-        if (rc != JM_OK) {
-          if (g) (void)hipGraphDestroy(g);
-          return rc;
-        }
-        if (e != hipSuccess)
-          return fail(JM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        hipGraphExec_t ex = nullptr;
-        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess)
-          return fail(JM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        it = h->graphs.emplace(key, ex).first;
+    hipGraphExec_t ex[kMaxGroups] = {nullptr};
+    if (h->tun.use_graphs && h->total_chunks > 0)
+      for (int g = 0; g < h->n_groups; ++g)
+        if (h->grp[g].cnt > 0) JM_TRY(group_graph(h, g, n, b, &ex[g]));
+    if (sort) JM_TRY(enqueue_sort(h));
+    if (h->total_chunks > 0) {
+      JM_TRY(fork_groups(h));
+      for (int g = 0; g < h->n_groups; ++g) {
+        if (h->grp[g].cnt == 0) continue;
+        if (ex[g])
+          HIP_TRY(hipGraphLaunch(ex[g], group_stream(h, g)));
+        else
+          JM_TRY(enqueue_group_kernels(h, g, 0, n, n, b));
       }
-      HIP_TRY(hipGraphLaunch(it->second, h->stream));
-    } else {
-      JM_TRY(enqueue_fused_block(h, n, b, sort));
+      JM_TRY(join_groups(h));
     }
     finish_block(h, n, b, sort);
     left -= n;
@@ -693,7 +781,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   h->G = nx * ny;
   h->cap = cfg->n_particles;
   h->k = make_consts(*cfg, nx, ny);
-  h->tun = {100, 100, 256, 4, 1, 1, 1, 8};
+  h->tun = {100, 100, 256, 4, 1, 1, 1, 8, 2};
   const int M = h->M;
   h->h_param[JM_PARAM_ACTUATION].assign(M, 500.0f);       // mpm_sim.py:146
   h->h_param[JM_PARAM_CONTRACTION].assign(M, 0.20f);      // :148
@@ -711,8 +799,13 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
     delete h;
     return rc;
   };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess)
     return cleanup(fail(JM_ERR_HIP, "hipStreamCreateWithFlags"));
+  for (int g = 1; g < kMaxGroups; ++g)  // default priority, like the main stream
+    if (hipStreamCreateWithFlags(&h->gstream[g], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->gdone[g], hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(JM_ERR_HIP, "instance group streams"));
   const size_t n = (size_t)M * h->cap;
   const bool glow = cfg->track_glow != 0;
   int rc = alloc_state(h->st, n, glow);
@@ -725,7 +818,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   h->key_bits = sort_key_bits(nx, ny, h->tun.sort_strip);
   bool ok = dm((void**)&h->sortbuf, n * 5 * 4) &&
             dm((void**)&h->hist, (size_t)M * 256 * h->tmax * 4) &&
-            dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8) &&
+            dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8 * kMaxGroups) &&
             dm((void**)&h->params, (size_t)6 * M * 4) && dm((void**)&h->n_live_d, M * 4) &&
             dm((void**)&h->nchunks_d, M * 4) && dm((void**)&h->stats_d, (size_t)M * 6 * 4) &&
             dm((void**)&h->staging, (size_t)M * 2 * h->G * 4) &&
@@ -738,7 +831,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
     return cleanup(fail(JM_ERR_HIP, "damping table upload"));
   h->k.damp = h->damp;
   if (hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * 4, h->stream) != hipSuccess ||
-      hipMemsetAsync(h->t_buf, 0, 8, h->stream) != hipSuccess ||
+      hipMemsetAsync(h->t_buf, 0, 8 * kMaxGroups, h->stream) != hipSuccess ||
       hipMemsetAsync(h->stats_d, 0, (size_t)M * 6 * 4, h->stream) != hipSuccess ||
       hipMemsetAsync(h->n_live_d, 0, M * 4, h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
@@ -765,9 +858,12 @@ int jm_set_tuning(jm_handle* h, const jm_tuning* t) {
     return fail(JM_ERR_ARG, "unsupported (threads, ppt) combination");
   if (t->sort_strip < 0 || t->sort_strip > h->ny)
     return fail(JM_ERR_ARG, "sort_strip must be in [0, grid rows]");
+  if (t->instance_groups < 1 || t->instance_groups > kMaxGroups)
+    return fail(JM_ERR_ARG, "instance_groups must be in [1, 4]");
   JM_TRY(jm_sync(h));
   const bool relayout = t->threads * t->ppt != h->tun.threads * h->tun.ppt ||
-                        t->xcd_affinity != h->tun.xcd_affinity;
+                        t->xcd_affinity != h->tun.xcd_affinity ||
+                        t->instance_groups != h->tun.instance_groups;
   if (t->sort_strip != h->tun.sort_strip) h->since_sort = t->sort_interval;  // re-sort next
   h->tun = *t;
   h->key_bits = sort_key_bits(h->nx, h->ny, t->sort_strip);
@@ -836,7 +932,9 @@ int jm_get_instance_param(jm_handle* h, int32_t param, float* values) {
 
 int jm_set_time(jm_handle* h, float t) {
   JM_TRY(check_handle(h));
-  HIP_TRY(hipMemcpyAsync(h->t_buf, &t, sizeof(float), hipMemcpyHostToDevice, h->stream));
+  float clocks[2 * kMaxGroups] = {0.f};
+  for (int g = 0; g < kMaxGroups; ++g) clocks[2 * g] = t;  // every instance group's clock
+  HIP_TRY(hipMemcpyAsync(h->t_buf, clocks, sizeof(clocks), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return JM_OK;
 }
@@ -1084,15 +1182,24 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
   const int b = h->acc_base;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-  jm::FusedArgs a = fused_args(h);
+  // Events on the main stream around kernel 0, kernels 1..n-1 and kernel n of every
+  // instance group (each segment forked to the groups' streams and joined back), so the
+  // middle interval is the wall time of n - 1 substeps of all instances.
+  auto segment = [&](int kk0, int kk1) {
+    int rc = fork_groups(h);
+    for (int g = 0; g < h->n_groups && rc == JM_OK; ++g)
+      rc = enqueue_group_kernels(h, g, kk0, kk1, n, b);
+    if (rc == JM_OK) rc = join_groups(h);
+    return rc;
+  };
   int rc = JM_OK;
   if (hipEventRecord(ev[0], h->stream) != hipSuccess) rc = JM_ERR_HIP;
-  enqueue_fused_kernel(h, a, 0, n, b);
-  if (hipEventRecord(ev[1], h->stream) != hipSuccess) rc = JM_ERR_HIP;
-  for (int kk = 1; kk < n; ++kk) enqueue_fused_kernel(h, a, kk, n, b);
-  if (hipEventRecord(ev[2], h->stream) != hipSuccess) rc = JM_ERR_HIP;
-  enqueue_fused_kernel(h, a, n, n, b);
-  if (hipEventRecord(ev[3], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  if (rc == JM_OK) rc = segment(0, 0);
+  if (rc == JM_OK && hipEventRecord(ev[1], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  if (rc == JM_OK) rc = segment(1, n - 1);
+  if (rc == JM_OK && hipEventRecord(ev[2], h->stream) != hipSuccess) rc = JM_ERR_HIP;
+  if (rc == JM_OK) rc = segment(n, n);
+  if (rc == JM_OK && hipEventRecord(ev[3], h->stream) != hipSuccess) rc = JM_ERR_HIP;
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
   float mid = 0.f, all = 0.f;
@@ -1100,7 +1207,7 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
   if (e == hipSuccess) e = hipEventElapsedTime(&all, ev[0], ev[3]);
   for (auto& ee : ev) (void)hipEventDestroy(ee);
   if (e != hipSuccess) return fail(JM_ERR_HIP, std::string("timed block: ") + hipGetErrorString(e));
-  if (rc != JM_OK) return fail(rc, "hipEventRecord failed");
+  if (rc != JM_OK) return rc == JM_ERR_HIP ? fail(rc, "timed block: enqueue failed") : rc;
   finish_block(h, n, b, sort);
   if (fused_ms) *fused_ms = mid / (float)(n - 1);
   if (block_ms) *block_ms = all;

@@ -28,10 +28,23 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == syms
 
 
+def _header_fields(struct):
+    """Field names of ``typedef struct <struct> {...}`` in include/jellymx.h."""
+    import re
+    text = open(os.path.join(ROOT, "include", "jellymx.h")).read()
+    body = text[text.index(f"typedef struct {struct} {{"):]
+    body = body[body.index("{") + 1:body.index("}")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return [m.group(1) for m in re.finditer(r"\b(\w+)\s*;", body)]
+
+
 def test_struct_layouts_match_header():
+    assert [f for f, _ in _lib.JmConfig._fields_] == _header_fields("jm_config")
+    assert [f for f, _ in _lib.JmTuning._fields_] == _header_fields("jm_tuning")
+    assert [f for f, _ in _lib.JmDiffInputs._fields_] == _header_fields("jm_diff_inputs")
     assert ctypes.sizeof(_lib.JmConfig) == 40
     assert _lib.JmConfig.domain_height.offset == 32
-    assert ctypes.sizeof(_lib.JmTuning) == 32
+    assert ctypes.sizeof(_lib.JmTuning) == 36
 
 
 def test_default_config_is_the_reference_default():

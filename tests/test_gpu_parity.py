@@ -260,13 +260,15 @@ def test_headless_matches_oracle(require_gpu):
 
 
 def test_variants_agree(require_gpu):
-    """Graphs on/off, lazy v/C on/off, XCD affinity and several launch shapes."""
+    """Graphs on/off, lazy v/C on/off, XCD affinity, 1-3 concurrent instance groups and
+    several launch shapes."""
     say("fused variants")
     cases = ragged_cases(12000)[:2] * 5          # 10 instances (affinity needs >= 8)
     base = None
     variants = [dict(), dict(use_graphs=0), dict(lazy_vc=0), dict(xcd_affinity=0),
                 dict(threads=256, ppt=4), dict(threads=512, ppt=8), dict(threads=128, ppt=16),
-                dict(sort_interval=7, block_substeps=5)]
+                dict(sort_interval=7, block_substeps=5), dict(instance_groups=1),
+                dict(instance_groups=3, use_graphs=0), dict(instance_groups=4)]
     for var in variants:
         say(f"variant {var}")
         eng, orc = make_pair(cases, **var)
