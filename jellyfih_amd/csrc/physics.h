@@ -157,7 +157,7 @@ __device__ __forceinline__ P2GOut p2g_update(int mat, const Mat2& F0, const Mat2
     const float fro2 = F.a00 * F.a00 + F.a01 * F.a01 + F.a10 * F.a10 + F.a11 * F.a11;
     if (det > 0.0f && det * det > 1e-12f * fro2) {
       const float J = det;
-      const float r = sqrtf(J);
+      const float r = __builtin_amdgcn_sqrtf(J);  // v_sqrt_f32 (1 ulp; J is near 1 here)
       const float dg = (la * J) * (J - 1.0f);
       o.q0 = o.q1 = 1.0f;
       o.jp_changes = false;
@@ -235,11 +235,16 @@ __host__ __device__ inline float damping_factor(int i, int n, int d) {
 
 // Grid update of one node (mpm_sim.py:274-309): normalise by mass, boundary damping band
 // of n/20 cells on all four sides (factors from the table in k.damp), then the wall
-// clamps. Nodes with no mass keep their (zero) momentum, as in the reference.
+// clamps. Nodes with no mass keep their (zero) momentum, as in the reference. The
+// normalisation multiplies by the hardware reciprocal (v_rcp_f32, 1 ulp) instead of two
+// IEEE divisions of ~10 instructions each: the node sums it divides already differ from a
+// sequential fp32 sum in their last bits, and the reference's own Taichi build divides
+// with fast math.
 __device__ __forceinline__ float2 grid_node(float px, float py, float m, int i, int j,
                                             const PhysConsts& k) {
   if (!(m > 0.0f)) return make_float2(px, py);
-  float g0 = px / m, g1 = py / m;
+  const float rm = __builtin_amdgcn_rcpf(m);
+  float g0 = px * rm, g1 = py * rm;
   const float damp = k.damp[i] * k.damp[k.nx + j];
   g0 = g0 * damp;
   g1 = g1 * damp;
