@@ -12,11 +12,11 @@
 // stencil nodes when the workgroup's particles are of one instance and close together
 // (particles packed instance by instance in spatial order, as tanks are filled), then add
 // the tile to the grid with one global atomic per touched node and plane; otherwise they
-// use global atomics directly. Adjacent lanes of one DPP row whose particles share a base
-// cell add their contributions pairwise first, so only one lane of a pair issues atomics.
-// The fp32 P2G tile accumulates in 64-bit fixed point (exact); the fp64 P2G tile and the
-// adjoint tile (whose values have no fixed range) in fp64 (gfx950's fp64 LDS atomic add is
-// much faster than its fp32 one). Every
+// use global atomics directly. Lanes of one DPP row whose particles share a base cell add
+// their contributions first (groups of 4 in the fp32 P2G, pairs elsewhere), so only one
+// lane per group issues atomics. The fp32 P2G tile accumulates in 64-bit fixed point
+// (exact); the fp64 P2G tile and the adjoint tile (whose values have no fixed range) in
+// fp64 (gfx950's fp64 LDS atomic add is much faster than its fp32 one). Every
 // grid a kernel accumulates into starts zeroed: grids rotate through three buffers and
 // each kernel clears the one the kernel after next accumulates into (grid-stride), or a
 // fill kernel clears them at the start of a block.
@@ -25,6 +25,7 @@
 #include <climits>
 
 #include "adjoint.h"
+#include "lanes.h"
 
 namespace jmd {
 
@@ -189,36 +190,104 @@ __device__ __forceinline__ double next_lane(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// 64-bit fixed point for the fp32 kernels' P2G tile: c * 2^50 rounded to the nearest
-// integer, as two's-complement int64 bits, for |c| < 1 (an error of at most 2^-51, finer
-// than fp32 rounding of any contribution above 2^-27). c * 2^50 + 1.5 * 2^52 in double
-// (one FMA, exact but for that rounding) lies in [2^52, 2^53), where the mantissa holds
-// c * 2^50 + 2^51: the low word is the integer's low word, the high word minus the
-// exponent and the 2^51 bit (0x43380000) its high word. A node of a workgroup's tile
-// receives at most 256 contributions (one per particle), so its sum stays below 2^58
-// units: no overflow. Larger contributions (and NaN) go to the global grid. The fp64
-// kernels accumulate in fp64 instead (fixed point would be coarser than fp64 rounding).
-__device__ __forceinline__ unsigned long long to_fixed(float c) {
-  const unsigned long long b =
-      __double_as_longlong(__fma_rn((double)c, 1125899906842624.0, 6755399441055744.0));
-  const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
-  return ((unsigned long long)hi << 32) | (unsigned)b;
-}
-__device__ __forceinline__ float from_fixed(unsigned long long v) {
-  return float((double)(long long)v * (1.0 / 1125899906842624.0));  // 2^-50
-}
 
-// Adds a 3-plane LDS tile of fixed-point sums into the instance's grid planes.
-__device__ __forceinline__ void tile_flush_fixed(const unsigned long long* acc, const Tile& t,
-                                                 float* g, int nx, int G) {
-  const int n = t.tw * t.th;
-  for (int li = threadIdx.x; li < n; li += kDiffBlock) {
-    const int jj = li / t.tw;
-    const int node = (t.oy + jj) * nx + t.ox + (li - jj * t.tw);
+// fp32 P2G of one particle per lane into the workgroup's LDS tile (3 planes of kTileNodes
+// 64-bit fixed-point sums, jm::fixed_bits) and its flush, called by every thread of the
+// workgroup: the same contributions as p2g_particle, w (m v + A (x_node - x)) and w m,
+// evaluated in the separable form wy_j P_i + ey_j Q_i (P_i = wx_i (m v + A_0 dx_i),
+// Q_i = wx_i A_1, ey_j = wy_j dy_j), then summed over same-cell groups of 4 lanes
+// (jm::lane_groups / presum3) so that one lane per group issues the atomics. A particle
+// whose contributions may reach 1 in magnitude (or are NaN), or any particle when the tile
+// is not used, scatters to the global grid with fp32 atomics and then carries zeros; lanes
+// without a particle carry zeros computed from finite stand-ins, so every lane's values are
+// finite in the pre-sums. Updates p.F for in-range particles.
+__device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active, int mt, float d0,
+                                                  float d1, float beta, float mu, float la,
+                                                  float pdens, const Consts<float>& k, int b0,
+                                                  int b1, float fx0, float fx1, const Tile& tl,
+                                                  double* lds, float* gi, float* gtile, int G) {
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds);
+  const int n = tl.tw * tl.th;
+  if (tl.use) {
+    for (int i = threadIdx.x; i < n; i += kDiffBlock) {
+      acc[i] = 0ull;
+      acc[kTileNodes + i] = 0ull;
+      acc[2 * kTileNodes + i] = 0ull;
+    }
+    __syncthreads();
+  }
+  float mass = 0.f;
+  Mat<float> A = {0.f, 0.f, 0.f, 0.f};
+  if (active) {
+    Mat<float> Fn;
+    p2g_forward(mt, p.F, p.C, d0, d1, beta, mu, la, pdens, k, Fn, A, mass);
+    p.F = Fn;
+  }
+  const float f0 = active ? fx0 : 0.5f, f1 = active ? fx1 : 0.5f;
+  const float mv0 = mass * (active ? p.v0 : 0.f), mv1 = mass * (active ? p.v1 : 0.f);
+  float wx[3], wy[3], dw[3], ey[3], P0[3], Q0[3], P1[3], Q1[3], Mi[3];
+  weights(f0, wx, dw);
+  weights(f1, wy, dw);
+  const float bnd = 1.5f * k.dx;
+  const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(A.a) + fabsf(A.b)),
+                                fabsf(mv1) + bnd * (fabsf(A.c) + fabsf(A.d))),
+                          mass);
+  const bool lds_ok = active && tl.use && lim < 1.0f;
+  const float sc = lds_ok ? jm::kFixPre : (active ? 1.0f : 0.0f);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float v = from_fixed(acc[c * n + li]);
-      if (v != 0.0f) unsafeAtomicAdd(g + (size_t)c * G + node, v);
+  for (int ii = 0; ii < 3; ++ii) {
+    const float ai = wx[ii] * sc;
+    const float bi = ai * ((float(ii) - f0) * k.dx);
+    P0[ii] = ai * mv0 + bi * A.a;
+    Q0[ii] = ai * A.b;
+    P1[ii] = ai * mv1 + bi * A.c;
+    Q1[ii] = ai * A.d;
+    Mi[ii] = ai * mass;
+  }
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * ((float(jj) - f1) * k.dx);
+  if (active && !lds_ok) {
+    for (int jj = 0; jj < 3; ++jj)
+      for (int ii = 0; ii < 3; ++ii) {
+        const int node = (b1 + jj) * k.nx + (b0 + ii);
+        unsafeAtomicAdd(gi + node, wy[jj] * P0[ii] + ey[jj] * Q0[ii]);
+        unsafeAtomicAdd(gi + G + node, wy[jj] * P1[ii] + ey[jj] * Q1[ii]);
+        unsafeAtomicAdd(gi + 2 * (size_t)G + node, wy[jj] * Mi[ii]);
+      }
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      P0[ii] = Q0[ii] = P1[ii] = Q1[ii] = Mi[ii] = 0.f;
+      wy[ii] = ey[ii] = 0.f;
+    }
+  }
+  const jm::LaneGroups lg =
+      jm::lane_groups(lds_ok ? b1 * k.nx + b0 + 1 : -(1 + (int)(threadIdx.x & 63)), lds_ok);
+  const int q0 = (b1 - tl.oy) * tl.tw + (b0 - tl.ox);
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj) {
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      float c0 = wy[jj] * P0[ii] + ey[jj] * Q0[ii], c1 = wy[jj] * P1[ii] + ey[jj] * Q1[ii],
+            c2 = wy[jj] * Mi[ii];
+      jm::presum3(c0, c1, c2, lg.f1, lg.f2);
+      if (lg.emit) {
+        const int q = q0 + jj * tl.tw + ii;
+        atomicAdd(&acc[q], jm::fixed_bits(c0));
+        atomicAdd(&acc[kTileNodes + q], jm::fixed_bits(c1));
+        atomicAdd(&acc[2 * kTileNodes + q], jm::fixed_bits(c2));
+      }
+    }
+  }
+  if (tl.use) {
+    __syncthreads();
+    for (int li = threadIdx.x; li < n; li += kDiffBlock) {
+      const int jj = li / tl.tw;
+      const int node = (tl.oy + jj) * k.nx + tl.ox + (li - jj * tl.tw);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float v = jm::from_fixed_bits(acc[c * kTileNodes + li]);
+        if (v != 0.0f) unsafeAtomicAdd(gtile + (size_t)c * G + node, v);
+      }
     }
   }
 }
@@ -295,7 +364,7 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
   }
   if constexpr (DO_P2G) {
     int b0 = 0, b1 = 0;
-    T fx0, fx1;
+    T fx0 = T(0.5), fx1 = T(0.5);
     const bool active = valid && mt >= 0 && base_cell(s.x0, s.x1, k, b0, b1, fx0, fx1);
     const Tile tl = block_tile(active, m, b0, b1);
     Act<T> act = {T(0), T(0), T(0)};
@@ -310,56 +379,42 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
     }
     const int n = tl.tw * tl.th;
     T* gi = grid_cur + (size_t)m * 3 * G;
-    const GlobalEmit<T> direct{gi, k.nx, G};
-    // fp32: 64-bit fixed point in LDS (exact, order-independent sums), a contribution
-    // outside the fixed-point range (or NaN) going to the global grid directly; fp64: fp64
-    constexpr bool kFixed = sizeof(T) == 4;
-    unsigned long long* accf = reinterpret_cast<unsigned long long*>(lds);
-    const LanePairs pr = lane_pairs(active && tl.use ? b1 * k.nx + b0 + 1 : -(1 + (int)threadIdx.x));
-    if (tl.use) {
-      for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) {
-        if (kFixed)
-          accf[i] = 0ull;
-        else
-          lds[i] = 0.0;
+    if constexpr (sizeof(T) == 4) {
+      p2g_scatter_fixed(s, active, mt, d0, d1, act.a * amp * as, mu, la, pdens, k, b0, b1,
+                        fx0, fx1, tl, lds, gi, grid_cur + (size_t)tl.inst * 3 * G, G);
+    } else {
+      // fp64: an fp64 LDS tile (fixed point would be coarser than fp64 rounding), lane pairs
+      const GlobalEmit<T> direct{gi, k.nx, G};
+      const LanePairs pr =
+          lane_pairs(active && tl.use ? b1 * k.nx + b0 + 1 : -(1 + (int)threadIdx.x));
+      if (tl.use) {
+        for (int i = threadIdx.x; i < 3 * n; i += kDiffBlock) lds[i] = 0.0;
+        __syncthreads();
       }
-      __syncthreads();
-    }
-    if (active)
-      p2g_particle(s, mt, d0, d1, act.a * amp * as, mu, la, pdens, k,
-                   [&](int i, int j, T c0, T c1, T c2) {
-                     if (tl.use) {
-                       const T n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
-                       if (pr.same_next) {
-                         c0 += n0;
-                         c1 += n1;
-                         c2 += n2;
-                       }
-                       if (pr.head) {
-                         const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-                         const T cs[3] = {c0, c1, c2};
-#pragma unroll
-                         for (int c = 0; c < 3; ++c) {
-                           if constexpr (kFixed) {
-                             if (fabsf(cs[c]) < 1.0f)
-                               atomicAdd(&accf[c * n + li], to_fixed(cs[c]));
-                             else
-                               unsafeAtomicAdd(gi + (size_t)c * G + j * k.nx + i, cs[c]);
-                           } else {
-                             atomicAdd(&lds[c * n + li], double(cs[c]));
-                           }
+      if (active)
+        p2g_particle(s, mt, d0, d1, act.a * amp * as, mu, la, pdens, k,
+                     [&](int i, int j, T c0, T c1, T c2) {
+                       if (tl.use) {
+                         const T n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
+                         if (pr.same_next) {
+                           c0 += n0;
+                           c1 += n1;
+                           c2 += n2;
                          }
+                         if (pr.head) {
+                           const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
+                           atomicAdd(&lds[li], double(c0));
+                           atomicAdd(&lds[n + li], double(c1));
+                           atomicAdd(&lds[2 * n + li], double(c2));
+                         }
+                       } else {
+                         direct(i, j, c0, c1, c2);
                        }
-                     } else {
-                       direct(i, j, c0, c1, c2);
-                     }
-                   });
-    if (tl.use) {
-      __syncthreads();
-      if constexpr (kFixed)
-        tile_flush_fixed(accf, tl, grid_cur + (size_t)tl.inst * 3 * G, k.nx, G);
-      else
+                     });
+      if (tl.use) {
+        __syncthreads();
         tile_flush(lds, tl, grid_cur + (size_t)tl.inst * 3 * G, k.nx, G);
+      }
     }
     if (valid) {
       T* fo = fnew_out + 4 * (size_t)p;

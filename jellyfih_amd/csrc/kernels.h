@@ -12,6 +12,7 @@
 
 #include <limits.h>
 
+#include "lanes.h"
 #include "physics.h"
 
 namespace jm {
@@ -245,28 +246,11 @@ __device__ __forceinline__ void block_bbox(int& mn0, int& mx0, int& mn1, int& mx
 
 constexpr int kBoxEmpty = 1 << 29;
 
-// P2G accumulates in LDS as 64-bit fixed point with 50 fractional bits: integer LDS
-// atomics run at full rate on gfx950 where fp32 LDS atomics do not, and integer sums are
-// exact and independent of the order of the contributions. A contribution converts with
-// an error of at most 2^-51 (c * 2^50 rounded to the nearest integer). A window gathers at most 4096 particles,
-// so with |c| < 1 every node sum is below 2^12 = 2^62 / 2^50 and fits in int64 (partial
-// sums may wrap; two's-complement addition still yields the exact final sum).
-// Particles with a contribution outside that range (or NaN) scatter straight to the global
-// fp32 grid instead, so non-finite values propagate as they do in fp32.
-constexpr double kFixInv = 1.0 / 1125899906842624.0;   // 2^-50
-constexpr float kFixPre = 1125899906842624.0f;          // 2^50
+// P2G accumulates in LDS as 64-bit fixed point (fixed_bits, lanes.h). A window gathers at
+// most 4096 particles, so with |c| < 1 every node sum fits int64. Particles with a
+// contribution outside that range (or NaN) scatter straight to the global fp32 grid
+// instead, so non-finite values propagate as they do in fp32.
 constexpr float kFixMax = 1.0f;
-
-// y = c * 2^50 (|y| < 2^51; the caller pre-scales its per-particle factors by 2^50, which
-// is exact) rounded to the nearest integer, as two's-complement int64 bits, in three VALU
-// operations: y + 1.5 * 2^52 in double is exact up to that rounding and lands in
-// [2^52, 2^53), where the 52 mantissa bits hold y + 2^51; the low word is then y's low word
-// and the high word, minus the exponent and the 2^51 bit (0x43380000), is y's high word.
-__device__ __forceinline__ unsigned long long fixed_bits(float y) {
-  const unsigned long long b = __double_as_longlong((double)y + 6755399441055744.0);
-  const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
-  return ((unsigned long long)hi << 32) | (unsigned)b;
-}
 
 __device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float y) {
   atomicAdd(slot, fixed_bits(y));
@@ -277,26 +261,6 @@ __device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float
 // error below 2^-22) is off by less than that.
 __device__ __forceinline__ int window_row(int q, float rcp_ww) {
   return (int)(((float)q + 0.5f) * rcp_ww);
-}
-
-// The two cross-lane pre-sum steps on three values: c += (c of lane + 1) * f1, then
-// c += (c of lane + 2) * f2, within DPP rows of 16 lanes (bound_ctrl: a lane past the end
-// of its row reads 0). One v_fmac_f32_dpp per value and step (the compiler keeps a
-// separate v_mov_b32_dpp before each FMA). Runs with every lane active. The leading
-// s_nop 1 gives the two wait states a DPP read of a VGPR needs after a VALU write of it
-// (the three values are written just before); inside the block each DPP source was
-// written at least two instructions earlier.
-__device__ __forceinline__ void presum3(float& c0, float& c1, float& c2, float f1, float f2) {
-  asm volatile(
-      "s_nop 1\n\t"
-      "v_fmac_f32_dpp %0, %0, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-      "v_fmac_f32_dpp %1, %1, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-      "v_fmac_f32_dpp %2, %2, %3 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-      "v_fmac_f32_dpp %0, %0, %4 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-      "v_fmac_f32_dpp %1, %1, %4 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-      "v_fmac_f32_dpp %2, %2, %4 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-      : "+v"(c0), "+v"(c1), "+v"(c2)
-      : "v"(f1), "v"(f2));
 }
 
 // Rare path: a particle whose contributions leave the fixed-point range scatters its nine
@@ -321,13 +285,6 @@ __device__ __forceinline__ void scatter_global(float* gw, int G, int nx, int b0,
     }
 }
 
-__device__ __forceinline__ float from_fixed(unsigned long long v) {
-#ifdef JM_LDS_F64
-  return (float)__longlong_as_double((long long)v);
-#else
-  return (float)((double)(long long)v * kFixInv);
-#endif
-}
 
 // G2P of one in-range particle from a 3x3 block of grid velocities (row stride `ld`):
 // v = sum w g, C = 4 inv_dx sum w g (x) (node - x) (mpm_sim.py:317-327), evaluated in the
@@ -589,7 +546,6 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     __syncthreads();
     float* gw = a.acc_w + (size_t)m * 3 * G;
     const int lane = tid & 63;
-    const unsigned long long lanes_le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       const int li = p * BLOCK + tid;
@@ -656,28 +612,16 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
         }
       }
       // Cell-sorted particles of one base cell sit in adjacent lanes (water: 4 per cell).
-      // Runs of lanes with the same cell, within a DPP row of 16 lanes, add their 27
-      // contributions across lanes in two steps (row_shl:1, then row_shl:2), so the first
-      // lane of every group of 4 in a run holds the group's sums and alone issues the 27
-      // 64-bit LDS atomics: a quarter of the atomic traffic of one set per particle, and no
-      // two lanes of a water cell hitting one address, in a fixed order (deterministic).
-      // A neighbour's term enters as t * f with f = 1 when it has the same cell and 0
-      // otherwise (one FMA, exact either way: every lane's terms are finite, since
-      // out-of-range lanes carry zeros). After the first step, lane l + 2 holds lanes
-      // l + 2 and l + 3 when they share its cell; lane l takes it only when l + 1 and l + 2
-      // both share l's cell, i.e. l + 2 is in l's run (between sorts a cell can recur
-      // after a gap, A B A A, where the second A run emits by itself). Lanes outside the
+      // Same-cell runs add their 27 contributions across lanes in groups of 4
+      // (lane_groups, presum3), so only the first lane of each group issues the 27 64-bit
+      // LDS atomics: a quarter of the atomic traffic of one set per particle, no two lanes
+      // of a water cell hitting one address, in a fixed order (deterministic). A
+      // neighbour's term enters as t * f with f = 1 or 0 (one FMA, exact either way: every
+      // lane's terms are finite, since out-of-range lanes carry zeros). Lanes outside the
       // fixed-point LDS path carry unique keys and add nothing. Measured at 32 x 80K:
       // 44.3 us per substep against 45.7 us with pairs only.
       const int key = fixed_ok ? (b1 * nx + b0 + 1) : -(1 + lane);
-      const int k1 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x101, 0xF, 0xF, false);
-      const int k2 = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x102, 0xF, 0xF, false);
-      const int kp = __builtin_amdgcn_update_dpp(INT_MIN, key, 0x111, 0xF, 0xF, false);
-      const float f1 = k1 == key ? 1.0f : 0.0f;
-      const float f2 = (k1 == key && k2 == key) ? 1.0f : 0.0f;
-      const unsigned long long heads = __ballot(kp != key);
-      const int runstart = 63 - __clzll(heads & lanes_le);
-      const bool emit = fixed_ok && (((lane - runstart) & 3) == 0);
+      const LaneGroups lg = lane_groups(key, fixed_ok);
       const int q0 = (b1 - mn1) * ww + (b0 - mn0);
 #pragma unroll
       for (int jj = 0; jj < 3; ++jj) {
@@ -685,8 +629,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
         for (int ii = 0; ii < 3; ++ii) {
           float c[3] = {wy[jj] * P0[ii] + ey[jj] * Q0[ii], wy[jj] * P1[ii] + ey[jj] * Q1[ii],
                         wy[jj] * Mi[ii]};
-          presum3(c[0], c[1], c[2], f1, f2);
-          if (emit) {
+          presum3(c[0], c[1], c[2], lg.f1, lg.f2);
+          if (lg.emit) {
             const int q = q0 + jj * ww + ii;
             add_fixed_scaled(&lds64[q], c[0]);
             add_fixed_scaled(&lds64[WMAX + q], c[1]);
@@ -699,8 +643,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     for (int q = tid; q < nodes; q += BLOCK) {
       const int jj = window_row(q, rcp_ww), ii = q - jj * ww;
       const int node = (mn1 + jj) * nx + (mn0 + ii);
-      const float c0 = from_fixed(lds64[q]), c1 = from_fixed(lds64[WMAX + q]),
-                  cm = from_fixed(lds64[2 * WMAX + q]);
+      const float c0 = from_fixed_bits(lds64[q]), c1 = from_fixed_bits(lds64[WMAX + q]),
+                  cm = from_fixed_bits(lds64[2 * WMAX + q]);
       if (c0 != 0.f) unsafeAtomicAdd(gw + node, c0);
       if (c1 != 0.f) unsafeAtomicAdd(gw + G + node, c1);
       if (cm != 0.f) unsafeAtomicAdd(gw + 2 * G + node, cm);
