@@ -318,6 +318,9 @@ int build_chunks(jm_handle* h) {
     gc.cnt = (int)all.size() - gc.off;
   }
   h->total_chunks = (int)all.size();
+  // the old table may still be read by queued kernels of every group (joined into the
+  // main stream)
+  HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->chunks_d) (void)hipFree(h->chunks_d);
   h->chunks_d = nullptr;
   if (!all.empty()) {
