@@ -618,11 +618,11 @@ int run_fused(jm_handle* h, int64_t steps) {
       JM_TRY(fork_groups(h));
       for (int g = 0; g < h->n_groups; ++g) {
         if (h->grp[g].cnt == 0) continue;
-        if (ex[g])
-          HIP_TRY(hipGraphLaunch(ex[g], group_stream(h, g)));
-        else
-          JM_TRY(enqueue_group_kernels(h, g, 0, n, n, b));
+        if (ex[g]) HIP_TRY(hipGraphLaunch(ex[g], group_stream(h, g)));
       }
+      if (!h->tun.use_graphs)  // interleaved, as in jm_timed_block
+        for (int kk = 0; kk <= n; ++kk)
+          for (int g = 0; g < h->n_groups; ++g) JM_TRY(enqueue_group_kernels(h, g, kk, kk, n, b));
       JM_TRY(join_groups(h));
     }
     finish_block(h, n, b, sort);
@@ -1188,10 +1188,13 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
   // Events on the main stream around kernel 0, kernels 1..n-1 and kernel n of every
   // instance group (each segment forked to the groups' streams and joined back), so the
   // middle interval is the wall time of n - 1 substeps of all instances.
+  // The groups' launches are enqueued interleaved (kernel kk of every group, then kk + 1),
+  // so that no group waits for the host to enqueue another group's whole segment.
   auto segment = [&](int kk0, int kk1) {
     int rc = fork_groups(h);
-    for (int g = 0; g < h->n_groups && rc == JM_OK; ++g)
-      rc = enqueue_group_kernels(h, g, kk0, kk1, n, b);
+    for (int kk = kk0; kk <= kk1 && rc == JM_OK; ++kk)
+      for (int g = 0; g < h->n_groups && rc == JM_OK; ++g)
+        rc = enqueue_group_kernels(h, g, kk, kk, n, b);
     if (rc == JM_OK) rc = join_groups(h);
     return rc;
   };
