@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE — the oracle. Not part of the product path.
 
 A CPU restatement of the reference's batched 2D MLS-MPM simulation core
-(c-m-x-r/jellyfih ``mpm_sim.py``), used only as the checker by ``tests/``,
+(c-m-x-r/jellyfih ``mpm_sim.py``), used only as the reference results of ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg:
 
 * ``liboracle.so`` (``mpm_oracle.cpp``): the substep in fp32 and fp64, following

@@ -3,7 +3,8 @@
 // CPU restatement of the reference's batched 2D MLS-MPM simulation core, templated on
 // float (fp32, the reference's arithmetic type) and double (fp64, used to attribute fp32
 // error). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
-// this library, and only as the checker / CPU baseline, never as the thing measured.
+// this library, and only as the reference results / CPU baseline, never as the thing
+// measured.
 //
 // Every function follows the reference line by line (citations are file:line in
 // c-m-x-r/jellyfih):

@@ -45,7 +45,7 @@ def test_truncation_when_too_many_particles():
     assert np.all(mat >= 0)
 
 
-def test_self_intersection_checker():
+def test_self_intersection_test():
     square = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
     bowtie = np.array([[0, 0], [1, 1], [1, 0], [0, 1]], float)
     assert not phenotype.polygon_self_intersects(square)
