@@ -19,7 +19,9 @@ Also reported on the same line:
   particles, 2000 substeps, checkpoint every 100), in fp32 and fp64: forward through the
   autograd blocks, forward + backward of the payload CoM y with recomputation from the
   checkpoints, the same keeping every block's stored trajectory (no recomputation), and the
-  fused (non-differentiable, fp32) forward of the same episode for comparison.
+  fused (non-differentiable, fp32) forward of the same episode for comparison;
+* render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
+  map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -223,6 +225,59 @@ def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
             "fused_rate": live * steps / fused}
 
 
+WEB_PALETTE = [(0, 0.933, 0.949, 0.957), (1, 0.678, 0.882, 0.875), (3, 0.976, 0.757, 0.765),
+               (2, 0.961, 0.824, 0.576)]     # mpm_sim.py:60-65
+
+
+def render_frames(eng, live: int, reps: int = 20) -> dict:
+    """One 1024x1024 video frame of the whole batch per pipeline, as evolve.py's view mode
+    and the helpers draw it (mpm_sim.py:343-712): the abyss glow splat + tone map, the web
+    palette's four flat passes over white, and the vorticity grid + RdBu composite; each
+    timed with HIP events on the engine's stream over `reps` frames, plus the read-back."""
+    import math
+    import torch
+    M = eng.M
+    grid_side = int(math.ceil(math.sqrt(M)))
+    res_sub = 1024 // grid_side
+    radius = max(1.0, res_sub / 128 * 0.6)                  # evolve.py:380
+    hue = np.linspace(0.0, 0.8, M).astype(np.float32)
+    mhue = ((hue + 0.15) % 1.0).astype(np.float32)
+    stream = torch.cuda.ExternalStream(eng.stream)
+
+    def abyss():
+        eng.render_clear(1024, 0.0)
+        eng.render_abyss(res_sub, grid_side, radius, hue, mhue, 0)
+        eng.render_tone_map()
+
+    def web():
+        eng.render_clear(1024, 0.98)
+        for mat_id, r, g, b in WEB_PALETTE:
+            eng.render_flat(res_sub, grid_side, radius, mat_id, r, g, b)
+
+    def vorticity():
+        eng.render_vorticity_grid(0.002)
+        eng.render_vorticity(1, res_sub, grid_side, 0.6)
+
+    out = {"frame": [1024, 1024], "grid_side": grid_side, "res_sub": res_sub,
+           "radius": radius, "live_particles": live}
+    for name, fn in (("abyss_ms", abyss), ("web_ms", web), ("vorticity_rdbu_ms", vorticity)):
+        fn()
+        eng.sync()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        eng.sync()
+        out[name] = e0.elapsed_time(e1) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.read_frame()
+    out["read_frame_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +294,8 @@ def main() -> int:
                     help="run only the differentiable legs (no throughput line fields)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched tall-tank forward + backward")
+    ap.add_argument("--only-render", action="store_true",
+                    help="run only the renderer leg (after the warm-up blocks)")
     ap.add_argument("--batched-steps", type=int, default=4000)
     ap.add_argument("--batched-checkpoint", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -287,6 +344,10 @@ def main() -> int:
     eng.substep(args.warmup * BLOCK)
     eng.sync()
 
+    if args.only_render:
+        print(json.dumps({"render": render_frames(eng, live)}), flush=True)
+        return 0
+
     stream = torch.cuda.ExternalStream(eng.stream)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -321,6 +382,10 @@ def main() -> int:
                           f"instance-group launches", "kernel_ms": fused_ms,
                 "algorithmic_bytes_per_launch": ALGO_BYTES * live}
     log(f"fused kernel {fused_ms * 1e3:.1f} us -> {achieved / 1e9:.0f} GB/s algorithmic")
+
+    log("renderer (one video frame per pipeline)")
+    rend = render_frames(eng, live)
+    log(f"render {rend}")
 
     plain_value = None
     if not args.no_plain:
@@ -379,6 +444,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "diff": diff,
+        "render": rend,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -168,6 +168,48 @@ int jm_sort_now(jm_handle* h);
  * jm_substep(h, n). No reference counterpart (the reference has no timing hooks). */
 int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms);
 
+/* ---- Renderer (mpm_sim.py:343-712) ----------------------------------------------------
+ * The frame buffer is res x res x 3 fp32 on the device, [py][px][c] like the reference's
+ * frame_buffer field (video_res = 1024 there); instance m is drawn in tile
+ * (m / grid_side, m % grid_side) of res_sub pixels. All calls run on the handle's stream
+ * after the pending loads are committed. */
+
+/* clear_frame_buffer (:361-364, value 0), clear_frame_buffer_white (:483-485, 0.98): fill the
+ * frame buffer with `value`; allocates (or reallocates) it at res x res. */
+int jm_render_clear(jm_handle* h, int32_t res, float value);
+
+/* render_frame_abyss (:385-477): additive splat of every live particle, coloured by
+ * material (water by water_mode = water_angle_color: 0 ghost, 1 plain, 2 rainbow; jelly
+ * from hue[m]; muscle from muscle_hue[m] and the activation at sim_time; payload orange).
+ * hue, muscle_hue: host arrays of n_instances floats (instance_hue, instance_muscle_hue). */
+int jm_render_abyss(jm_handle* h, int32_t res_sub, int32_t grid_side, float radius,
+                    const float* hue, const float* muscle_hue, int32_t water_mode);
+
+/* render_flat_pass (:487-517): particles of `material` set the pixels of their disc to
+ * (r, g, b). */
+int jm_render_flat(jm_handle* h, int32_t res_sub, int32_t grid_side, float radius,
+                   int32_t material, float r, float g, float b);
+
+/* tone_map_and_encode (:366-383), in place. */
+int jm_render_tone_map(jm_handle* h);
+
+/* compute_vorticity_grid (:560-578): vort (n_instances, nx, ny) = tanh(curl(grid_v) *
+ * scale) at interior nodes with mass, from the grid the last substep's G2P read. */
+int jm_render_vorticity_grid(jm_handle* h, float scale);
+
+/* The vorticity passes: mode 0 render_vorticity_overlay (:520-558, param = vort_scale,
+ * curl computed directly), 1 render_vorticity_rdbu (:581-624, param = alpha_max),
+ * 2 render_vorticity_hueshift (:626-668, param = strength), 3 render_vorticity_white
+ * (:670-712, param = alpha_max; fills the buffer with 1.0 first). Modes 1-3 read the grid of
+ * the last jm_render_vorticity_grid. */
+int jm_render_vorticity(jm_handle* h, int32_t mode, int32_t res_sub, int32_t grid_side,
+                        float param);
+
+/* Copy the frame buffer (res*res*3 floats) or the vorticity grid (n_instances*nx*ny floats,
+ * [m][i][j] like vort_grid) to host memory; n is the number of floats. */
+int jm_read_frame(jm_handle* h, float* out, int64_t n);
+int jm_read_vorticity(jm_handle* h, float* out, int64_t n);
+
 /* ---- Tank building (host code, no GPU) -------------------------------------------------
  * Replaces the k-d tree query of make_jelly.py:359-366 (fill_tank, step 3): keep[i] = 1 when
  * every body point lies farther than `margin` from candidate i (Euclidean distance in

@@ -36,6 +36,8 @@ EXPORTS = (
     "jm_stream", "jm_last_error", "jm_water_mask", "jm_polygon_self_intersects",
     "jm_diff_create", "jm_diff_destroy",
     "jm_fwd_block", "jm_bwd_block",
+    "jm_render_clear", "jm_render_abyss", "jm_render_flat", "jm_render_tone_map",
+    "jm_render_vorticity_grid", "jm_render_vorticity", "jm_read_frame", "jm_read_vorticity",
 )
 
 
@@ -131,6 +133,17 @@ def load() -> ctypes.CDLL:
         "jm_bwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                          ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.c_void_p],
+        "jm_render_clear": [H, ctypes.c_int32, ctypes.c_float],
+        "jm_render_abyss": [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, f32p, f32p,
+                            ctypes.c_int32],
+        "jm_render_flat": [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
+                           ctypes.c_float, ctypes.c_float, ctypes.c_float],
+        "jm_render_tone_map": [H],
+        "jm_render_vorticity_grid": [H, ctypes.c_float],
+        "jm_render_vorticity": [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                ctypes.c_float],
+        "jm_read_frame": [H, f32p, ctypes.c_int64],
+        "jm_read_vorticity": [H, f32p, ctypes.c_int64],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -20,6 +20,7 @@
 #include "../../include/jellymx.h"
 #include "kernels.h"
 #include "diff_kernels.h"
+#include "render.h"
 
 namespace {
 
@@ -83,6 +84,11 @@ struct jm_handle {
   float* stats_d = nullptr;   // M * 6
   float* staging = nullptr;   // M * 2 * G floats (grid velocity readout)
   float* damp = nullptr;      // nx + ny floats: boundary damping factors (k.damp)
+  float* frame = nullptr;     // renderer: frame_res^2 x 3 floats (allocated on first use)
+  int frame_res = 0;
+  float* vort = nullptr;      // renderer: M * G floats, node (i, j) at j * nx + i
+  float* rhue = nullptr;      // renderer: 2 * M floats, instance_hue then instance_muscle_hue
+  std::vector<float> rhue_host;  // what rhue holds
 
   // host mirrors
   std::vector<float> h_param[5];  // amp, contraction, freq, pdens, E_jelly
@@ -123,12 +129,14 @@ struct jm_handle {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {st.base, tmp.base, sortbuf, hist, acc, t_buf, params, n_live_d,
-                    chunks_d, nchunks_d, stats_d, staging, damp};
+                    chunks_d, nchunks_d, stats_d, staging, damp, frame, vort, rhue};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     st.base = tmp.base = nullptr;
     sortbuf = hist = nullptr;
-    acc = t_buf = params = stats_d = staging = damp = nullptr;
+    acc = t_buf = params = stats_d = staging = damp = frame = vort = rhue = nullptr;
+    frame_res = 0;
+    rhue_host.clear();
     n_live_d = nullptr;
     chunks_d = nullptr;
     nchunks_d = nullptr;
@@ -1227,6 +1235,196 @@ int jm_sync(jm_handle* h) {
 }
 
 void* jm_stream(jm_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"
+
+// -----------------------------------------------------------------------------------------
+// Renderer (mpm_sim.py:343-712)
+
+namespace {
+
+jm::RenderArgs render_args(jm_handle* h, int res_sub, int grid_side, float radius) {
+  jm::RenderArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const jm::InstParams ip = inst_params(h);
+  a.s = h->st.s;
+  a.n_live = h->n_live_d;
+  a.cap = h->cap;
+  a.M = h->M;
+  a.freq = ip.freq;
+  a.contraction = ip.contraction;
+  a.hue = h->rhue;
+  a.muscle_hue = h->rhue ? h->rhue + h->M : nullptr;
+  a.t_buf = h->t_buf;
+  a.fb = h->frame;
+  a.res = h->frame_res;
+  a.res_sub = res_sub;
+  a.grid_side = grid_side;
+  a.radius = radius;
+  a.domain_height = (float)h->cfg.domain_height;
+  a.k = h->k;
+  return a;
+}
+
+int render_ready(jm_handle* h, int res_sub, int grid_side) {
+  JM_TRY(check_handle(h));
+  if (!h->frame) return fail(JM_ERR_STATE, "no frame buffer: call jm_render_clear first");
+  if (res_sub < 1 || res_sub > 16384 || grid_side < 1 || grid_side > 16384)
+    return fail(JM_ERR_ARG, "res_sub and grid_side must be in [1, 16384]");
+  return commit(h);
+}
+
+// The grid the last G2P gathered and whether its planes already hold velocities (plain path).
+const float* render_grid(jm_handle* h, int* vel_planes) {
+  *vel_planes = h->grid_is_velocity ? 1 : 0;
+  return h->last_read >= 0 ? grid_buf(h, h->last_read) : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jm_render_clear(jm_handle* h, int32_t res, float value) {
+  JM_TRY(check_handle(h));
+  if (res < 1 || res > 16384) return fail(JM_ERR_ARG, "res must be in [1, 16384]");
+  if (res != h->frame_res) {
+    JM_TRY(jm_sync(h));
+    if (h->frame) (void)hipFree(h->frame);
+    h->frame = nullptr;
+    h->frame_res = 0;
+    if (hipMalloc(&h->frame, (size_t)res * res * 3 * sizeof(float)) != hipSuccess)
+      return fail(JM_ERR_NOMEM, "hipMalloc(frame buffer)");
+    h->frame_res = res;
+  }
+  const int64_t n = (int64_t)res * res * 3;
+  hipLaunchKernelGGL(jm::k_fb_fill, dim3(blocks_for(n, 256 * 8)), dim3(256), 0, h->stream,
+                     h->frame, n, value);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_render_abyss(jm_handle* h, int32_t res_sub, int32_t grid_side, float radius,
+                    const float* hue, const float* muscle_hue, int32_t water_mode) {
+  JM_TRY(render_ready(h, res_sub, grid_side));
+  if (!(radius >= 0.0f && radius <= 16384.0f))
+    return fail(JM_ERR_ARG, "radius must be in [0, 16384]");
+  if (!hue || !muscle_hue) return fail(JM_ERR_ARG, "null hue arrays");
+  if (!h->rhue && hipMalloc(&h->rhue, (size_t)2 * h->M * sizeof(float)) != hipSuccess)
+    return fail(JM_ERR_NOMEM, "hipMalloc(render hues)");
+  // the hues live on the device; upload only when they change (then wait for the copy, so
+  // that the host copy can be rewritten by the next change)
+  const size_t M = h->M;
+  if (h->rhue_host.size() != 2 * M ||
+      std::memcmp(h->rhue_host.data(), hue, M * sizeof(float)) != 0 ||
+      std::memcmp(h->rhue_host.data() + M, muscle_hue, M * sizeof(float)) != 0) {
+    h->rhue_host.assign(hue, hue + M);
+    h->rhue_host.insert(h->rhue_host.end(), muscle_hue, muscle_hue + M);
+    HIP_TRY(hipMemcpyAsync(h->rhue, h->rhue_host.data(), 2 * M * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  jm::RenderArgs a = render_args(h, res_sub, grid_side, radius);
+  a.water_mode = water_mode;
+  hipLaunchKernelGGL(jm::k_render_abyss, dim3(blocks_for(h->cap, 256), h->M), dim3(256), 0,
+                     h->stream, a);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_render_flat(jm_handle* h, int32_t res_sub, int32_t grid_side, float radius,
+                   int32_t material, float r, float g, float b) {
+  JM_TRY(render_ready(h, res_sub, grid_side));
+  if (!(radius >= 0.0f && radius <= 16384.0f))
+    return fail(JM_ERR_ARG, "radius must be in [0, 16384]");
+  jm::RenderArgs a = render_args(h, res_sub, grid_side, radius);
+  a.mat_sel = material;
+  a.flat = make_float3(r, g, b);
+  hipLaunchKernelGGL(jm::k_render_flat, dim3(blocks_for(h->cap, 256), h->M), dim3(256), 0,
+                     h->stream, a);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_render_tone_map(jm_handle* h) {
+  JM_TRY(check_handle(h));
+  if (!h->frame) return fail(JM_ERR_STATE, "no frame buffer: call jm_render_clear first");
+  const int64_t n = (int64_t)h->frame_res * h->frame_res * 3;
+  hipLaunchKernelGGL(jm::k_tone_map, dim3(blocks_for(n, 256 * 8)), dim3(256), 0, h->stream,
+                     h->frame, n);
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_render_vorticity_grid(jm_handle* h, float scale) {
+  JM_TRY(check_handle(h));
+  JM_TRY(commit(h));
+  const int M = h->M, G = h->G;
+  if (!h->vort && hipMalloc(&h->vort, (size_t)M * G * sizeof(float)) != hipSuccess)
+    return fail(JM_ERR_NOMEM, "hipMalloc(vorticity grid)");
+  int vel_planes = 0;
+  const float* g = render_grid(h, &vel_planes);
+  if (!g) {  // no substep yet: the reference's reset grid has no curl
+    hipLaunchKernelGGL(jm::k_fb_fill, dim3(blocks_for((int64_t)M * G, 256 * 8)), dim3(256), 0,
+                       h->stream, h->vort, (int64_t)M * G, 0.0f);
+  } else {
+    hipLaunchKernelGGL(jm::k_vorticity_grid, dim3(blocks_for(G, 256), M), dim3(256), 0,
+                       h->stream, g, h->vort, M, G, scale, vel_planes, h->k);
+  }
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_render_vorticity(jm_handle* h, int32_t mode, int32_t res_sub, int32_t grid_side,
+                        float param) {
+  JM_TRY(render_ready(h, res_sub, grid_side));
+  if (mode < 0 || mode > 3) return fail(JM_ERR_ARG, "mode must be 0..3");
+  const int M = h->M, G = h->G;
+  jm::RenderArgs a = render_args(h, res_sub, grid_side, 0.0f);
+  if (mode == 0) {
+    int vel_planes = 0;
+    const float* g = render_grid(h, &vel_planes);
+    if (g)
+      hipLaunchKernelGGL(jm::k_vorticity_overlay, dim3(blocks_for(G, 256), M), dim3(256), 0,
+                         h->stream, g, M, G, param, vel_planes, a);
+  } else {
+    if (!h->vort) return fail(JM_ERR_STATE, "no vorticity grid: call jm_render_vorticity_grid");
+    if (mode == jm::kVortWhite) {
+      const int64_t n = (int64_t)h->frame_res * h->frame_res * 3;
+      hipLaunchKernelGGL(jm::k_fb_fill, dim3(blocks_for(n, 256 * 8)), dim3(256), 0, h->stream,
+                         h->frame, n, 1.0f);
+    }
+    hipLaunchKernelGGL(jm::k_vorticity_pixels, dim3(blocks_for(h->frame_res, 256), h->frame_res),
+                       dim3(256), 0, h->stream, h->vort, M, G, mode, param, a);
+  }
+  HIP_TRY(hipGetLastError());
+  return JM_OK;
+}
+
+int jm_read_frame(jm_handle* h, float* out, int64_t n) {
+  JM_TRY(check_handle(h));
+  if (!out) return fail(JM_ERR_ARG, "null out");
+  if (!h->frame) return fail(JM_ERR_STATE, "no frame buffer: call jm_render_clear first");
+  if (n != (int64_t)h->frame_res * h->frame_res * 3) return fail(JM_ERR_ARG, "n != res*res*3");
+  HIP_TRY(hipMemcpyAsync(out, h->frame, n * 4, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int jm_read_vorticity(jm_handle* h, float* out, int64_t n) {
+  JM_TRY(check_handle(h));
+  if (!out) return fail(JM_ERR_ARG, "null out");
+  if (!h->vort) return fail(JM_ERR_STATE, "no vorticity grid: call jm_render_vorticity_grid");
+  const int M = h->M, nx = h->nx, ny = h->ny, G = h->G;
+  if (n != (int64_t)M * G) return fail(JM_ERR_ARG, "n != n_instances*nx*ny");
+  std::vector<float> tmp((size_t)M * G);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), h->vort, tmp.size() * 4, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int m = 0; m < M; ++m)
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        out[((size_t)m * nx + i) * ny + j] = tmp[(size_t)m * G + (size_t)j * nx + i];
+  return JM_OK;
+}
 
 }  // extern "C"
 

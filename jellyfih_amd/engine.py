@@ -160,6 +160,51 @@ class Engine:
         call("jm_timed_block", self._h, int(n), ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
+    # -- renderer (mpm_sim.py:343-712) -------------------------------------------------
+    def render_clear(self, res: int = 1024, value: float = 0.0) -> None:
+        """Fill the res x res x 3 frame buffer with ``value`` (clear_frame_buffer: 0,
+        clear_frame_buffer_white: 0.98), allocating it on first use."""
+        self._frame_res = int(res)
+        call("jm_render_clear", self._h, int(res), float(value))
+
+    def render_abyss(self, res_sub: int, grid_side: int, radius: float, hue, muscle_hue,
+                     water_mode: int = 0) -> None:
+        h = np.ascontiguousarray(hue, np.float32)
+        mh = np.ascontiguousarray(muscle_hue, np.float32)
+        if h.shape != (self.M,) or mh.shape != (self.M,):
+            raise ValueError(f"hue arrays must be ({self.M},)")
+        call("jm_render_abyss", self._h, int(res_sub), int(grid_side), float(radius),
+             _fptr(h), _fptr(mh), int(water_mode))
+
+    def render_flat(self, res_sub: int, grid_side: int, radius: float, material: int,
+                    r: float, g: float, b: float) -> None:
+        call("jm_render_flat", self._h, int(res_sub), int(grid_side), float(radius),
+             int(material), float(r), float(g), float(b))
+
+    def render_tone_map(self) -> None:
+        call("jm_render_tone_map", self._h)
+
+    def render_vorticity_grid(self, scale: float) -> None:
+        call("jm_render_vorticity_grid", self._h, float(scale))
+
+    def render_vorticity(self, mode: int, res_sub: int, grid_side: int, param: float) -> None:
+        """mode 0 overlay (param vort_scale), 1 RdBu, 2 hue shift, 3 white (param alpha)."""
+        call("jm_render_vorticity", self._h, int(mode), int(res_sub), int(grid_side),
+             float(param))
+
+    def read_frame(self) -> np.ndarray:
+        res = getattr(self, "_frame_res", 0)
+        if not res:
+            raise RuntimeError("no frame buffer: call render_clear first")
+        out = np.empty((res, res, 3), np.float32)
+        call("jm_read_frame", self._h, _fptr(out), out.size)
+        return out
+
+    def read_vorticity(self) -> np.ndarray:
+        out = np.empty((self.M, self.nx, self.ny), np.float32)
+        call("jm_read_vorticity", self._h, _fptr(out), out.size)
+        return out
+
     def sort_now(self) -> None:
         call("jm_sort_now", self._h)
 

@@ -14,8 +14,8 @@ those names and their meaning, backed by the HIP engine (``libjellymx.so``):
   and the fitness-buffer kernels behave as in the reference.
 
 The device engine is created on first use, so importing this module never touches a GPU.
-The renderer (``mpm_sim.py:343-712``) and the axisymmetric variant are not part of this
-engine; their names raise ``NotImplementedError``.
+The renderer (``mpm_sim.py:343-712``) draws into a device frame buffer read back with
+``frame_buffer.to_numpy()``. The axisymmetric variant is not part of this engine.
 """
 from __future__ import annotations
 
@@ -279,14 +279,95 @@ def sync() -> None:
     engine().sync()
 
 
-_RENDER_NAMES = {
-    "clear_frame_buffer", "clear_frame_buffer_white", "render_frame_abyss", "render_flat_pass",
-    "tone_map_and_encode", "compute_vorticity_grid", "render_vorticity_overlay",
-    "render_vorticity_rdbu", "render_vorticity_hueshift", "frame_buffer", "vort_grid",
-}
+# --- renderer (mpm_sim.py:343-712) -----------------------------------------------------------
+# The frame buffer (video_res x video_res x 3 fp32, indexed [py, px, c]) and the vorticity
+# grid live on the device; they are allocated on first use.
+_frame_ready = False
 
 
-def __getattr__(name):
-    if name in _RENDER_NAMES:
-        raise NotImplementedError(f"mpm_sim.{name}: the renderer is not part of this engine")
-    raise AttributeError(name)
+def _frame() -> "Engine":
+    global _frame_ready
+    e = engine()
+    if not _frame_ready:
+        e.render_clear(video_res, 0.0)
+        _frame_ready = True
+    return e
+
+
+def clear_frame_buffer() -> None:
+    """mpm_sim.py:361-364: black."""
+    global _frame_ready
+    engine().render_clear(video_res, 0.0)
+    _frame_ready = True
+
+
+def clear_frame_buffer_white() -> None:
+    """mpm_sim.py:483-486: 0.98 (#FAFAFA)."""
+    global _frame_ready
+    engine().render_clear(video_res, 0.98)
+    _frame_ready = True
+
+
+def render_frame_abyss(p_res_sub: int, p_grid_side: int, radius: float) -> None:
+    """Additive glow splat of every live particle (mpm_sim.py:385-477), coloured by material
+    with instance_hue, instance_muscle_hue and water_angle_color."""
+    _frame().render_abyss(p_res_sub, p_grid_side, radius, instance_hue.to_numpy(),
+                          instance_muscle_hue.to_numpy(), int(water_angle_color[None]))
+
+
+def render_flat_pass(p_res_sub: int, p_grid_side: int, radius: float, target_mat: int,
+                     cr: float, cg: float, cb: float) -> None:
+    """Flat-colour discs of one material (mpm_sim.py:487-517)."""
+    _frame().render_flat(p_res_sub, p_grid_side, radius, target_mat, cr, cg, cb)
+
+
+def tone_map_and_encode() -> None:
+    """Reinhard tone map and gamma of the frame buffer in place (mpm_sim.py:366-383)."""
+    _frame().render_tone_map()
+
+
+def render_vorticity_overlay(p_res_sub: int, p_grid_side: int, vort_scale: float) -> None:
+    """Additive warm/cool squares from the grid curl (mpm_sim.py:520-558)."""
+    _frame().render_vorticity(0, p_res_sub, p_grid_side, vort_scale)
+
+
+def compute_vorticity_grid(vort_scale: float) -> None:
+    """vort_grid = tanh(curl * vort_scale) (mpm_sim.py:560-578)."""
+    engine().render_vorticity_grid(vort_scale)
+
+
+def render_vorticity_rdbu(p_res_sub: int, p_grid_side: int, alpha_max: float) -> None:
+    """mpm_sim.py:581-624"""
+    _frame().render_vorticity(1, p_res_sub, p_grid_side, alpha_max)
+
+
+def render_vorticity_hueshift(p_res_sub: int, p_grid_side: int, strength: float) -> None:
+    """mpm_sim.py:627-668"""
+    _frame().render_vorticity(2, p_res_sub, p_grid_side, strength)
+
+
+def render_vorticity_white(p_res_sub: int, p_grid_side: int, alpha_max: float) -> None:
+    """mpm_sim.py:671-712 (fills the frame buffer with white first)."""
+    _frame().render_vorticity(3, p_res_sub, p_grid_side, alpha_max)
+
+
+class _FrameBuffer:
+    """frame_buffer: (video_res, video_res, 3) fp32, read back with to_numpy()."""
+
+    shape = (video_res, video_res, 3)
+
+    def to_numpy(self) -> np.ndarray:
+        return _frame().read_frame()
+
+
+class _VortGrid:
+    """vort_grid: (n_instances, n_grid, n_grid_y) fp32, read back with to_numpy()."""
+
+    shape = (n_instances, n_grid, n_grid_y)
+
+    def to_numpy(self) -> np.ndarray:
+        return engine().read_vorticity()
+
+
+frame_buffer = _FrameBuffer()
+vort_grid = _VortGrid()

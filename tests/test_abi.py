@@ -98,5 +98,9 @@ def test_facade_constants_match_reference():
     assert s.mu_0_base == pytest.approx(416.6666666666667)
     assert s.lambda_0_base == pytest.approx(277.77777777777777)
     assert s.grid_side == 4 and s.res_sub == 256
-    with pytest.raises(NotImplementedError):
-        s.render_frame_abyss
+    assert s.frame_buffer.shape == (1024, 1024, 3)
+    for name in ("clear_frame_buffer", "clear_frame_buffer_white", "render_frame_abyss",
+                 "render_flat_pass", "tone_map_and_encode", "compute_vorticity_grid",
+                 "render_vorticity_overlay", "render_vorticity_rdbu",
+                 "render_vorticity_hueshift", "render_vorticity_white"):
+        assert callable(getattr(s, name))
