@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JM_ABI_VERSION 2
+#define JM_ABI_VERSION 3
 
 typedef enum jm_status {
   JM_OK = 0,
@@ -82,7 +82,9 @@ typedef struct jm_config {
   int32_t n_grid_y;      /* JELLY_GRID_Y, mpm_sim.py:15; 0 means n_grid */
   int32_t track_glow;    /* 1: run the render-only glow pass (mpm_sim.py:335-338) */
   int32_t path;          /* jm_path */
-  int32_t reserved;
+  int32_t axisym;        /* JELLY_AXISYM, mpm_sim.py:17: 1 = axisymmetric (r, z) variant
+                            (ring-weighted P2G, hoop stress, axis BC; mpm_sim.py:127-133,
+                            163-165,236-304); runs on JM_PATH_PLAIN only */
   double domain_height;  /* JELLY_DOMAIN_H, mpm_sim.py:16 */
 } jm_config;
 

@@ -12,7 +12,7 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libjellymx.so")
 
-JM_ABI_VERSION = 2
+JM_ABI_VERSION = 3
 
 # jm_status
 JM_OK, JM_ERR_ARG, JM_ERR_HIP, JM_ERR_STATE, JM_ERR_NOMEM = range(5)
@@ -50,7 +50,7 @@ class JmConfig(ctypes.Structure):
         ("n_grid_y", ctypes.c_int32),
         ("track_glow", ctypes.c_int32),
         ("path", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("axisym", ctypes.c_int32),
         ("domain_height", ctypes.c_double),
     ]
 

@@ -89,6 +89,7 @@ struct jm_handle {
   float* vort = nullptr;      // renderer: M * G floats, node (i, j) at j * nx + i
   float* rhue = nullptr;      // renderer: 2 * M floats, instance_hue then instance_muscle_hue
   std::vector<float> rhue_host;  // what rhue holds
+  float* hoop = nullptr;      // axisymmetric variant: [m][2][G] stress grids of the hoop term
 
   // host mirrors
   std::vector<float> h_param[5];  // amp, contraction, freq, pdens, E_jelly
@@ -129,12 +130,12 @@ struct jm_handle {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {st.base, tmp.base, sortbuf, hist, acc, t_buf, params, n_live_d,
-                    chunks_d, nchunks_d, stats_d, staging, damp, frame, vort, rhue};
+                    chunks_d, nchunks_d, stats_d, staging, damp, frame, vort, rhue, hoop};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     st.base = tmp.base = nullptr;
     sortbuf = hist = nullptr;
-    acc = t_buf = params = stats_d = staging = damp = frame = vort = rhue = nullptr;
+    acc = t_buf = params = stats_d = staging = damp = frame = vort = rhue = hoop = nullptr;
     frame_res = 0;
     rhue_host.clear();
     n_live_d = nullptr;
@@ -188,6 +189,8 @@ jm::PhysConsts make_consts(const jm_config& c, int nx, int ny) {
   k.pay_lo = (float)(1 - 5e-3);
   k.pay_hi = (float)(1 + 5e-3);
   k.pi = 3.14159265f;
+  k.r_ref_floor = (float)(dx * 0.1);  // mpm_sim.py:246
+  k.r_floor = (float)(dx * 0.05);     // mpm_sim.py:247
   return k;
 }
 
@@ -199,16 +202,17 @@ T* carve(char*& p, size_t n) {
   return r;
 }
 
-size_t state_bytes(size_t n, bool glow) {
+size_t state_bytes(size_t n, bool glow, bool axisym) {
   auto rnd = [](size_t b) { return (b + 255) & ~size_t(255); };
   size_t b = rnd(n * 8) * 3 + rnd(n * 16) * 2 + rnd(n * 4) * 3;  // x v fiber; C F; Jp mat slot
   if (glow) b += rnd(n * 4);
+  if (axisym) b += rnd(n * 4);
   return b;
 }
 
-int alloc_state(DevState& d, size_t n, bool glow) {
+int alloc_state(DevState& d, size_t n, bool glow, bool axisym) {
   void* base = nullptr;
-  size_t bytes = state_bytes(n, glow);
+  size_t bytes = state_bytes(n, glow, axisym);
   if (hipMalloc(&base, bytes) != hipSuccess) return fail(JM_ERR_NOMEM, "hipMalloc(state)");
   d.base = base;
   char* p = static_cast<char*>(base);
@@ -221,6 +225,7 @@ int alloc_state(DevState& d, size_t n, bool glow) {
   d.s.mat = carve<int>(p, n);
   d.s.slot = carve<int>(p, n);
   d.s.glow = glow ? carve<float>(p, n) : nullptr;
+  d.s.r_ref = axisym ? carve<float>(p, n) : nullptr;
   return JM_OK;
 }
 
@@ -644,6 +649,7 @@ jm::PlainArgs plain_args(jm_handle* h, int m0) {
   a.s = h->st.s;
   a.ip = inst_params(h);
   a.acc = grid_buf(h, 0);
+  a.hoop = h->hoop;
   a.t_buf = h->t_buf;
   a.n_live = h->n_live_d;
   a.cap = h->cap;
@@ -664,6 +670,9 @@ int run_plain(jm_handle* h, int64_t steps) {
       jm::PlainArgs a = plain_args(h, m);
       hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for(3 * G, 256 * 4)), dim3(256), 0,
                          h->stream, grid_buf(h, 0) + (size_t)m * 3 * G, (int64_t)3 * G, 0.f);
+      if (h->hoop)  // mpm_sim.py:163-165
+        hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for(2 * G, 256 * 4)), dim3(256), 0,
+                           h->stream, h->hoop + (size_t)m * 2 * G, (int64_t)2 * G, 0.f);
       const dim3 pg(blocks_for(h->n_live[m], 256), 1);
       hipLaunchKernelGGL(jm::k_plain_p2g, pg, dim3(256), 0, h->stream, a);
       hipLaunchKernelGGL(jm::k_plain_gridop, dim3(blocks_for(G, 256), 1), dim3(256), 0,
@@ -777,6 +786,9 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   if (cfg->path != JM_PATH_FUSED && cfg->path != JM_PATH_PLAIN)
     return fail(JM_ERR_ARG, "unknown path");
   if (!(cfg->domain_height > 0.002)) return fail(JM_ERR_ARG, "domain_height too small");
+  if (cfg->axisym != 0 && cfg->axisym != 1) return fail(JM_ERR_ARG, "axisym must be 0 or 1");
+  if (cfg->axisym && cfg->path != JM_PATH_PLAIN)
+    return fail(JM_ERR_ARG, "the axisymmetric variant runs on JM_PATH_PLAIN");
   const int nx = 128 * cfg->quality;
   const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
   if (ny < 8 || ny % 4 != 0) return fail(JM_ERR_ARG, "n_grid_y must be a multiple of 4, >= 8");
@@ -819,8 +831,9 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
       return cleanup(fail(JM_ERR_HIP, "instance group streams"));
   const size_t n = (size_t)M * h->cap;
   const bool glow = cfg->track_glow != 0;
-  int rc = alloc_state(h->st, n, glow);
-  if (rc == JM_OK) rc = alloc_state(h->tmp, n, glow);
+  const bool axisym = cfg->axisym != 0;
+  int rc = alloc_state(h->st, n, glow, axisym);
+  if (rc == JM_OK) rc = alloc_state(h->tmp, n, glow, axisym);
   if (rc != JM_OK) return cleanup(rc);
   auto dm = [&](void** p, size_t bytes) {
     return hipMalloc(p, std::max<size_t>(bytes, 256)) == hipSuccess;
@@ -833,7 +846,8 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
             dm((void**)&h->params, (size_t)6 * M * 4) && dm((void**)&h->n_live_d, M * 4) &&
             dm((void**)&h->nchunks_d, M * 4) && dm((void**)&h->stats_d, (size_t)M * 6 * 4) &&
             dm((void**)&h->staging, (size_t)M * 2 * h->G * 4) &&
-            dm((void**)&h->damp, (size_t)(nx + ny) * 4);
+            dm((void**)&h->damp, (size_t)(nx + ny) * 4) &&
+            (!axisym || dm((void**)&h->hoop, (size_t)M * 2 * h->G * 4));
   if (!ok) return cleanup(fail(JM_ERR_NOMEM, "hipMalloc(engine buffers)"));
   std::vector<float> damp(nx + ny);
   for (int i = 0; i < nx; ++i) damp[i] = jm::damping_factor(i, nx, h->k.damp_x);
@@ -1726,6 +1740,7 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
       n_particles > (int64_t)INT32_MAX / 16)
     return fail(JM_ERR_ARG, "bad sizes");
   if (scalar_bits != 32 && scalar_bits != 64) return fail(JM_ERR_ARG, "scalar_bits: 32 or 64");
+  if (cfg->axisym) return fail(JM_ERR_ARG, "the differentiable path has no axisymmetric variant");
   const int nx = 128 * cfg->quality;
   const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
   if (ny < 8) return fail(JM_ERR_ARG, "n_grid_y too small");

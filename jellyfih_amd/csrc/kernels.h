@@ -27,6 +27,7 @@ struct State {
   float2* fiber;
   float* glow;
   int* slot;
+  float* r_ref;  // spawn radius per particle (axisymmetric variant only, else null)
 };
 
 __device__ __forceinline__ Mat2 as_mat(float4 a) { return {a.x, a.y, a.z, a.w}; }
@@ -59,6 +60,7 @@ __global__ void k_init_particles(State s, int64_t begin, int64_t end) {
     s.F[i] = make_float4(1.f, 0.f, 0.f, 1.f);
     s.Jp[i] = 1.f;
     if (s.glow) s.glow[i] = 0.f;
+    if (s.r_ref) s.r_ref[i] = s.x[i].x;  // mpm_sim.py:727
   }
 }
 
@@ -69,6 +71,7 @@ struct PlainArgs {
   State s;
   InstParams ip;
   float* acc;          // one grid buffer: [m][3][G]
+  float* hoop;         // axisymmetric variant: [m][2][G] mass-weighted s_rr, s_tt; else null
   float* t_buf;        // t_buf[0] = sim_time
   const int* n_live;
   int cap;
@@ -108,6 +111,34 @@ __global__ void __launch_bounds__(256) k_plain_p2g(PlainArgs a) {
   float* px = a.acc + (size_t)m * 3 * a.G;
   float* py = px + a.G;
   float* pm = py + a.G;
+  if (a.hoop) {
+    // axisymmetric variant (mpm_sim.py:236-247,258-267): each particle is a ring of radius
+    // r = x; mass and momentum are weighted by r, and the grid gathers the mass-weighted
+    // Kirchhoff s_rr and the hoop stress of the circumferential stretch r / r_ref
+    const float rp = x.x;
+    const float rr = fmaxf(a.s.r_ref[i], k.r_ref_floor);
+    const float lam = fmaxf(rp, k.r_floor) / rr;
+    const float khoop = ((2.0f * o.mu) * (lam - 1.0f)) + ((o.la * o.J) * (o.J - 1.0f));
+    const float mr = o.mass * rp;
+    float* psr = a.hoop + (size_t)m * 2 * a.G;
+    float* psh = psr + a.G;
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const float d0 = ((float)ii - fx0) * k.dx, d1 = ((float)jj - fx1) * k.dx;
+        const float w = wx[ii] * wy[jj];
+        const float wm = w * o.mass;
+        const int node = (b1 + jj) * k.nx + (b0 + ii);
+        unsafeAtomicAdd(px + node, w * (mr * v.x + (o.affine.a00 * d0 + o.affine.a01 * d1)));
+        unsafeAtomicAdd(py + node, w * (mr * v.y + (o.affine.a10 * d0 + o.affine.a11 * d1)));
+        unsafeAtomicAdd(pm + node, wm * rp);
+        unsafeAtomicAdd(psr + node, wm * o.s00);
+        unsafeAtomicAdd(psh + node, wm * khoop);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int ii = 0; ii < 3; ++ii) {
 #pragma unroll
@@ -131,7 +162,13 @@ __global__ void __launch_bounds__(256) k_plain_gridop(PlainArgs a) {
   float* py = px + a.G;
   const float* pm = py + a.G;
   const int j = node / a.k.nx, i = node - j * a.k.nx;
-  float2 g = grid_node(px[node], py[node], pm[node], i, j, a.k);
+  float2 g;
+  if (a.hoop) {
+    const float* psr = a.hoop + (size_t)m * 2 * a.G;
+    g = grid_node_axisym(px[node], py[node], pm[node], psr[node], psr[a.G + node], i, j, a.k);
+  } else {
+    g = grid_node(px[node], py[node], pm[node], i, j, a.k);
+  }
   px[node] = g.x;
   py[node] = g.y;
 }
@@ -742,6 +779,7 @@ __global__ void __launch_bounds__(256) k_sort_keys(SortArgs a) {
   a.tmp.fiber[i] = a.s.fiber[i];
   a.tmp.slot[i] = a.s.slot[i];
   if (a.s.glow) a.tmp.glow[i] = a.s.glow[i];
+  if (a.s.r_ref) a.tmp.r_ref[i] = a.s.r_ref[i];
 }
 
 __global__ void __launch_bounds__(kSortTile) k_sort_rank(SortArgs a, const int* keys, int shift) {
@@ -841,6 +879,7 @@ __global__ void __launch_bounds__(256) k_sort_gather(SortArgs a, const int* perm
   a.s.fiber[i] = a.tmp.fiber[src];
   a.s.slot[i] = a.tmp.slot[src];
   if (a.s.glow) a.s.glow[i] = a.tmp.glow[src];
+  if (a.s.r_ref) a.s.r_ref[i] = a.tmp.r_ref[src];
 }
 
 // ---------------------------------------------------------------------------------------

@@ -22,6 +22,8 @@ struct PhysConsts {
   float dt, dx, inv_dx, p_mass, gravity_dt, stress_scale, apic_scale;
   float mu_base, la_base, mu_pay, la_pay, water_la, act_freq;
   float x_lo, x_hi, y_hi, sig_floor, pay_lo, pay_hi, pi;
+  // axisymmetric variant (mpm_sim.py:246-247): floors of the spawn and current radius
+  float r_ref_floor, r_floor;
 };
 
 struct Mat2 {
@@ -124,6 +126,9 @@ struct P2GOut {
   float mass;
   float q0, q1;      // Jp <- (Jp * q0) * q1
   bool jp_changes;   // false when q0 == q1 == 1 (every non-payload particle with finite F)
+  // for the axisymmetric variant's hoop term: the unscaled Kirchhoff stress's (0,0)
+  // entry, the material's Lame parameters and J after plasticity
+  float s00, mu, la, J;
 };
 
 // F update, SVD, plasticity, fixed-corotated + active stress, APIC affine term for one
@@ -165,6 +170,10 @@ __device__ __forceinline__ P2GOut p2g_update(int mat, const Mat2& F0, const Mat2
       o.affine = {k.stress_scale * dg + mass * C.a00, mass * C.a01, mass * C.a10,
                   k.stress_scale * dg + mass * C.a11};
       o.mass = mass;
+      o.s00 = dg;
+      o.mu = mu;
+      o.la = la;
+      o.J = J;
       return o;
     }
   }
@@ -218,6 +227,10 @@ __device__ __forceinline__ P2GOut p2g_update(int mat, const Mat2& F0, const Mat2
               k.stress_scale * st.a10 + mass * C.a10, k.stress_scale * st.a11 + mass * C.a11};
   o.F = F;
   o.mass = mass;
+  o.s00 = st.a00;
+  o.mu = mu;
+  o.la = la;
+  o.J = J;
   return o;
 }
 
@@ -249,6 +262,31 @@ __device__ __forceinline__ float2 grid_node(float px, float py, float m, int i, 
   g0 = g0 * damp;
   g1 = g1 * damp;
   if (i < 3 && g0 < 0.0f) g0 = 0.0f;
+  if (i > k.nx - 3 && g0 > 0.0f) g0 = 0.0f;
+  if (j < 3 && g1 < 0.0f) g1 = 0.0f;
+  if (j > k.ny - 3 && g1 > 0.0f) g1 = 0.0f;
+  return make_float2(g0, g1);
+}
+
+// Grid update of one node in the axisymmetric variant (mpm_sim.py:274-309 with axisym):
+// the same normalisation and damping, then the hoop correction dt (s_rr - s_tt) / (m r) off
+// the axis cell (s_rr, s_tt: the node's mass-weighted Kirchhoff stresses, r = (i + 1/2) dx)
+// and the axis condition v_r = 0 at i = 0 in place of the left wall clamp.
+__device__ __forceinline__ float2 grid_node_axisym(float px, float py, float m, float srr,
+                                                   float shoop, int i, int j,
+                                                   const PhysConsts& k) {
+  if (!(m > 0.0f)) return make_float2(px, py);
+  const float rm = __builtin_amdgcn_rcpf(m);
+  float g0 = px * rm, g1 = py * rm;
+  const float damp = k.damp[i] * k.damp[k.nx + j];
+  g0 = g0 * damp;
+  g1 = g1 * damp;
+  const float r_node = ((float)i + 0.5f) * k.dx;
+  if (r_node > k.dx) {
+    const float hoop = (srr - shoop) / m;
+    g0 = g0 + (k.dt * hoop) / r_node;
+  }
+  if (i == 0) g0 = 0.0f;
   if (i > k.nx - 3 && g0 > 0.0f) g0 = 0.0f;
   if (j < 3 && g1 < 0.0f) g1 = 0.0f;
   if (j > k.ny - 3 && g1 > 0.0f) g1 = 0.0f;

@@ -62,17 +62,13 @@ __device__ __forceinline__ float2 splat_centre(float2 pos, int m, const RenderAr
                      ((a.domain_height - pos.y) / a.domain_height + (float)row) * (float)a.res_sub);
 }
 
-// render_frame_abyss (mpm_sim.py:385-477): one thread per live particle (blockIdx.y =
-// instance); colour and intensity by material, then an additive splat with a (1 - d^2)^4
-// falloff over a disc of radius 1.5 * radius.
-__global__ void __launch_bounds__(256) k_render_abyss(RenderArgs a) {
-  const int m = blockIdx.y;
-  const int local = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= a.M || local >= a.n_live[m]) return;
-  const size_t i = (size_t)m * a.cap + local;
-  const float2 pos = a.s.x[i];
+// Colour x intensity of one particle's glow (mpm_sim.py:390-459); false where the
+// reference draws nothing (dead slot, x < 0, intensity <= 0.001).
+__device__ __forceinline__ bool abyss_colour(const RenderArgs& a, int m, size_t i, float2& pos,
+                                             float3& ci) {
+  pos = a.s.x[i];
   const int mat = a.s.mat[i];
-  if (mat < 0 || pos.x < 0.0f) return;
+  if (mat < 0 || pos.x < 0.0f) return false;
   const PhysConsts& k = a.k;
   const float act = activation(a.t_buf[0], a.freq[m], a.contraction[m], k);
   const float2 vv = a.s.v[i];
@@ -112,26 +108,98 @@ __global__ void __launch_bounds__(256) k_render_abyss(RenderArgs a) {
     color = make_float3(1.0f, 0.2f, 0.0f);
     intensity = 0.8f;
   }
-  if (!(intensity > 0.001f)) return;
-  const float2 c = splat_centre(pos, m, a);
+  if (!(intensity > 0.001f)) return false;
+  ci = make_float3(color.x * intensity, color.y * intensity, color.z * intensity);
+  return true;
+}
+
+// LDS floats of a workgroup's splat tile: up to 1365 pixels x 3 channels (16 KB).
+constexpr int kSplatTile = 4096;
+
+// render_frame_abyss (mpm_sim.py:385-477): one thread per particle slot (blockIdx.y =
+// instance); colour and intensity by material, then an additive splat with a (1 - d^2)^4
+// falloff over a disc of radius 1.5 * radius. The particles of a workgroup are consecutive
+// in the engine's cell-sorted order, so their discs cover a small pixel box: when the box
+// fits kSplatTile, the workgroup adds its splats into an LDS copy of the box and then adds
+// the box's non-zero pixels to the frame with one global atomic per channel, instead of one
+// per disc pixel (fp32 sums in another order, as with the reference's atomics).
+__global__ void __launch_bounds__(256) k_render_abyss(RenderArgs a) {
+  __shared__ float tile[kSplatTile];
+  __shared__ int box[4];  // min px, max px, min py, max py of the workgroup's discs
+  const int m = blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  float2 pos = make_float2(0.0f, 0.0f);
+  float3 ci = make_float3(0.0f, 0.0f, 0.0f);
+  bool draw = m < a.M && local < a.n_live[m] &&
+              abyss_colour(a, m, (size_t)m * a.cap + local, pos, ci);
+  float2 c = make_float2(0.0f, 0.0f);
   const float draw_r = a.radius * 1.5f;
-  const int lx = (int)(c.x - draw_r), hx = (int)(c.x + draw_r);
-  const int ly = (int)(c.y - draw_r), hy = (int)(c.y + draw_r);
-  const float r2 = draw_r * draw_r;
-  // the reference's loops over [lx, hx] x [ly, hy] skip pixels outside the frame; iterate
-  // over the intersection only
-  for (int px = max(lx, 0); px <= min(hx, a.res - 1); ++px)
-    for (int py = max(ly, 0); py <= min(hy, a.res - 1); ++py) {
-      const float dxp = (float)px - c.x, dyp = (float)py - c.y;
-      const float dn = (dxp * dxp + dyp * dyp) / r2;
-      if (dn < 1.0f) {
-        const float u = 1.0f - dn, u2 = u * u, falloff = u2 * u2;
-        float* f = a.fb + ((size_t)py * a.res + px) * 3;
-        unsafeAtomicAdd(f + 0, color.x * intensity * falloff);
-        unsafeAtomicAdd(f + 1, color.y * intensity * falloff);
-        unsafeAtomicAdd(f + 2, color.z * intensity * falloff);
+  int lx = 0, hx = -1, ly = 0, hy = -1;
+  if (draw) {
+    c = splat_centre(pos, m, a);
+    // the reference's loops over [int(c - r), int(c + r)] skip pixels outside the frame;
+    // iterate over the intersection only
+    lx = max((int)(c.x - draw_r), 0);
+    hx = min((int)(c.x + draw_r), a.res - 1);
+    ly = max((int)(c.y - draw_r), 0);
+    hy = min((int)(c.y + draw_r), a.res - 1);
+    draw = lx <= hx && ly <= hy;
+  }
+  if (threadIdx.x == 0) {
+    box[0] = INT_MAX;
+    box[1] = -1;
+    box[2] = INT_MAX;
+    box[3] = -1;
+  }
+  __syncthreads();
+  if (draw) {
+    atomicMin(&box[0], lx);
+    atomicMax(&box[1], hx);
+    atomicMin(&box[2], ly);
+    atomicMax(&box[3], hy);
+  }
+  __syncthreads();
+  const int bx0 = box[0], by0 = box[2];
+  if (bx0 > box[1]) return;  // no disc in this workgroup (uniform)
+  const int bw = box[1] - bx0 + 1, bh = box[3] - by0 + 1;
+  const bool in_lds = (int64_t)bw * bh * 3 <= kSplatTile;  // uniform
+  if (in_lds) {
+    for (int q = threadIdx.x; q < bw * bh * 3; q += blockDim.x) tile[q] = 0.0f;
+    __syncthreads();
+  }
+  if (draw) {
+    const float r2 = draw_r * draw_r;
+    for (int px = lx; px <= hx; ++px)
+      for (int py = ly; py <= hy; ++py) {
+        const float dxp = (float)px - c.x, dyp = (float)py - c.y;
+        const float dn = (dxp * dxp + dyp * dyp) / r2;
+        if (dn < 1.0f) {
+          const float u = 1.0f - dn, u2 = u * u, falloff = u2 * u2;
+          if (in_lds) {
+            float* t = tile + ((py - by0) * bw + (px - bx0)) * 3;
+            atomicAdd(t + 0, ci.x * falloff);
+            atomicAdd(t + 1, ci.y * falloff);
+            atomicAdd(t + 2, ci.z * falloff);
+          } else {
+            float* f = a.fb + ((size_t)py * a.res + px) * 3;
+            unsafeAtomicAdd(f + 0, ci.x * falloff);
+            unsafeAtomicAdd(f + 1, ci.y * falloff);
+            unsafeAtomicAdd(f + 2, ci.z * falloff);
+          }
+        }
       }
+  }
+  if (!in_lds) return;
+  __syncthreads();
+  for (int q = threadIdx.x; q < bw * bh; q += blockDim.x) {
+    const int py = by0 + q / bw, px = bx0 + q % bw;
+    float* f = a.fb + ((size_t)py * a.res + px) * 3;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const float val = tile[3 * q + ch];
+      if (val != 0.0f) unsafeAtomicAdd(f + ch, val);
     }
+  }
 }
 
 // render_flat_pass (mpm_sim.py:487-517): particles of one material set the pixels of their
@@ -160,15 +228,25 @@ __global__ void __launch_bounds__(256) k_render_flat(RenderArgs a) {
 }
 
 // tone_map_and_encode (mpm_sim.py:367-383): Reinhard with exposure 2.5, a 0.15 knee, then
-// gamma 1/2.2, per pixel in place.
+// gamma 1/2.2, per value in place; four values per thread and load (the frame buffer is a
+// hipMalloc allocation, so 16-byte aligned), the n % 4 tail one at a time.
+__device__ __forceinline__ float tone_map(float x) {
+  const float e = x * 2.5f;
+  float mapped = e / (e + 1.0f);
+  mapped = mapped / (mapped + 0.15f);
+  return powf(mapped, (float)(1.0 / 2.2));  // the reference's exponent: 1/2.2 rounded once
+}
+
 __global__ void k_tone_map(float* fb, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+  const int64_t n4 = n / 4;
+  float4* fb4 = reinterpret_cast<float4*>(fb);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float e = fb[i] * 2.5f;
-    float mapped = e / (e + 1.0f);
-    mapped = mapped / (mapped + 0.15f);
-    fb[i] = powf(mapped, (float)(1.0 / 2.2));  // the reference's exponent: 1/2.2 rounded once
+    const float4 q = fb4[i];
+    fb4[i] = make_float4(tone_map(q.x), tone_map(q.y), tone_map(q.z), tone_map(q.w));
   }
+  if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4) fb[4 * n4 + threadIdx.x] =
+      tone_map(fb[4 * n4 + threadIdx.x]);
 }
 
 // Curl of the post-grid-op velocity at interior node (i, j) of instance m with mass, from

@@ -52,7 +52,7 @@ class Engine:
 
     def __init__(self, n_instances: int = 16, n_particles: int = 80000, quality: int = 1,
                  n_grid_y: int | None = None, domain_height: float = 1.0,
-                 track_glow: bool = False, path: str = "fused"):
+                 track_glow: bool = False, path: str = "fused", axisym: bool = False):
         L = load()
         cfg = JmConfig()
         call("jm_default_config", ctypes.byref(cfg))
@@ -63,6 +63,7 @@ class Engine:
         cfg.track_glow = int(bool(track_glow))
         cfg.path = {"fused": _lib.JM_PATH_FUSED, "plain": _lib.JM_PATH_PLAIN}[path]
         cfg.domain_height = float(domain_height)
+        cfg.axisym = int(bool(axisym))
         h = ctypes.c_void_p()
         call("jm_create", ctypes.byref(cfg), ctypes.byref(h))
         self._L = L
@@ -75,6 +76,7 @@ class Engine:
         self.domain_height = cfg.domain_height
         self.path = path
         self.track_glow = bool(track_glow)
+        self.axisym = bool(axisym)
 
     # -- lifecycle ------------------------------------------------------------------
     def close(self) -> None:

@@ -15,7 +15,8 @@ those names and their meaning, backed by the HIP engine (``libjellymx.so``):
 
 The device engine is created on first use, so importing this module never touches a GPU.
 The renderer (``mpm_sim.py:343-712``) draws into a device frame buffer read back with
-``frame_buffer.to_numpy()``. The axisymmetric variant is not part of this engine.
+``frame_buffer.to_numpy()``. ``JELLY_AXISYM=1`` selects the axisymmetric variant
+(``mpm_sim.py:127-133,163-165,236-304``), which runs on the engine's plain path.
 """
 from __future__ import annotations
 
@@ -67,9 +68,6 @@ video_res = 1024
 grid_side = int(math.ceil(math.sqrt(n_instances)))
 res_sub = video_res // grid_side
 
-if axisym:
-    raise NotImplementedError("JELLY_AXISYM=1: the axisymmetric variant is not in this engine")
-
 _engine = None
 
 
@@ -78,9 +76,11 @@ def engine():
     global _engine
     if _engine is None:
         from .engine import Engine
+        # the axisymmetric variant (JELLY_AXISYM=1) runs on the plain path
         _engine = Engine(n_instances=n_instances, n_particles=n_particles, quality=quality,
                          n_grid_y=n_grid_y, domain_height=domain_height,
-                         track_glow=os.environ.get("JELLY_GLOW", "0") == "1")
+                         track_glow=os.environ.get("JELLY_GLOW", "0") == "1",
+                         path="plain" if axisym else "fused", axisym=axisym)
     return _engine
 
 

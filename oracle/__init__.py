@@ -39,7 +39,7 @@ class OracleConsts(ctypes.Structure):
         ("nx", ctypes.c_int32),
         ("ny", ctypes.c_int32),
         ("track_glow", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("axisym", ctypes.c_int32),
         ("dt", ctypes.c_double),
         ("dx", ctypes.c_double),
         ("inv_dx", ctypes.c_double),
@@ -110,6 +110,11 @@ def lib():
             fn.argtypes = ([ctypes.POINTER(OracleConsts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             P, P, P, P, ctypes.POINTER(ctypes.c_int32), P, P, P, P, P, P,
                             P, P, P, P, P, P, ctypes.c_int, ctypes.c_int])
+            fn = getattr(L, f"oracle_run_axisym_{suffix}")
+            fn.restype = ctypes.c_int
+            fn.argtypes = ([ctypes.POINTER(OracleConsts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            P, P, P, P, ctypes.POINTER(ctypes.c_int32), P, P, P, P, P, P,
+                            P, P, P, P, P, P, P, P, P, ctypes.c_int, ctypes.c_int])
             fn = getattr(L, f"oracle_activation_{suffix}")
             fn.restype = None
             fn.argtypes = [ctypes.POINTER(OracleConsts), ctypes.c_int, P, P, P, P]
@@ -211,8 +216,11 @@ class OracleSim:
     Arrays use the reference layout: x, v, fiber (M,N,2); C, F (M,N,2,2); material, Jp (M,N);
     grid_v (M,nx,ny,2); grid_m (M,nx,ny)."""
 
-    def __init__(self, M: int, N: int, consts: OracleConsts | None = None, dtype=np.float32):
-        self.c = consts if consts is not None else reference_consts()
+    def __init__(self, M: int, N: int, consts: OracleConsts | None = None, dtype=np.float32,
+                 axisym: bool = False):
+        self.c = OracleConsts.from_buffer_copy(consts) if consts is not None \
+            else reference_consts()
+        self.c.axisym = int(bool(axisym))
         self.M, self.N = int(M), int(N)
         self.dtype = np.dtype(dtype)
         nx, ny = self.c.nx, self.c.ny
@@ -228,6 +236,11 @@ class OracleSim:
         self.grid_v = np.zeros((M, nx, ny, 2), d)
         self.grid_m = np.zeros((M, nx, ny), d)
         self.t = np.zeros(1, d)
+        # axisymmetric variant (mpm_sim.py:127-133): spawn radius and the hoop-term grids
+        self.axisym = bool(axisym)
+        self.r_ref = np.zeros((M, N), d)
+        self.grid_srr = np.zeros((M, nx, ny), d) if axisym else None
+        self.grid_shoop = np.zeros((M, nx, ny), d) if axisym else None
         # mpm_sim.py:143-149 defaults
         self.amp = np.full(M, 500.0, d)
         self.contraction = np.full(M, 0.20, d)
@@ -253,17 +266,23 @@ class OracleSim:
         self.C[m] = 0
         self.Jp[m] = 1
         self.fiber[m] = np.asarray(fiber, np.float32)[:N].astype(self.dtype)
+        self.r_ref[m] = self.x[m, :, 0]                      # mpm_sim.py:727
 
     def run(self, nsteps: int = 1, stages: int = ST_SUBSTEP, nthreads: int = 1) -> None:
         sfx, ct = _suffix(self.dtype)
-        fn = getattr(lib(), f"oracle_run_{sfx}")
         glow = self.glow if (stages & ST_GLOW) else None
-        rc = fn(ctypes.byref(self.c), stages, self.M, self.N, _ptr(self.x, ct), _ptr(self.v, ct),
+        args = [ctypes.byref(self.c), stages, self.M, self.N, _ptr(self.x, ct), _ptr(self.v, ct),
                 _ptr(self.C, ct), _ptr(self.F, ct), _ptr(self.material, ctypes.c_int32),
                 _ptr(self.Jp, ct), _ptr(self.fiber, ct), _ptr(glow, ct), _ptr(self.grid_v, ct),
                 _ptr(self.grid_m, ct), _ptr(self.t, ct), _ptr(self.amp, ct),
                 _ptr(self.contraction, ct), _ptr(self.freq, ct), _ptr(self.pdens, ct),
-                _ptr(self.mu_j, ct), _ptr(self.la_j, ct), int(nsteps), int(nthreads))
+                _ptr(self.mu_j, ct), _ptr(self.la_j, ct)]
+        if self.axisym:
+            fn = getattr(lib(), f"oracle_run_axisym_{sfx}")
+            args += [_ptr(self.r_ref, ct), _ptr(self.grid_srr, ct), _ptr(self.grid_shoop, ct)]
+        else:
+            fn = getattr(lib(), f"oracle_run_{sfx}")
+        rc = fn(*args, int(nsteps), int(nthreads))
         if rc != 0:
             raise RuntimeError(f"oracle_run_{sfx} failed ({rc})")
 

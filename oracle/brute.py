@@ -10,7 +10,8 @@ shared transcription mistake is unlikely:
 * the B-spline weights and the grid node loop are vectorised with numpy;
 * the grid op is a dense vectorised pass.
 
-Used only to pin ``liboracle.so``'s fp64 substep on states with ``det F > 0``.
+Used only to pin ``liboracle.so``'s fp64 substep on states with ``det F > 0``, in the planar
+and the axisymmetric variant (``mpm_sim.py:127-133,163-165,236-304``).
 """
 from __future__ import annotations
 
@@ -22,10 +23,11 @@ def _weights(fx: np.ndarray) -> np.ndarray:
     return np.stack([0.5 * (1.5 - fx) ** 2, 0.75 - (fx - 1.0) ** 2, 0.5 * (fx - 0.5) ** 2])
 
 
-def substep(state: dict, c, act_params: dict) -> dict:
+def substep(state: dict, c, act_params: dict, axisym: bool = False) -> dict:
     """One substep of one instance, fp64. ``state`` keys: x (N,2) v (N,2) C (N,2,2) F (N,2,2)
-    mat (N,) Jp (N,) fiber (N,2), t (float). ``c`` is an OracleConsts. ``act_params`` keys:
-    amp, contraction, freq, pdens, mu_j, la_j. Returns the new state and the grid."""
+    mat (N,) Jp (N,) fiber (N,2), t (float), and r_ref (N,) for the axisymmetric variant.
+    ``c`` is an OracleConsts. ``act_params`` keys: amp, contraction, freq, pdens, mu_j, la_j.
+    Returns the new state and the grid."""
     nx, ny = c.nx, c.ny
     dx, inv_dx, dt = c.dx, c.inv_dx, c.dt
     x = state["x"].astype(np.float64).copy()
@@ -44,6 +46,8 @@ def substep(state: dict, c, act_params: dict) -> dict:
 
     gv = np.zeros((nx, ny, 2))
     gm = np.zeros((nx, ny))
+    s_rr = np.zeros((nx, ny))       # axisymmetric: mass-weighted Kirchhoff sigma_rr
+    s_tt = np.zeros((nx, ny))       # and sigma_theta_theta
     for p in range(x.shape[0]):
         if mat[p] < 0:
             continue
@@ -76,6 +80,11 @@ def substep(state: dict, c, act_params: dict) -> dict:
         stress = 2 * mu * (F[p] - R) @ F[p].T + np.eye(2) * la * J * (J - 1)
         if mat[p] == 3:
             stress += np.outer(fiber[p], fiber[p]) * act_params["amp"] * a * J
+        ring = 1.0
+        if axisym:   # the particle is a ring of radius r = x; hoop stretch r / r_ref
+            ring = x[p, 0]
+            lam = max(x[p, 0], 0.05 * dx) / max(state["r_ref"][p], 0.1 * dx)
+            k_rr, k_tt = stress[0, 0], 2 * mu * (lam - 1) + la * J * (J - 1)
         stress = (-dt * c.p_vol * 4 * inv_dx * inv_dx) * stress
         affine = stress + mass * C[p]
         for i in range(3):
@@ -83,8 +92,12 @@ def substep(state: dict, c, act_params: dict) -> dict:
                 off = np.array([i, j])
                 dpos = (off - fx) * dx
                 wt = w[i, 0] * w[j, 1]
-                gv[base[0] + i, base[1] + j] += wt * (mass * v[p] + affine @ dpos)
-                gm[base[0] + i, base[1] + j] += wt * mass
+                node = (base[0] + i, base[1] + j)
+                gv[node] += wt * (mass * ring * v[p] + affine @ dpos)
+                gm[node] += wt * mass * ring
+                if axisym:
+                    s_rr[node] += wt * mass * k_rr
+                    s_tt[node] += wt * mass * k_tt
 
     # grid op (mpm_sim.py:273-309), vectorised over nodes
     ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
@@ -98,7 +111,13 @@ def substep(state: dict, c, act_params: dict) -> dict:
     damp = np.where(jj > ny - by, damp * (0.95 + 0.05 * (ny - jj) / by), damp)
     g = np.where(has[..., None], g * damp[..., None], g)
     g0, g1 = g[..., 0], g[..., 1]
-    g0 = np.where(has & (ii < 3) & (g0 < 0), 0.0, g0)
+    if axisym:
+        r_node = (ii + 0.5) * dx
+        hoop = (s_rr - s_tt) / np.where(has, gm, 1.0)
+        g0 = np.where(has & (r_node > dx), g0 + dt * hoop / r_node, g0)
+        g0 = np.where(has & (ii == 0), 0.0, g0)
+    else:
+        g0 = np.where(has & (ii < 3) & (g0 < 0), 0.0, g0)
     g0 = np.where(has & (ii > nx - 3) & (g0 > 0), 0.0, g0)
     g1 = np.where(has & (jj < 3) & (g1 < 0), 0.0, g1)
     g1 = np.where(has & (jj > ny - 3) & (g1 > 0), 0.0, g1)

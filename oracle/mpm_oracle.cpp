@@ -18,6 +18,9 @@
 //     G2P                mpm_sim.py:311-333
 //     glow (optional)    mpm_sim.py:335-338
 //     time advance       mpm_sim.py:340
+//   axisymmetric variant mpm_sim.py:127-133 (fields), :163-165 (reset), :236-247 (sigma_rr,
+//                        hoop stress), :258-267 (ring-weighted scatter), :292-304 (hoop
+//                        correction, axis BC); r_ref = spawn x (:727)
 //   payload/body stats   mpm_sim.py:740-762
 //   2x2 SVD              taichi 1.7.4 python/taichi/_funcs.py `_polar_decompose2d` and
 //                        `_svd2d` (third-party, not vendored in the reference; the
@@ -55,7 +58,7 @@ struct OracleConsts {
   int32_t nx;             // n_grid (x cells), mpm_sim.py:14
   int32_t ny;             // n_grid_y (y cells), mpm_sim.py:15
   int32_t track_glow;     // run the render-only glow pass (mpm_sim.py:335-338)
-  int32_t reserved;
+  int32_t axisym;  // JELLY_AXISYM (mpm_sim.py:17)
   double dt;              // 2e-5 / quality, mpm_sim.py:19
   double dx;              // 1 / n_grid, mpm_sim.py:18
   double inv_dx;          // float(n_grid), mpm_sim.py:18
@@ -110,6 +113,8 @@ struct Consts {
   T x_lo, x_hi, y_hi;
   T sig_floor, pay_lo, pay_hi;
   T pi;
+  bool axisym;
+  T r_ref_floor, r_floor;
 
   explicit Consts(const OracleConsts& c) {
     nx = c.nx;
@@ -137,6 +142,9 @@ struct Consts {
     pay_lo = T(1.0 - 5e-3);             // mpm_sim.py:215
     pay_hi = T(1.0 + 5e-3);
     pi = T(3.14159265);  // mpm_sim.py:180
+    axisym = c.axisym != 0;
+    r_ref_floor = T(c.dx * 0.1);  // mpm_sim.py:246 (Python f64 product, then fp)
+    r_floor = T(c.dx * 0.05);     // mpm_sim.py:247
   }
 };
 
@@ -216,6 +224,9 @@ struct Fields {
   const T* pdens;        // (M,) instance_payload_density
   const T* mu_j;         // (M,) mu_jelly (per instance; E_jelly promoted to runtime)
   const T* la_j;         // (M,) lambda_jelly
+  const T* r_ref;        // (M,N) spawn radius (axisymmetric variant only, else null)
+  T* g_srr;              // (M,nx,ny) mass-weighted Kirchhoff sigma_rr (axisymmetric only)
+  T* g_shoop;            // (M,nx,ny) mass-weighted Kirchhoff sigma_theta_theta
 };
 
 enum Stage : int {
@@ -231,6 +242,8 @@ void p2g_instance(const Consts<T>& k, const Fields<T>& f, int m, T t) {
   const int N = f.N, nx = k.nx, ny = k.ny;
   T* gv = f.grid_v + (size_t)m * nx * ny * 2;
   T* gm = f.grid_m + (size_t)m * nx * ny;
+  T* gsr = k.axisym ? f.g_srr + (size_t)m * nx * ny : nullptr;
+  T* gsh = k.axisym ? f.g_shoop + (size_t)m * nx * ny : nullptr;
   const T act = activation(k, t, f.freq[m], f.contraction[m]);
   for (int p = 0; p < N; ++p) {
     const size_t ip = (size_t)m * N + p;
@@ -311,6 +324,17 @@ void p2g_instance(const Consts<T>& k, const Fields<T>& f, int m, T t) {
       st.a11 = st.a11 + ((d1 * d1) * cp) * J;
     }
 
+    // mpm_sim.py:236-247 axisymmetric variant: Kirchhoff sigma_rr and the hoop stress of
+    // the circumferential stretch r / r_ref, taken before the stress is scaled
+    T krr = T(0), khoop = T(0), rp = T(0);
+    if (k.axisym) {
+      krr = st.a00;
+      rp = x0;
+      const T rr = std::fmax(f.r_ref[ip], k.r_ref_floor);
+      const T lam = std::fmax(rp, k.r_floor) / rr;
+      khoop = ((T(2) * mu) * (lam - T(1))) + ((la * J) * (J - T(1)));
+    }
+
     // mpm_sim.py:250,254
     st = {k.stress_scale * st.a00, k.stress_scale * st.a01, k.stress_scale * st.a10,
           k.stress_scale * st.a11};
@@ -328,9 +352,18 @@ void p2g_instance(const Consts<T>& k, const Fields<T>& f, int m, T t) {
       for (int j = 0; j < 3; ++j) {
         T d0 = (T(i) - fx0) * k.dx, d1 = (T(j) - fx1) * k.dx;
         T wgt = wx[i] * wy[j];
+        size_t node = (size_t)(b0 + i) * ny + (b1 + j);
+        if (k.axisym) {  // mpm_sim.py:260-267: ring mass m r, stresses weighted by m
+          T mr = mass * rp;
+          gv[2 * node] += wgt * ((mr * v0) + (af.a00 * d0 + af.a01 * d1));
+          gv[2 * node + 1] += wgt * ((mr * v1) + (af.a10 * d0 + af.a11 * d1));
+          gm[node] += (wgt * mass) * rp;
+          gsr[node] += (wgt * mass) * krr;
+          gsh[node] += (wgt * mass) * khoop;
+          continue;
+        }
         T c0 = wgt * (mv0 + (af.a00 * d0 + af.a01 * d1));
         T c1 = wgt * (mv1 + (af.a10 * d0 + af.a11 * d1));
-        size_t node = (size_t)(b0 + i) * ny + (b1 + j);
         gv[2 * node] += c0;
         gv[2 * node + 1] += c1;
         gm[node] += wgt * mass;
@@ -344,6 +377,8 @@ void gridop_instance(const Consts<T>& k, const Fields<T>& f, int m) {
   const int nx = k.nx, ny = k.ny;
   T* gv = f.grid_v + (size_t)m * nx * ny * 2;
   T* gm = f.grid_m + (size_t)m * nx * ny;
+  const T* gsr = k.axisym ? f.g_srr + (size_t)m * nx * ny : nullptr;
+  const T* gsh = k.axisym ? f.g_shoop + (size_t)m * nx * ny : nullptr;
   for (int i = 0; i < nx; ++i) {
     for (int j = 0; j < ny; ++j) {
       size_t node = (size_t)i * ny + j;
@@ -358,8 +393,18 @@ void gridop_instance(const Consts<T>& k, const Fields<T>& f, int m) {
       if (j > ny - k.damp_y) damp = damp * (T(0.95) + T(0.05) * T(ny - j) / T(k.damp_y));
       g0 = g0 * damp;
       g1 = g1 * damp;
-      // mpm_sim.py:306-309
-      if (i < 3 && g0 < T(0)) g0 = T(0);
+      if (k.axisym) {
+        // mpm_sim.py:295-299 hoop correction (sigma_rr - sigma_tt) / r, off the axis cell
+        const T r_node = (T(i) + T(0.5)) * k.dx;
+        if (r_node > k.dx) {
+          const T hoop = (gsr[node] - gsh[node]) / mass;
+          g0 = g0 + (k.dt * hoop) / r_node;
+        }
+        if (i == 0) g0 = T(0);  // mpm_sim.py:302-304 axis: no radial velocity
+      } else if (i < 3 && g0 < T(0)) {
+        g0 = T(0);  // mpm_sim.py:306
+      }
+      // mpm_sim.py:307-309
       if (i > nx - 3 && g0 > T(0)) g0 = T(0);
       if (j < 3 && g1 < T(0)) g1 = T(0);
       if (j > ny - 3 && g1 > T(0)) g1 = T(0);
@@ -439,6 +484,10 @@ void run_instance(const Consts<T>& k, const Fields<T>& f, int m, T t0, int nstep
     if (stages & ST_RESET) {
       std::fill(f.grid_v + (size_t)m * G * 2, f.grid_v + (size_t)(m + 1) * G * 2, T(0));
       std::fill(f.grid_m + (size_t)m * G, f.grid_m + (size_t)(m + 1) * G, T(0));
+      if (k.axisym) {  // mpm_sim.py:163-165
+        std::fill(f.g_srr + (size_t)m * G, f.g_srr + (size_t)(m + 1) * G, T(0));
+        std::fill(f.g_shoop + (size_t)m * G, f.g_shoop + (size_t)(m + 1) * G, T(0));
+      }
     }
     if (stages & ST_P2G) p2g_instance(k, f, m, t);
     if (stages & ST_GRIDOP) gridop_instance(k, f, m);
@@ -452,11 +501,12 @@ template <typename T>
 int run(const OracleConsts* c, int stages, int M, int N, T* x, T* v, T* C, T* F,
         const int32_t* mat, T* Jp, const T* fiber, T* glow, T* grid_v, T* grid_m, T* t,
         const T* amp, const T* contraction, const T* freq, const T* pdens, const T* mu_j,
-        const T* la_j, int nsteps, int nthreads) {
+        const T* la_j, const T* r_ref, T* g_srr, T* g_shoop, int nsteps, int nthreads) {
   if (!c || M <= 0 || N < 0 || nsteps < 0 || c->nx < 3 || c->ny < 3) return 1;
+  if (c->axisym && (!r_ref || !g_srr || !g_shoop)) return 2;
   Consts<T> k(*c);
   Fields<T> f{M,   N,       x,    v,     C,     F,     mat, Jp,  fiber, glow,
-              grid_v, grid_m, amp, contraction, freq, pdens, mu_j, la_j};
+              grid_v, grid_m, amp, contraction, freq, pdens, mu_j, la_j, r_ref, g_srr, g_shoop};
   const T t0 = *t;
 #ifdef _OPENMP
   int nt = nthreads > 0 ? nthreads : 1;
@@ -502,7 +552,8 @@ int oracle_run_f32(const OracleConsts* c, int stages, int M, int N, float* x, fl
                    const float* contraction, const float* freq, const float* pdens,
                    const float* mu_j, const float* la_j, int nsteps, int nthreads) {
   return run<float>(c, stages, M, N, x, v, C, F, mat, Jp, fiber, glow, grid_v, grid_m, t, amp,
-                    contraction, freq, pdens, mu_j, la_j, nsteps, nthreads);
+                    contraction, freq, pdens, mu_j, la_j, nullptr, nullptr, nullptr, nsteps,
+                    nthreads);
 }
 
 int oracle_run_f64(const OracleConsts* c, int stages, int M, int N, double* x, double* v,
@@ -511,7 +562,35 @@ int oracle_run_f64(const OracleConsts* c, int stages, int M, int N, double* x, d
                    const double* contraction, const double* freq, const double* pdens,
                    const double* mu_j, const double* la_j, int nsteps, int nthreads) {
   return run<double>(c, stages, M, N, x, v, C, F, mat, Jp, fiber, glow, grid_v, grid_m, t, amp,
-                     contraction, freq, pdens, mu_j, la_j, nsteps, nthreads);
+                     contraction, freq, pdens, mu_j, la_j, nullptr, nullptr, nullptr, nsteps,
+                     nthreads);
+}
+
+// The axisymmetric variant (JELLY_AXISYM=1, mpm_sim.py:127-133,163-165,236-304): as above
+// with the spawn radii r_ref (M,N) and the two stress grids (M,nx,ny) of the hoop term;
+// consts->axisym must be set.
+int oracle_run_axisym_f32(const OracleConsts* c, int stages, int M, int N, float* x, float* v,
+                          float* C, float* F, const int32_t* mat, float* Jp,
+                          const float* fiber, float* glow, float* grid_v, float* grid_m,
+                          float* t, const float* amp, const float* contraction,
+                          const float* freq, const float* pdens, const float* mu_j,
+                          const float* la_j, const float* r_ref, float* g_srr, float* g_shoop,
+                          int nsteps, int nthreads) {
+  return run<float>(c, stages, M, N, x, v, C, F, mat, Jp, fiber, glow, grid_v, grid_m, t, amp,
+                    contraction, freq, pdens, mu_j, la_j, r_ref, g_srr, g_shoop, nsteps,
+                    nthreads);
+}
+
+int oracle_run_axisym_f64(const OracleConsts* c, int stages, int M, int N, double* x,
+                          double* v, double* C, double* F, const int32_t* mat, double* Jp,
+                          const double* fiber, double* glow, double* grid_v, double* grid_m,
+                          double* t, const double* amp, const double* contraction,
+                          const double* freq, const double* pdens, const double* mu_j,
+                          const double* la_j, const double* r_ref, double* g_srr,
+                          double* g_shoop, int nsteps, int nthreads) {
+  return run<double>(c, stages, M, N, x, v, C, F, mat, Jp, fiber, glow, grid_v, grid_m, t, amp,
+                     contraction, freq, pdens, mu_j, la_j, r_ref, g_srr, g_shoop, nsteps,
+                     nthreads);
 }
 
 // Activation value per instance (mpm_sim.py:173-181).

@@ -57,7 +57,8 @@ def test_default_config_is_the_reference_default():
 
 @pytest.mark.parametrize("field,value", [("abi_version", 99), ("n_instances", 0),
                                          ("n_particles", -5), ("quality", 0), ("path", 7),
-                                         ("n_grid_y", 6), ("domain_height", 0.0)])
+                                         ("n_grid_y", 6), ("domain_height", 0.0),
+                                         ("axisym", 2)])
 def test_create_rejects_bad_config_before_device_work(field, value):
     cfg = _lib.JmConfig()
     _lib.call("jm_default_config", ctypes.byref(cfg))
@@ -85,6 +86,13 @@ def test_engine_fails_loudly_without_a_device():
     from jellyfih_amd.engine import Engine
     with pytest.raises(_lib.JmError):
         Engine(2, 100)
+
+
+def test_axisym_needs_the_plain_path():
+    """The axisymmetric variant is refused on the fused path before any device work."""
+    from jellyfih_amd.engine import Engine
+    with pytest.raises(_lib.JmError, match="JM_PATH_PLAIN"):
+        Engine(2, 100, axisym=True)
 
 
 def test_facade_constants_match_reference():
