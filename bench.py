@@ -21,7 +21,9 @@ Also reported on the same line:
   checkpoints, the same keeping every block's stored trajectory (no recomputation), and the
   fused (non-differentiable, fp32) forward of the same episode for comparison;
 * render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
-  map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back.
+  map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back;
+* axisym: the axisymmetric variant's rate on the same batch and on the tall view
+  configuration (windows of 8 substeps from fresh loads).
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -278,6 +280,46 @@ def render_frames(eng, live: int, reps: int = 20) -> dict:
     return out
 
 
+def axisym_rates(pos, mat, fib, genomes, reps: int = 10, n: int = 8) -> dict:
+    """The axisymmetric variant (JELLY_AXISYM=1, plain path, one launch per stage for the
+    batch): live particle-substeps/s on the same 32x80K population, and on
+    helpers/view_axisym.py's configuration (one seeded tank, 128x256, H=2, 160K slots).
+    The reference's variant diverges within ~10 substeps next to the axis (its affine
+    momentum is not weighted by r), so each timed window is `n` substeps from a fresh load,
+    `reps` windows, HIP events on the engine's stream."""
+    import torch
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.engine import Engine
+    out = {"window_substeps": n, "windows": reps}
+    cases = {"batch_32x80k": (pos, mat, fib, genomes, {}),
+             "view_tall_1x160k": None}
+    tall = phenotype.population([0], 160000, 128, domain_height=2.0)
+    cases["view_tall_1x160k"] = (tall[0], tall[1], tall[2], tall[3],
+                                 {"n_grid_y": 256, "domain_height": 2.0})
+    for name, (P, Mt, Fb, G, kw) in cases.items():
+        eng = Engine(P.shape[0], P.shape[1], path="plain", axisym=True, **kw)
+        stream = torch.cuda.ExternalStream(eng.stream)
+        live = int((Mt >= 0).sum())
+        total = 0.0
+        for r in range(reps + 1):
+            load_engine(eng, P, Mt, Fb, G)
+            eng.set_time(0.0)
+            eng.substep(1)               # commits the load; not timed
+            eng.sync()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            eng.substep(n)
+            e1.record(stream)
+            eng.sync()
+            if r:                        # the first window warms up
+                total += e0.elapsed_time(e1) / 1e3
+        eng.close()
+        out[name] = {"live_particles": live, "particle_substeps_per_s": live * n * reps / total,
+                     "ms_per_substep": total / (n * reps) * 1e3}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +338,8 @@ def main() -> int:
                     help="run only the batched tall-tank forward + backward")
     ap.add_argument("--only-render", action="store_true",
                     help="run only the renderer leg (after the warm-up blocks)")
+    ap.add_argument("--only-axisym", action="store_true",
+                    help="run only the axisymmetric variant's leg")
     ap.add_argument("--batched-steps", type=int, default=4000)
     ap.add_argument("--batched-checkpoint", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -327,6 +371,10 @@ def main() -> int:
     pos, mat, fib, genomes, infos = build_workload(M, N, range(M))
     live = int((mat >= 0).sum())
     log(f"live particles {live} (slots {M * N})")
+
+    if args.only_axisym:
+        print(json.dumps({"axisym": axisym_rates(pos, mat, fib, genomes)}), flush=True)
+        return 0
 
     if args.copy_probe:
         src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -387,6 +435,10 @@ def main() -> int:
     rend = render_frames(eng, live)
     log(f"render {rend}")
 
+    log("axisymmetric variant (plain path)")
+    axisym = axisym_rates(pos, mat, fib, genomes)
+    log(f"axisym {axisym}")
+
     plain_value = None
     if not args.no_plain:
         log("plain path (one launch per stage per instance)")
@@ -445,6 +497,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "diff": diff,
         "render": rend,
+        "axisym": axisym,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -661,18 +661,37 @@ jm::PlainArgs plain_args(jm_handle* h, int m0) {
 }
 
 // Plain path: per substep, for each instance separately: reset its grid, P2G, grid op, G2P;
-// then the time advance.
+// then the time advance. The axisymmetric variant, which runs only here, launches each
+// stage once for the whole batch instead (blockIdx.y = instance).
 int run_plain(jm_handle* h, int64_t steps) {
   const int M = h->M, G = h->G;
+  if (h->hoop) {
+    jm::PlainArgs a = plain_args(h, 0);
+    int max_live = 0;
+    for (int m = 0; m < M; ++m) max_live = std::max(max_live, h->n_live[m]);
+    const dim3 pg(blocks_for(max_live, 256), M), gg(blocks_for(G, 256), M);
+    for (int64_t s = 0; s < steps; ++s) {
+      hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for((int64_t)3 * M * G, 256 * 4)),
+                         dim3(256), 0, h->stream, grid_buf(h, 0), (int64_t)3 * M * G, 0.f);
+      hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for((int64_t)2 * M * G, 256 * 4)),
+                         dim3(256), 0, h->stream, h->hoop, (int64_t)2 * M * G, 0.f);
+      hipLaunchKernelGGL(jm::k_axisym_p2g, pg, dim3(256), 0, h->stream, a);
+      hipLaunchKernelGGL(jm::k_plain_gridop, gg, dim3(256), 0, h->stream, a);
+      hipLaunchKernelGGL(jm::k_plain_g2p, pg, dim3(256), 0, h->stream, a);
+      hipLaunchKernelGGL(jm::k_time_advance, dim3(1), dim3(64), 0, h->stream, h->t_buf,
+                         h->k.dt);
+    }
+    HIP_TRY(hipGetLastError());
+    h->last_read = 0;
+    h->grid_is_velocity = true;
+    return JM_OK;
+  }
   for (int64_t s = 0; s < steps; ++s) {
     for (int m = 0; m < M; ++m) {
       if (h->n_live[m] == 0) continue;
       jm::PlainArgs a = plain_args(h, m);
       hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for(3 * G, 256 * 4)), dim3(256), 0,
                          h->stream, grid_buf(h, 0) + (size_t)m * 3 * G, (int64_t)3 * G, 0.f);
-      if (h->hoop)  // mpm_sim.py:163-165
-        hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for(2 * G, 256 * 4)), dim3(256), 0,
-                           h->stream, h->hoop + (size_t)m * 2 * G, (int64_t)2 * G, 0.f);
       const dim3 pg(blocks_for(h->n_live[m], 256), 1);
       hipLaunchKernelGGL(jm::k_plain_p2g, pg, dim3(256), 0, h->stream, a);
       hipLaunchKernelGGL(jm::k_plain_gridop, dim3(blocks_for(G, 256), 1), dim3(256), 0,

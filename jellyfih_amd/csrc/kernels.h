@@ -111,34 +111,6 @@ __global__ void __launch_bounds__(256) k_plain_p2g(PlainArgs a) {
   float* px = a.acc + (size_t)m * 3 * a.G;
   float* py = px + a.G;
   float* pm = py + a.G;
-  if (a.hoop) {
-    // axisymmetric variant (mpm_sim.py:236-247,258-267): each particle is a ring of radius
-    // r = x; mass and momentum are weighted by r, and the grid gathers the mass-weighted
-    // Kirchhoff s_rr and the hoop stress of the circumferential stretch r / r_ref
-    const float rp = x.x;
-    const float rr = fmaxf(a.s.r_ref[i], k.r_ref_floor);
-    const float lam = fmaxf(rp, k.r_floor) / rr;
-    const float khoop = ((2.0f * o.mu) * (lam - 1.0f)) + ((o.la * o.J) * (o.J - 1.0f));
-    const float mr = o.mass * rp;
-    float* psr = a.hoop + (size_t)m * 2 * a.G;
-    float* psh = psr + a.G;
-#pragma unroll
-    for (int ii = 0; ii < 3; ++ii) {
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj) {
-        const float d0 = ((float)ii - fx0) * k.dx, d1 = ((float)jj - fx1) * k.dx;
-        const float w = wx[ii] * wy[jj];
-        const float wm = w * o.mass;
-        const int node = (b1 + jj) * k.nx + (b0 + ii);
-        unsafeAtomicAdd(px + node, w * (mr * v.x + (o.affine.a00 * d0 + o.affine.a01 * d1)));
-        unsafeAtomicAdd(py + node, w * (mr * v.y + (o.affine.a10 * d0 + o.affine.a11 * d1)));
-        unsafeAtomicAdd(pm + node, wm * rp);
-        unsafeAtomicAdd(psr + node, wm * o.s00);
-        unsafeAtomicAdd(psh + node, wm * khoop);
-      }
-    }
-    return;
-  }
 #pragma unroll
   for (int ii = 0; ii < 3; ++ii) {
 #pragma unroll
@@ -149,6 +121,124 @@ __global__ void __launch_bounds__(256) k_plain_p2g(PlainArgs a) {
       unsafeAtomicAdd(px + node, w * (mv0 + (o.affine.a00 * d0 + o.affine.a01 * d1)));
       unsafeAtomicAdd(py + node, w * (mv1 + (o.affine.a10 * d0 + o.affine.a11 * d1)));
       unsafeAtomicAdd(pm + node, w * o.mass);
+    }
+  }
+}
+
+// LDS floats of an axisymmetric P2G workgroup's node box: 5 planes x up to 1638 nodes.
+constexpr int kAxiTile = 8192;
+
+// P2G of the axisymmetric variant (mpm_sim.py:167-271 with axisym, 236-247,258-267), one
+// launch for the batch (blockIdx.y = instance): each particle is a ring of radius r = x, so
+// mass and particle momentum are weighted by r, and two more grids gather the mass-weighted
+// Kirchhoff s_rr and the hoop stress of the circumferential stretch r / r_ref. The 256
+// particles of a workgroup are consecutive in the engine's order (fill order, or cell order
+// after a sort) and reach a small box of nodes: when the box's five planes fit kAxiTile,
+// the workgroup adds into an LDS copy and then adds each non-zero box value to the grid
+// once; otherwise it adds to the grid directly.
+__global__ void __launch_bounds__(256) k_axisym_p2g(PlainArgs a) {
+  __shared__ float tile[kAxiTile];
+  __shared__ int box[4];  // min/max node column, min/max node row
+  const int m = a.m0 + blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  const PhysConsts& k = a.k;
+  bool act = false;
+  int b0 = 0, b1 = 0;
+  float fx0 = 0.f, fx1 = 0.f;
+  size_t i = 0;
+  int mt = -1;
+  float2 x = make_float2(0.f, 0.f);
+  if (local < a.n_live[m]) {
+    i = (size_t)m * a.cap + local;
+    mt = a.s.mat[i];
+    if (mt >= 0) {
+      x = a.s.x[i];
+      base_frac(x.x, k.inv_dx, b0, fx0);
+      base_frac(x.y, k.inv_dx, b1, fx1);
+      act = in_range(b0, b1, k);
+    }
+  }
+  P2GOut o{};
+  float khoop = 0.f, mr = 0.f;
+  float2 v = make_float2(0.f, 0.f);
+  if (act) {
+    const float actv = activation(a.t_buf[0], a.ip.freq[m], a.ip.contraction[m], k);
+    float2 fib = make_float2(0.f, 1.f);
+    if (mt == 3) fib = a.s.fiber[i];
+    o = p2g_update(mt, as_mat(a.s.F[i]), as_mat(a.s.C[i]), fib.x, fib.y, a.ip.amp[m] * actv,
+                   a.ip.mu_j[m], a.ip.la_j[m], a.ip.pdens[m], k);
+    a.s.F[i] = as_f4(o.F);
+    if (o.jp_changes) a.s.Jp[i] = (a.s.Jp[i] * o.q0) * o.q1;
+    v = a.s.v[i];
+    const float rp = x.x;
+    const float rr = fmaxf(a.s.r_ref[i], k.r_ref_floor);
+    const float lam = fmaxf(rp, k.r_floor) / rr;
+    khoop = ((2.0f * o.mu) * (lam - 1.0f)) + ((o.la * o.J) * (o.J - 1.0f));
+    mr = o.mass * rp;
+  }
+  if (threadIdx.x == 0) {
+    box[0] = INT_MAX;
+    box[1] = -1;
+    box[2] = INT_MAX;
+    box[3] = -1;
+  }
+  __syncthreads();
+  if (act) {
+    atomicMin(&box[0], b0);
+    atomicMax(&box[1], b0 + 2);
+    atomicMin(&box[2], b1);
+    atomicMax(&box[3], b1 + 2);
+  }
+  __syncthreads();
+  const int bx0 = box[0], by0 = box[2];
+  if (bx0 > box[1]) return;  // no in-range particle in this workgroup (uniform)
+  const int bw = box[1] - bx0 + 1, area = bw * (box[3] - by0 + 1);
+  const bool in_lds = 5 * area <= kAxiTile;  // uniform
+  if (in_lds) {
+    for (int q = threadIdx.x; q < 5 * area; q += blockDim.x) tile[q] = 0.f;
+    __syncthreads();
+  }
+  float* px = a.acc + (size_t)m * 3 * a.G;
+  float* psr = a.hoop + (size_t)m * 2 * a.G;
+  if (act) {
+    float wx[3], wy[3];
+    bspline(fx0, wx);
+    bspline(fx1, wy);
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        const float d0 = ((float)ii - fx0) * k.dx, d1 = ((float)jj - fx1) * k.dx;
+        const float w = wx[ii] * wy[jj];
+        const float wm = w * o.mass;
+        const float c[5] = {w * (mr * v.x + (o.affine.a00 * d0 + o.affine.a01 * d1)),
+                            w * (mr * v.y + (o.affine.a10 * d0 + o.affine.a11 * d1)),
+                            wm * x.x, wm * o.s00, wm * khoop};
+        if (in_lds) {
+          const int q = (b1 + jj - by0) * bw + (b0 + ii - bx0);
+#pragma unroll
+          for (int pl = 0; pl < 5; ++pl) atomicAdd(&tile[pl * area + q], c[pl]);
+        } else {
+          const int node = (b1 + jj) * k.nx + (b0 + ii);
+#pragma unroll
+          for (int pl = 0; pl < 3; ++pl) unsafeAtomicAdd(px + pl * a.G + node, c[pl]);
+          unsafeAtomicAdd(psr + node, c[3]);
+          unsafeAtomicAdd(psr + a.G + node, c[4]);
+        }
+      }
+    }
+  }
+  if (!in_lds) return;
+  __syncthreads();
+  for (int q = threadIdx.x; q < area; q += blockDim.x) {
+    const int node = (by0 + q / bw) * k.nx + bx0 + q % bw;
+#pragma unroll
+    for (int pl = 0; pl < 5; ++pl) {
+      const float val = tile[pl * area + q];
+      if (val != 0.f) {
+        float* dst = pl < 3 ? px + pl * a.G : psr + (pl - 3) * a.G;
+        unsafeAtomicAdd(dst + node, val);
+      }
     }
   }
 }
