@@ -199,14 +199,15 @@ def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
     tank, st0 = DiffTank.from_tanks(pos, mat, fib, n_grid_y=256, domain_height=2.0)
     dev = tank.device
 
-    def episode(grad: bool, n: int = steps) -> float:
+    def episode(grad: bool, n: int = steps, resort: int = 0) -> float:
         st = st0.clone()
         vec = lambda a: torch.tensor(np.asarray(a, np.float32), device=dev).requires_grad_(grad)
         amp, c, f = vec(np.full(M, 500.0)), vec(con), vec(freq)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         with torch.set_grad_enabled(grad):
-            out = tank.run(st, None, amp, c, f, 700.0, 2.5, 0.0, n, checkpoint)
+            out = tank.run(st, None, amp, c, f, 700.0, 2.5, 0.0, n, checkpoint,
+                           resort_every=resort)
             y = tank.payload_com_y(out).sum()
             if grad:
                 y.backward()
@@ -218,12 +219,19 @@ def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
     log(f"batched diff forward {fwd:.2f} s")
     fb = episode(True)
     log(f"batched diff forward + backward {fb:.2f} s")
+    episode(True, 2 * checkpoint, 1)
+    fwd_rs = episode(False, steps, 1)
+    fb_rs = episode(True, steps, 1)
+    log(f"re-sorted every block: forward {fwd_rs:.2f} s, forward + backward {fb_rs:.2f} s")
     tank.close()
     return {"workload": f"{M} tall tanks 128x256 (H=2, {slots} slots, {live} live), {steps} "
                         f"substeps, checkpoint every {checkpoint}, fp32",
             "live_particles": live, "fused_forward_s": fused, "forward_s": fwd,
             "fwd_bwd_s": fb, "fwd_bwd_over_forward": fb / fwd,
             "fwd_bwd_over_fused_forward": fb / fused,
+            "resorted_every_block": {"forward_s": fwd_rs, "fwd_bwd_s": fb_rs,
+                                     "fwd_bwd_over_forward": fb_rs / fwd_rs,
+                                     "fwd_bwd_over_fused_forward": fb_rs / fused},
             "fused_rate": live * steps / fused}
 
 

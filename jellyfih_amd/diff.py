@@ -140,6 +140,7 @@ class DiffTank:
              64 if dtype == torch.float64 else 32, ctypes.byref(h))
         self._h = h
         self._L = load()
+        self._owner = None      # a view (see _view) keeps the tank that owns the handle
 
     @classmethod
     def from_tanks(cls, pos, mat, fiber, order: str = "cell", **kw):
@@ -170,9 +171,39 @@ class DiffTank:
         return tank, torch.from_numpy(state).to(tank.device, dtype)
 
     def close(self) -> None:
+        if getattr(self, "_owner", None) is not None:   # a view: the owner frees the handle
+            self._h = None
+            return
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.jm_diff_destroy(self._h)
         self._h = None
+
+    # -- re-sorting between blocks ----------------------------------------------------------
+    def cell_permutation(self, state: torch.Tensor) -> torch.Tensor:
+        """The permutation (P,) that orders the particles of each instance by grid cell at
+        their current positions, in the ``order="cell"`` packing order (snaking strips of 8
+        cell rows), ties by current position in the array; instances keep their blocks."""
+        if self._order is not None:
+            raise ValueError("re-sorting needs particles packed instance by instance")
+        x = state[:, :2].detach().to(torch.float32)
+        b = torch.trunc(x * float(self.nx) - 0.5).nan_to_num(0.0).long()
+        bx = b[:, 0].clamp(0, self.nx - 1)
+        by = b[:, 1].clamp(0, self.ny - 1)
+        s, r = by // 8, by % 8
+        key = (s * self.nx + torch.where(s % 2 == 1, self.nx - 1 - bx, bx)) * 8 + r
+        span = ((self.ny + 7) // 8) * 8 * self.nx
+        return torch.argsort(self.inst.long() * span + key, stable=True)
+
+    def _view(self, perm: torch.Tensor) -> "DiffTank":
+        """This tank with its particles in the order ``perm`` (a within-instance permutation
+        of this tank's order), sharing the device handle."""
+        v = object.__new__(DiffTank)
+        v.__dict__.update(self.__dict__)
+        v._owner = self if self._owner is None else self._owner
+        v.inst = self.inst[perm].contiguous()
+        v.mat = self.mat[perm].contiguous()
+        v.fiber = self.fiber[perm].contiguous()
+        return v
 
     def __del__(self):
         try:
@@ -263,14 +294,22 @@ class DiffTank:
     # -- autograd ---------------------------------------------------------------------
     def run(self, state0: torch.Tensor, amp_scale=None, amp=500.0, contraction=0.2, freq=1.0,
             E_jelly=700.0, payload_density=2.5, t0: float = 0.0, nsteps: int = 100,
-            checkpoint: int = 100, recompute: bool = True) -> torch.Tensor:
+            checkpoint: int = 100, recompute: bool = True, resort_every: int = 0
+            ) -> torch.Tensor:
         """``nsteps`` substeps from ``state0`` at time ``t0``, as autograd blocks of
         ``checkpoint`` substeps. Per-instance arguments are scalars or (M,) tensors.
 
         ``recompute`` (default): a block keeps only its input state and re-runs its
         substeps in the backward pass. ``recompute=False`` keeps every block's states and
         grids from the forward pass instead: no second forward, at ((checkpoint + 1) * P * 12
-        + checkpoint * M * 3 * nx * ny) scalars of device memory per block."""
+        + checkpoint * M * 3 * nx * ny) scalars of device memory per block.
+
+        ``resort_every`` = K > 0: before every K-th block after the first, the particles
+        are re-ordered by their current cells (``cell_permutation``), as the fused engine
+        re-sorts every K substeps, so the kernels' LDS tiles stay compact over long
+        episodes. The reordering is a gather of the state and of ``amp_scale`` (autograd
+        scatters their adjoints back); the returned state is in this tank's order. Results
+        change only by the order of floating-point sums."""
         if amp_scale is None:
             amp_scale = torch.ones(self.P, dtype=self.dtype, device=self.device)
 
@@ -285,13 +324,24 @@ class DiffTank:
         state = state0
         fp64 = self.dtype == torch.float64
         t = float(t0) if fp64 else float(np.float32(t0))
-        done = 0
+        done = blocks = 0
+        cur, perm, amp_cur = self, None, amp_scale
         while done < nsteps:
+            if resort_every > 0 and blocks > 0 and blocks % int(resort_every) == 0:
+                p = cur.cell_permutation(state)
+                state, amp_cur = state[p], amp_cur[p]
+                perm = p if perm is None else perm[p]
+                cur = self._view(perm)
             n = min(int(checkpoint), nsteps - done)
-            state = _Block.apply(state, amp_scale, amp, contraction, freq, E_jelly, pdens,
-                                 self, t, n, recompute)
+            state = _Block.apply(state, amp_cur, amp, contraction, freq, E_jelly, pdens,
+                                 cur, t, n, recompute)
             t = clock(t, n, self.dt, fp64)
             done += n
+            blocks += 1
+        if perm is not None:
+            inv = torch.empty_like(perm)
+            inv[perm] = torch.arange(perm.numel(), device=perm.device)
+            state = state[inv]
         return state
 
     def payload_com_y(self, state: torch.Tensor) -> torch.Tensor:

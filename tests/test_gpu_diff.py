@@ -159,7 +159,7 @@ def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype):
     assert np.all(gpar[:, 3] == 0)
 
 
-def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True):
+def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every=0):
     """Autograd through DiffTank.run: d(payload CoM y displacement) w.r.t. the state,
     the per-particle multiplier and the per-instance parameters."""
     tank, st, amp, par = _gpu(cs, dtype)
@@ -167,7 +167,8 @@ def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True):
     amp.requires_grad_(True)
     leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2, 4)]
     out = tank.run(st, amp, *leaves[:3], E_jelly=leaves[3], payload_density=par[:, 3],
-                   t0=cs.t0, nsteps=cs.steps, checkpoint=checkpoint, recompute=recompute)
+                   t0=cs.t0, nsteps=cs.steps, checkpoint=checkpoint, recompute=recompute,
+                   resort_every=resort_every)
     y = tank.payload_com_y(out) - tank.payload_com_y(st.detach())
     y.sum().backward()
     g = lambda t: t.grad.detach().cpu().numpy().astype(np.float64)
@@ -250,3 +251,29 @@ def test_gradients_independent_of_checkpoint_interval(require_gpu):
             say(f"checkpoint {k} vs 300, {n}: {err:.2e} (repeat of 300: {floor[n]:.2e})")
             if n != "state":
                 assert err <= max(10 * floor[n], 1e-9) and err <= 1e-6, (k, n, err)
+
+
+def test_gradients_with_resorting_between_blocks(require_gpu):
+    """Re-sorting the particles by cell before every checkpoint block (a gather of the state
+    and multiplier; autograd scatters the adjoints back) gives the objective and gradients
+    of the unsorted run, in the caller's particle order, up to the run-to-run noise of
+    atomics order (measured by repeating the unsorted run). fp64, 300 substeps, blocks of
+    100."""
+    say("reference tank seed0, 300 substeps, fp64: re-sorted every block vs not")
+    cs, _, _, _ = reference_tank_case(300)
+    base = _gpu_gradients(cs, checkpoint=100, dtype="float64")
+    again = _gpu_gradients(cs, checkpoint=100, dtype="float64")
+    rs = _gpu_gradients(cs, checkpoint=100, dtype="float64", resort_every=1)
+    body = cs.mat > 0
+
+    def groups(r):
+        return {"body v": r[1][body, 2:4], "amp_scale": r[2], "params": r[3]}
+
+    assert abs(rs[0] - base[0]) <= 1e-9 * abs(base[0]) + 1e-15, (rs[0], base[0])
+    b = groups(base)
+    floor = {n: np.linalg.norm(v - b[n]) / np.linalg.norm(b[n]) for n, v in groups(again).items()}
+    for n, v in groups(rs).items():
+        err = np.linalg.norm(v - b[n]) / np.linalg.norm(b[n])
+        say(f"re-sorted vs not, {n}: {err:.2e} (repeat: {floor[n]:.2e})")
+        if n != "body v":
+            assert err <= max(10 * floor[n], 1e-9) and err <= 1e-6, (n, err)
