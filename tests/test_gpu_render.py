@@ -203,3 +203,45 @@ def test_vorticity_overlays(H):
         assert changed.sum() > 0
         _ulps(got, want, 4, name)
     eng.close()
+
+
+def test_render_edge_cases():
+    """Edge cases against the oracle: an empty instance, instances whose tiles fall outside
+    the frame (grid_side^2 < M and oversized tiles), radius 0 (a disc of no pixels), a disc
+    larger than the LDS box of a workgroup (the global-atomic fallback), and re-allocation
+    of the frame at another resolution."""
+    say("render edge cases")
+    M, N = 5, 12000
+    eng = Engine(M, N)
+    for m in range(M):
+        if m == 2:
+            continue                                 # instance 2 stays empty
+        pos, mat, fib = small_tank(10 + m, N, region=(0.3, 0.7, 0.1, 0.4))
+        eng.load_particles(m, pos, mat, fib)
+    eng.substep(50)
+    eng.sync()
+    st = _state(eng)
+    hue = np.linspace(0.1, 0.9, M).astype(np.float32)
+    mhue = np.linspace(0.9, 0.1, M).astype(np.float32)
+    cases = [(128, 2, 96, 1.0), (128, 2, 64, 0.0), (200, 3, 80, 12.0), (96, 1, 96, 2.0)]
+    for res, grid_side, res_sub, radius in cases:
+        say(f"  res {res} grid_side {grid_side} res_sub {res_sub} radius {radius}")
+        eng.render_clear(res, 0.0)
+        eng.render_abyss(res_sub, grid_side, radius, hue, mhue, 0)
+        got = eng.read_frame()
+        want = render.render_frame_abyss(np.zeros((res, res, 3), np.float32), st["x"],
+                                         st["v"], st["Jp"], st["mat"], st["t"], st["freq"],
+                                         st["contraction"], hue, mhue, 0, res_sub, grid_side,
+                                         radius, eng.domain_height, _consts(eng))
+        if radius == 0.0:
+            assert not want.any() and not got.any()
+        else:
+            assert want.max() > 0
+            _close_additive(got, want, f"abyss res {res} radius {radius}")
+        eng.render_clear(res, 0.98)
+        eng.render_flat(res_sub, grid_side, radius, 1, 0.1, 0.2, 0.3)
+        wflat = render.render_flat_pass(np.full((res, res, 3), 0.98, np.float32), st["x"],
+                                        st["mat"], 1, (0.1, 0.2, 0.3), res_sub, grid_side,
+                                        radius, eng.domain_height)
+        assert (eng.read_frame() == wflat).all()
+    eng.close()
