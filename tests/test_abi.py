@@ -79,6 +79,19 @@ def test_null_handle_is_rejected():
         _lib.call("jm_set_time", None, ctypes.c_float(0.0))
 
 
+def test_renderer_entry_points_reject_a_null_handle():
+    L = _lib.load()
+    f = (ctypes.c_float * 4)()
+    assert L.jm_render_clear(None, 64, 0.0) == _lib.JM_ERR_ARG
+    assert L.jm_render_abyss(None, 32, 2, 1.0, f, f, 0) == _lib.JM_ERR_ARG
+    assert L.jm_render_flat(None, 32, 2, 1.0, 1, 0.1, 0.2, 0.3) == _lib.JM_ERR_ARG
+    assert L.jm_render_tone_map(None) == _lib.JM_ERR_ARG
+    assert L.jm_render_vorticity_grid(None, 1.0) == _lib.JM_ERR_ARG
+    assert L.jm_render_vorticity(None, 1, 32, 2, 0.5) == _lib.JM_ERR_ARG
+    assert L.jm_read_frame(None, f, 4) == _lib.JM_ERR_ARG
+    assert L.jm_read_vorticity(None, f, 4) == _lib.JM_ERR_ARG
+
+
 def test_engine_fails_loudly_without_a_device():
     import torch
     if torch.cuda.is_available():
