@@ -122,11 +122,13 @@ def render_frame_abyss(fb, x, v, Jp, mat, t, freq, contraction, hue, muscle_hue,
     ci = color * intensity[:, None]
 
     def add(dist_sq):
-        dn = dist_sq / r2
-        keep = dn < f32(1.0)
-        u = f32(1.0) - dn
-        u2 = u * u
-        return ci * (u2 * u2)[:, None], keep
+        # radius 0: d^2 / 0 is inf or NaN, so no pixel passes dn < 1, as in the reference
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dn = dist_sq / r2
+            keep = dn < f32(1.0)
+            u = f32(1.0) - dn
+            u2 = u * u
+            return ci * (u2 * u2)[:, None], keep
 
     for py, px, vals, _ in _splat(fb, cx, cy, draw_r, add):
         for c in range(3):
