@@ -86,17 +86,7 @@ struct Tile {
   bool use;
 };
 
-// Minimum over the 64 lanes (all active), uniform in the wave: DPP row_shr by 1, 2, 4, 8
-// leaves each row's minimum in its last lane (a lane with no source keeps its own value),
-// then the four row results combine through readlane.
-__device__ __forceinline__ int wave_min(int v) {
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));  // row_shr:1
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));  // row_shr:2
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));  // row_shr:4
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));  // row_shr:8
-  return min(min(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
-             min(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
-}
+using jm::wave_min;
 
 // Bounding box of the active particles' base cells over the workgroup, and whether they
 // share one instance and their stencil window (box + 2-node halo) fits `cap` nodes.

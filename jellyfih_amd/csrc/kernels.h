@@ -125,8 +125,9 @@ __global__ void __launch_bounds__(256) k_plain_p2g(PlainArgs a) {
   }
 }
 
-// LDS floats of an axisymmetric P2G workgroup's node box: 5 planes x up to 1638 nodes.
-constexpr int kAxiTile = 8192;
+// LDS accumulators of an axisymmetric P2G workgroup's node box: 5 planes x up to 819
+// nodes, 64-bit fixed point (32 KB).
+constexpr int kAxiTile = 4096;
 
 // P2G of the axisymmetric variant (mpm_sim.py:167-271 with axisym, 236-247,258-267), one
 // launch for the batch (blockIdx.y = instance): each particle is a ring of radius r = x, so
@@ -134,11 +135,16 @@ constexpr int kAxiTile = 8192;
 // Kirchhoff s_rr and the hoop stress of the circumferential stretch r / r_ref. The 256
 // particles of a workgroup are consecutive in the engine's order (fill order, or cell order
 // after a sort) and reach a small box of nodes: when the box's five planes fit kAxiTile,
-// the workgroup adds into an LDS copy and then adds each non-zero box value to the grid
-// once; otherwise it adds to the grid directly.
+// the workgroup adds into an LDS copy, as 64-bit fixed point with 50 fractional bits as in
+// k_fused (integer LDS atomics run at full rate on gfx950, fp32 ones do not; the r-weighted
+// node masses near the axis are ~1e-8, so they need the fine resolution), and then adds
+// each non-zero box value to the grid once with an fp32 atomic; a particle whose
+// contributions could reach 1 in magnitude (or are not finite: a diverged state) and every
+// particle of a box that does not fit add to the grid directly. The box bounds are wave minima
+// combined by one LDS atomic per wave.
 __global__ void __launch_bounds__(256) k_axisym_p2g(PlainArgs a) {
-  __shared__ float tile[kAxiTile];
-  __shared__ int box[4];  // min/max node column, min/max node row
+  __shared__ unsigned long long tile[kAxiTile];
+  __shared__ int box[4];  // min node column, -max, min node row, -max
   const int m = a.m0 + blockIdx.y;
   const int local = blockIdx.x * blockDim.x + threadIdx.x;
   const PhysConsts& k = a.k;
@@ -176,31 +182,38 @@ __global__ void __launch_bounds__(256) k_axisym_p2g(PlainArgs a) {
     khoop = ((2.0f * o.mu) * (lam - 1.0f)) + ((o.la * o.J) * (o.J - 1.0f));
     mr = o.mass * rp;
   }
-  if (threadIdx.x == 0) {
-    box[0] = INT_MAX;
-    box[1] = -1;
-    box[2] = INT_MAX;
-    box[3] = -1;
-  }
+  if (threadIdx.x < 4) box[threadIdx.x] = INT_MAX;
   __syncthreads();
-  if (act) {
-    atomicMin(&box[0], b0);
-    atomicMax(&box[1], b0 + 2);
-    atomicMin(&box[2], b1);
-    atomicMax(&box[3], b1 + 2);
+  {
+    const int w0 = wave_min(act ? b0 : INT_MAX), w1 = wave_min(act ? -(b0 + 2) : INT_MAX);
+    const int w2 = wave_min(act ? b1 : INT_MAX), w3 = wave_min(act ? -(b1 + 2) : INT_MAX);
+    if ((threadIdx.x & 63) == 0 && w0 != INT_MAX) {
+      atomicMin(&box[0], w0);
+      atomicMin(&box[1], w1);
+      atomicMin(&box[2], w2);
+      atomicMin(&box[3], w3);
+    }
   }
   __syncthreads();
   const int bx0 = box[0], by0 = box[2];
-  if (bx0 > box[1]) return;  // no in-range particle in this workgroup (uniform)
-  const int bw = box[1] - bx0 + 1, area = bw * (box[3] - by0 + 1);
+  if (bx0 == INT_MAX) return;  // no in-range particle in this workgroup (uniform)
+  const int bw = -box[1] - bx0 + 1, area = bw * (-box[3] - by0 + 1);
   const bool in_lds = 5 * area <= kAxiTile;  // uniform
   if (in_lds) {
-    for (int q = threadIdx.x; q < 5 * area; q += blockDim.x) tile[q] = 0.f;
+    for (int q = threadIdx.x; q < 5 * area; q += blockDim.x) tile[q] = 0ull;
     __syncthreads();
   }
   float* px = a.acc + (size_t)m * 3 * a.G;
   float* psr = a.hoop + (size_t)m * 2 * a.G;
   if (act) {
+    // |w| <= 1 and |d| <= 1.5 dx bound every contribution by this sum, which a NaN or
+    // infinity anywhere turns into NaN or inf, failing the comparison (no fmaxf: it drops
+    // NaN operands)
+    const float aff = fabsf(o.affine.a00) + fabsf(o.affine.a01) + fabsf(o.affine.a10) +
+                      fabsf(o.affine.a11);
+    const float bnd = fabsf(mr) * (fabsf(v.x) + fabsf(v.y)) + aff * (1.5f * k.dx) +
+                      o.mass * (fabsf(x.x) + fabsf(o.s00) + fabsf(khoop));
+    const bool to_lds = in_lds && bnd < 1.0f;
     float wx[3], wy[3];
     bspline(fx0, wx);
     bspline(fx1, wy);
@@ -214,10 +227,11 @@ __global__ void __launch_bounds__(256) k_axisym_p2g(PlainArgs a) {
         const float c[5] = {w * (mr * v.x + (o.affine.a00 * d0 + o.affine.a01 * d1)),
                             w * (mr * v.y + (o.affine.a10 * d0 + o.affine.a11 * d1)),
                             wm * x.x, wm * o.s00, wm * khoop};
-        if (in_lds) {
+        if (to_lds) {
           const int q = (b1 + jj - by0) * bw + (b0 + ii - bx0);
 #pragma unroll
-          for (int pl = 0; pl < 5; ++pl) atomicAdd(&tile[pl * area + q], c[pl]);
+          for (int pl = 0; pl < 5; ++pl)
+            atomicAdd(&tile[pl * area + q], fixed_bits(c[pl] * kFixPre));
         } else {
           const int node = (b1 + jj) * k.nx + (b0 + ii);
 #pragma unroll
@@ -234,7 +248,7 @@ __global__ void __launch_bounds__(256) k_axisym_p2g(PlainArgs a) {
     const int node = (by0 + q / bw) * k.nx + bx0 + q % bw;
 #pragma unroll
     for (int pl = 0; pl < 5; ++pl) {
-      const float val = tile[pl * area + q];
+      const float val = from_fixed_bits(tile[pl * area + q]);
       if (val != 0.f) {
         float* dst = pl < 3 ? px + pl * a.G : psr + (pl - 3) * a.G;
         unsafeAtomicAdd(dst + node, val);

@@ -1,12 +1,25 @@
-// Cross-lane helpers of the P2G scatters (fused engine and differentiable forward): groups
-// of same-cell lanes that add their contributions before the LDS atomics, and the 64-bit
-// fixed-point encoding of those atomics. Device code for gfx950 (wave64, DPP rows of 16).
+// Cross-lane helpers of the LDS scatters (fused engine, differentiable forward, renderer):
+// wave reductions, groups of same-cell lanes that add their contributions before the LDS
+// atomics, and the 64-bit fixed-point encoding of those atomics. Device code for gfx950
+// (wave64, DPP rows of 16).
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
 namespace jm {
+
+// Minimum over the 64 lanes (all active), uniform in the wave: DPP row_shr by 1, 2, 4, 8
+// leaves each row's minimum in its last lane (a lane with no source keeps its own value),
+// then the four row results combine through readlane.
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));  // row_shr:1
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));  // row_shr:2
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));  // row_shr:4
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));  // row_shr:8
+  return min(min(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+             min(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+}
 
 // Lanes of one DPP row whose particles share a base cell form runs (cell-sorted particles
 // sit in adjacent lanes; a lane that scatters nothing through LDS carries a unique key).
@@ -76,6 +89,15 @@ __device__ __forceinline__ unsigned long long fixed_bits(float y) {
 
 __device__ __forceinline__ float from_fixed_bits(unsigned long long v) {
   return (float)((double)(long long)v * kFixInv);
+}
+
+// The same encoding with 40 fractional bits, for sums whose terms reach ~2^11 (frame
+// pixels): every pixel sum fits int64 with room to spare, at 2^-40 resolution.
+constexpr float kFix40Pre = 1099511627776.0f;          // 2^40
+constexpr double kFix40Inv = 1.0 / 1099511627776.0;    // 2^-40
+
+__device__ __forceinline__ float from_fixed40_bits(unsigned long long v) {
+  return (float)((double)(long long)v * kFix40Inv);
 }
 
 }  // namespace jm

@@ -113,19 +113,23 @@ __device__ __forceinline__ bool abyss_colour(const RenderArgs& a, int m, size_t 
   return true;
 }
 
-// LDS floats of a workgroup's splat tile: up to 1365 pixels x 3 channels (16 KB).
+// LDS accumulators of a workgroup's splat tile: up to 1365 pixels x 3 channels, 64-bit
+// fixed point (32 KB).
 constexpr int kSplatTile = 4096;
 
 // render_frame_abyss (mpm_sim.py:385-477): one thread per particle slot (blockIdx.y =
 // instance); colour and intensity by material, then an additive splat with a (1 - d^2)^4
 // falloff over a disc of radius 1.5 * radius. The particles of a workgroup are consecutive
 // in the engine's cell-sorted order, so their discs cover a small pixel box: when the box
-// fits kSplatTile, the workgroup adds its splats into an LDS copy of the box and then adds
-// the box's non-zero pixels to the frame with one global atomic per channel, instead of one
-// per disc pixel (fp32 sums in another order, as with the reference's atomics).
+// fits kSplatTile, the workgroup adds its splats into an LDS copy of the box, as 64-bit
+// fixed point with 40 fractional bits (integer LDS atomics run at full rate on gfx950; fp32
+// ones took 60% longer over a frame), and then adds the box's non-zero pixels to the frame
+// with one fp32 global atomic per channel, instead of one per disc pixel (sums in another
+// order than the reference's atomics, rounded to 2^-40 in LDS). The box bounds are wave
+// minima combined by one LDS atomic per wave.
 __global__ void __launch_bounds__(256) k_render_abyss(RenderArgs a) {
-  __shared__ float tile[kSplatTile];
-  __shared__ int box[4];  // min px, max px, min py, max py of the workgroup's discs
+  __shared__ unsigned long long tile[kSplatTile];
+  __shared__ int box[4];  // min px, -max px, min py, -max py of the workgroup's discs
   const int m = blockIdx.y;
   const int local = blockIdx.x * blockDim.x + threadIdx.x;
   float2 pos = make_float2(0.0f, 0.0f);
@@ -145,41 +149,44 @@ __global__ void __launch_bounds__(256) k_render_abyss(RenderArgs a) {
     hy = min((int)(c.y + draw_r), a.res - 1);
     draw = lx <= hx && ly <= hy;
   }
-  if (threadIdx.x == 0) {
-    box[0] = INT_MAX;
-    box[1] = -1;
-    box[2] = INT_MAX;
-    box[3] = -1;
-  }
+  if (threadIdx.x < 4) box[threadIdx.x] = INT_MAX;
   __syncthreads();
-  if (draw) {
-    atomicMin(&box[0], lx);
-    atomicMax(&box[1], hx);
-    atomicMin(&box[2], ly);
-    atomicMax(&box[3], hy);
+  {
+    const int w0 = wave_min(draw ? lx : INT_MAX), w1 = wave_min(draw ? -hx : INT_MAX);
+    const int w2 = wave_min(draw ? ly : INT_MAX), w3 = wave_min(draw ? -hy : INT_MAX);
+    if ((threadIdx.x & 63) == 0 && w0 != INT_MAX) {
+      atomicMin(&box[0], w0);
+      atomicMin(&box[1], w1);
+      atomicMin(&box[2], w2);
+      atomicMin(&box[3], w3);
+    }
   }
   __syncthreads();
   const int bx0 = box[0], by0 = box[2];
-  if (bx0 > box[1]) return;  // no disc in this workgroup (uniform)
-  const int bw = box[1] - bx0 + 1, bh = box[3] - by0 + 1;
+  if (bx0 == INT_MAX) return;  // no disc in this workgroup (uniform)
+  const int bw = -box[1] - bx0 + 1, bh = -box[3] - by0 + 1;
   const bool in_lds = (int64_t)bw * bh * 3 <= kSplatTile;  // uniform
   if (in_lds) {
-    for (int q = threadIdx.x; q < bw * bh * 3; q += blockDim.x) tile[q] = 0.0f;
+    for (int q = threadIdx.x; q < bw * bh * 3; q += blockDim.x) tile[q] = 0ull;
     __syncthreads();
   }
   if (draw) {
     const float r2 = draw_r * draw_r;
+    // the fixed-point tile takes contributions below 2^11 (falloff <= 1); a larger or
+    // non-finite colour (a diverged particle) adds to the frame directly (NaN fails the
+    // comparison)
+    const bool to_lds = in_lds && fabsf(ci.x) + fabsf(ci.y) + fabsf(ci.z) < 2048.0f;
     for (int px = lx; px <= hx; ++px)
       for (int py = ly; py <= hy; ++py) {
         const float dxp = (float)px - c.x, dyp = (float)py - c.y;
         const float dn = (dxp * dxp + dyp * dyp) / r2;
         if (dn < 1.0f) {
           const float u = 1.0f - dn, u2 = u * u, falloff = u2 * u2;
-          if (in_lds) {
-            float* t = tile + ((py - by0) * bw + (px - bx0)) * 3;
-            atomicAdd(t + 0, ci.x * falloff);
-            atomicAdd(t + 1, ci.y * falloff);
-            atomicAdd(t + 2, ci.z * falloff);
+          if (to_lds) {
+            unsigned long long* t = tile + ((py - by0) * bw + (px - bx0)) * 3;
+            atomicAdd(t + 0, fixed_bits(ci.x * falloff * kFix40Pre));
+            atomicAdd(t + 1, fixed_bits(ci.y * falloff * kFix40Pre));
+            atomicAdd(t + 2, fixed_bits(ci.z * falloff * kFix40Pre));
           } else {
             float* f = a.fb + ((size_t)py * a.res + px) * 3;
             unsafeAtomicAdd(f + 0, ci.x * falloff);
@@ -196,7 +203,7 @@ __global__ void __launch_bounds__(256) k_render_abyss(RenderArgs a) {
     float* f = a.fb + ((size_t)py * a.res + px) * 3;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-      const float val = tile[3 * q + ch];
+      const float val = from_fixed40_bits(tile[3 * q + ch]);
       if (val != 0.0f) unsafeAtomicAdd(f + ch, val);
     }
   }

@@ -13,6 +13,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
 
 
+def pytest_collection_finish(session):
+    """Initialise torch's device runtime before any test runs when GPU tests are selected,
+    so that it comes up first in the process, whatever order the tests run in (the engine
+    library brings up the HIP runtime it links when its first handle is created)."""
+    if any(item.get_closest_marker("gpu") for item in session.items):
+        gpu_available()
+
+
 @pytest.fixture(scope="session")
 def golden():
     return np.load(os.path.join(ROOT, "tests", "golden", "phenotype_golden.npz"))
