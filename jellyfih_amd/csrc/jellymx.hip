@@ -233,6 +233,14 @@ inline unsigned blocks_for(int64_t n, int threads) {
   return (unsigned)std::max<int64_t>(1, (n + threads - 1) / threads);
 }
 
+// Largest live count over the instances (after commit): the per-particle launches cover
+// [0, max_live) of every instance's slots.
+int max_live(const jm_handle* h) {
+  int n = 0;
+  for (int m = 0; m < h->M; ++m) n = std::max(n, h->n_live[m]);
+  return n;
+}
+
 jm::InstParams inst_params(const jm_handle* h) {
   const int M = h->M;
   return {h->params, h->params + M, h->params + 2 * M, h->params + 3 * M, h->params + 4 * M,
@@ -667,9 +675,7 @@ int run_plain(jm_handle* h, int64_t steps) {
   const int M = h->M, G = h->G;
   if (h->hoop) {
     jm::PlainArgs a = plain_args(h, 0);
-    int max_live = 0;
-    for (int m = 0; m < M; ++m) max_live = std::max(max_live, h->n_live[m]);
-    const dim3 pg(blocks_for(max_live, 256), M), gg(blocks_for(G, 256), M);
+    const dim3 pg(blocks_for(max_live(h), 256), M), gg(blocks_for(G, 256), M);
     for (int64_t s = 0; s < steps; ++s) {
       hipLaunchKernelGGL(jm::k_fill_f32, dim3(blocks_for((int64_t)3 * M * G, 256 * 4)),
                          dim3(256), 0, h->stream, grid_buf(h, 0), (int64_t)3 * M * G, 0.f);
@@ -1358,8 +1364,8 @@ int jm_render_abyss(jm_handle* h, int32_t res_sub, int32_t grid_side, float radi
   }
   jm::RenderArgs a = render_args(h, res_sub, grid_side, radius);
   a.water_mode = water_mode;
-  hipLaunchKernelGGL(jm::k_render_abyss, dim3(blocks_for(h->cap, 256), h->M), dim3(256), 0,
-                     h->stream, a);
+  hipLaunchKernelGGL(jm::k_render_abyss, dim3(blocks_for(max_live(h), 256), h->M), dim3(256),
+                     0, h->stream, a);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
@@ -1372,8 +1378,8 @@ int jm_render_flat(jm_handle* h, int32_t res_sub, int32_t grid_side, float radiu
   jm::RenderArgs a = render_args(h, res_sub, grid_side, radius);
   a.mat_sel = material;
   a.flat = make_float3(r, g, b);
-  hipLaunchKernelGGL(jm::k_render_flat, dim3(blocks_for(h->cap, 256), h->M), dim3(256), 0,
-                     h->stream, a);
+  hipLaunchKernelGGL(jm::k_render_flat, dim3(blocks_for(max_live(h), 256), h->M), dim3(256),
+                     0, h->stream, a);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
@@ -1382,7 +1388,7 @@ int jm_render_tone_map(jm_handle* h) {
   JM_TRY(check_handle(h));
   if (!h->frame) return fail(JM_ERR_STATE, "no frame buffer: call jm_render_clear first");
   const int64_t n = (int64_t)h->frame_res * h->frame_res * 3;
-  hipLaunchKernelGGL(jm::k_tone_map, dim3(blocks_for(n, 256 * 8)), dim3(256), 0, h->stream,
+  hipLaunchKernelGGL(jm::k_tone_map, dim3(blocks_for(n, 256 * 4)), dim3(256), 0, h->stream,
                      h->frame, n);
   HIP_TRY(hipGetLastError());
   return JM_OK;

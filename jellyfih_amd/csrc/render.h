@@ -122,8 +122,8 @@ constexpr int kSplatTile = 4096;
 // falloff over a disc of radius 1.5 * radius. The particles of a workgroup are consecutive
 // in the engine's cell-sorted order, so their discs cover a small pixel box: when the box
 // fits kSplatTile, the workgroup adds its splats into an LDS copy of the box, as 64-bit
-// fixed point with 40 fractional bits (integer LDS atomics run at full rate on gfx950; fp32
-// ones took 60% longer over a frame), and then adds the box's non-zero pixels to the frame
+// fixed point with 40 fractional bits (integer LDS atomics run at full rate on gfx950; with
+// fp32 ones the kernel took 3.5x as long), and then adds the box's non-zero pixels to the frame
 // with one fp32 global atomic per channel, instead of one per disc pixel (sums in another
 // order than the reference's atomics, rounded to 2^-40 in LDS). The box bounds are wave
 // minima combined by one LDS atomic per wave.
@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(256) k_render_flat(RenderArgs a) {
 
 // tone_map_and_encode (mpm_sim.py:367-383): Reinhard with exposure 2.5, a 0.15 knee, then
 // gamma 1/2.2, per value in place; four values per thread and load (the frame buffer is a
-// hipMalloc allocation, so 16-byte aligned), the n % 4 tail one at a time.
+// hipMalloc allocation, so 16-byte aligned; launched with one float4 per thread), the
+// n % 4 tail one at a time.
 __device__ __forceinline__ float tone_map(float x) {
   const float e = x * 2.5f;
   float mapped = e / (e + 1.0f);
