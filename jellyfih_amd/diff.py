@@ -176,6 +176,7 @@ class DiffTank:
             return
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.jm_diff_destroy(self._h)
+            self._h.value = None   # views share this object: their calls now fail cleanly
         self._h = None
 
     # -- re-sorting between blocks ----------------------------------------------------------
