@@ -56,9 +56,20 @@ def main(out_path: str) -> int:
     for mat_id, r, g, b in sim.WEB_PALETTE:
         sim.render_flat_pass(sim.res_sub, sim.grid_side, radius, mat_id, r, g, b)
     web = sim.frame_buffer.to_numpy()
+    # vorticity (helpers/view_random.py:116 overlay; RdBu composite after the tone map)
+    grid_v, grid_m = sim.grid_v.to_numpy(), sim.grid_m.to_numpy()
+    sim.compute_vorticity_grid(0.1)
+    vort = sim.vort_grid.to_numpy()
+    sim.clear_frame_buffer()
+    sim.render_frame_abyss(sim.res_sub, sim.grid_side, radius)
+    sim.tone_map_and_encode()
+    base = sim.frame_buffer.to_numpy()
+    sim.render_vorticity_rdbu(sim.res_sub, sim.grid_side, 0.6)
+    rdbu = sim.frame_buffer.to_numpy()
     print("[driver] frames done", flush=True)
     np.savez(out_path, results=results, muscle_counts=np.array(muscle_counts),
              genomes=np.array(genomes), hdr=hdr, ldr=ldr, web=web, t=t, radius=radius,
+             grid_v=grid_v, grid_m=grid_m, vort=vort, base=base, rdbu=rdbu,
              hue=sim.instance_hue.to_numpy(), muscle_hue=sim.instance_muscle_hue.to_numpy(),
              freq=sim.instance_freq.to_numpy(), contraction=sim.instance_act_contraction.to_numpy(),
              res_sub=sim.res_sub, grid_side=sim.grid_side, **state)

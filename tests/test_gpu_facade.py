@@ -6,7 +6,9 @@ environment), checked against the oracle on the same population:
   headless test), fitness (evolve.py:90-124) from both within 1e-4;
 * the frames the facade read back: the abyss HDR frame against the renderer oracle on the
   state the facade read back (splat tolerance of tests/test_gpu_render.py), the tone-mapped
-  frame against the oracle's tone map of that HDR frame (8 ulps), the web palette bit-exact.
+  frame against the oracle's tone map of that HDR frame (8 ulps), the web palette bit-exact;
+  ``vort_grid`` against the oracle's vorticity of the ``grid_v``/``grid_m`` read back (1e-6)
+  and the RdBu composite against the oracle's over the same tone-mapped frame (4 ulps).
 """
 from __future__ import annotations
 
@@ -68,3 +70,12 @@ def test_evolve_style_driver(tmp_path):
         render.render_flat_pass(web, d["x"], d["material"], mat_id, (r, g, b),
                                 int(d["res_sub"]), int(d["grid_side"]), float(d["radius"]), 1.0)
     np.testing.assert_array_equal(d["web"], web)
+    vort = render.compute_vorticity_grid(d["grid_v"], d["grid_m"], 1 / 128, 0.1)
+    assert np.abs(vort).max() > 0.02
+    assert np.abs(d["vort"] - vort).max() <= 1e-6
+    rdbu = render.render_vorticity_pixels(d["base"].copy(), d["vort"], "rdbu", 0.6,
+                                          int(d["res_sub"]), int(d["grid_side"]), 1.0, M)
+    assert (rdbu != d["base"]).any()
+    ulps = np.abs(d["rdbu"].astype(np.float64) - rdbu) / np.spacing(
+        np.maximum(np.abs(rdbu), 1e-30))
+    assert ulps.max() <= 4, ulps.max()
