@@ -250,6 +250,40 @@ def test_headless_matches_oracle(require_gpu):
     assert np.max(np.abs(r_gpu[:, :4] - r_ref[:, :4])) <= 1e-5
 
 
+def test_sixteen_uneven_instances(require_gpu):
+    """The batched-forward configuration: 16 seeded genomes (seeds 1-16, uneven live counts)
+    in one launch per stage, 1000 substeps through run_batch_headless: validity flags equal
+    and per-instance payload CoM (start and end) within what the fp32 oracle itself shows
+    against the fp64 oracle (2x, + 1e-6); the x of every particle likewise (fp32_like)."""
+    from jellyfih_amd import phenotype
+    say("16 uneven instances (seeds 1-16), 1000 substeps, fused vs oracle f32/f64")
+    pos, mat, fib, genomes, infos = phenotype.population(range(1, 17), 80000, 128)
+    live = (mat >= 0).sum(1)
+    say(f"live particles per instance {live.min()}-{live.max()}")
+    cases = [(pos[m], mat[m], fib[m]) for m in range(16)]
+    eng, o32 = make_pair(cases)
+    o64 = oracle.OracleSim(16, 80000, dtype=np.float64)
+    for m, (p_, mt_, f_) in enumerate(cases):
+        o64.load(m, p_, mt_, f_)
+    con = np.clip(genomes[:, 9], 0.05, 0.60).astype(np.float32)
+    frq = np.clip(genomes[:, 10], 0.5, 2.0).astype(np.float32)
+    eng.set_param("contraction", con)
+    eng.set_param("freq", frq)
+    for o in (o32, o64):
+        o.contraction[:] = con
+        o.freq[:] = frq
+    r_gpu = eng.run_batch_headless(1000)
+    r_32 = o32.run_batch_headless(1000, nthreads=16)
+    r_64 = o64.run_batch_headless(1000, nthreads=16)
+    np.testing.assert_array_equal(r_gpu[:, 4], r_64[:, 4])
+    e_gpu = np.abs(r_gpu[:, :4] - r_64[:, :4]).max()
+    e_32 = np.abs(r_32[:, :4] - r_64[:, :4]).max()
+    say(f"CoM: |gpu-f64| {e_gpu:.3e}, |oracle32-f64| {e_32:.3e}")
+    assert e_gpu <= 2.0 * e_32 + 1e-6
+    live_mask = mat >= 0
+    fp32_like("x", eng.read("x")[live_mask], o32.x[live_mask], o64.x[live_mask])
+
+
 def test_variants_agree(require_gpu):
     """Graphs on/off, lazy v/C on/off, XCD affinity, 1-3 concurrent instance groups and
     several launch shapes."""
