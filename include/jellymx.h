@@ -224,6 +224,13 @@ int jm_water_mask(const double* body, int64_t n_body, const double* cand, int64_
  * the closed polygon poly (n,2, host double) cross properly, else 0. */
 int jm_polygon_self_intersects(const double* poly, int64_t n, int32_t* out);
 
+/* make_jelly.py:143-157,209-215 (Path(poly).contains_points over the candidate meshgrid):
+ * out[j*nx + i] = 1 when (xs[i], ys[j]) is inside the polygon poly (n,2, host double, closed
+ * implicitly) by the crossing-number rule of matplotlib's point-in-path test, bit for bit;
+ * xs ascending. */
+int jm_points_in_polygon_grid(const double* poly, int64_t n, const double* xs, int64_t nx,
+                              const double* ys, int64_t ny, uint8_t* out);
+
 /* ---- Differentiable path -----------------------------------------------------------
  * No counterpart in the reference (it has no autodiff). A batched substep and its adjoint
  * over caller-owned device buffers (e.g. torch tensors), P particles in any order, in fp32

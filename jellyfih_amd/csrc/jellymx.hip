@@ -1579,6 +1579,63 @@ int jm_polygon_self_intersects(const double* poly, int64_t n, int32_t* out) {
   return JM_OK;
 }
 
+// Lattice points inside a polygon (make_jelly.py:150-157,209-215: matplotlib
+// Path(poly).contains_points over the meshgrid of xs and ys). The test is the crossing-number
+// rule in the multiply form of E. Haines ("Point in Polygon Strategies", Graphics Gems IV):
+// a point (tx, ty) toggles its inside flag on every edge a -> b (the polygon's edges in
+// order, then the closing edge) whose end points lie on opposite sides of y = ty
+// (a.y >= ty differs from b.y >= ty) and for which
+//   ((b.y - ty) * (a.x - b.x) >= (b.x - tx) * (a.y - b.y)) == (b.y >= ty),
+// each product and difference rounded in double on its own. Along one lattice row ty is
+// fixed and the right-hand side is monotone in tx (fl(b.x - tx) falls as tx grows, and the
+// product with the constant a.y - b.y is monotone), so each edge toggles a prefix or a
+// suffix of the row's sorted xs: a binary search with the same comparison finds its end,
+// and a running XOR over the row applies all edges. out[j * nx + i] is point (xs[i], ys[j]),
+// the meshgrid's ravel order.
+int jm_points_in_polygon_grid(const double* poly, int64_t n, const double* xs, int64_t nx,
+                              const double* ys, int64_t ny, uint8_t* out) {
+#pragma clang fp contract(off)
+  if (n < 0 || nx < 0 || ny < 0 || (n > 0 && !poly) || (nx > 0 && (!xs || !out)) ||
+      (ny > 0 && !ys))
+    return fail(JM_ERR_ARG, "bad arguments");
+  for (int64_t i = 1; i < nx; ++i)
+    if (!(xs[i - 1] <= xs[i])) return fail(JM_ERR_ARG, "xs must be ascending");
+  std::vector<uint8_t> flip((size_t)nx + 1);
+  for (int64_t j = 0; j < ny; ++j) {
+    const double ty = ys[j];
+    std::fill(flip.begin(), flip.end(), 0);
+    for (int64_t e = 0; e < n; ++e) {
+      const double ax = poly[2 * e], ay = poly[2 * e + 1];
+      const double bx = poly[2 * ((e + 1) % n)], by = poly[2 * ((e + 1) % n) + 1];
+      const bool f0 = ay >= ty, f1 = by >= ty;
+      if (f0 == f1) continue;
+      const double lhs = (by - ty) * (ax - bx), c = ay - by;
+      auto below = [&](double tx) { return lhs >= (bx - tx) * c; };
+      // first index whose comparison differs from the row's first entry's (monotone)
+      const bool first = below(xs[0]);
+      int64_t lo = 0, hi = nx;
+      while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (below(xs[mid]) == first) lo = mid + 1; else hi = mid;
+      }
+      // indices [0, lo) compare as `first`, [lo, nx) as !first; toggle those equal to f1
+      if (first == f1) {
+        flip[0] ^= 1;
+        flip[(size_t)lo] ^= 1;
+      } else {
+        flip[(size_t)lo] ^= 1;
+        flip[(size_t)nx] ^= 1;
+      }
+    }
+    uint8_t in = 0;
+    for (int64_t i = 0; i < nx; ++i) {
+      in ^= flip[(size_t)i];
+      out[(size_t)j * nx + i] = in;
+    }
+  }
+  return JM_OK;
+}
+
 // -----------------------------------------------------------------------------------------
 // Differentiable path (no reference counterpart: the reference has no autodiff).
 

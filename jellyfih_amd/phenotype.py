@@ -18,7 +18,6 @@ exclusion zone around the robot, padded with dead slots.
 from __future__ import annotations
 
 import numpy as np
-from matplotlib.path import Path
 from scipy.spatial import cKDTree
 
 PAYLOAD_WIDTH = 0.08                    # make_jelly.py:8
@@ -53,6 +52,21 @@ def polygon_self_intersects(poly: np.ndarray) -> bool:
     out = np.zeros(1, dtype=np.int32)
     _lib.call("jm_polygon_self_intersects", poly.ctypes.data, len(poly), out.ctypes.data)
     return bool(out[0])
+
+
+def points_in_polygon_grid(poly, xs, ys) -> np.ndarray:
+    """Inside flags of the meshgrid of ascending ``xs`` and ``ys`` (ravel order: rows of
+    ys), as ``Path(poly).contains_points`` decides them (make_jelly.py:150-157,209-215),
+    computed row by row by jm_points_in_polygon_grid in libjellymx.so."""
+    from . import _lib
+    _lib.load()
+    poly = np.ascontiguousarray(poly, dtype=np.float64).reshape(-1, 2)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    out = np.empty(len(xs) * len(ys), dtype=np.uint8)
+    _lib.call("jm_points_in_polygon_grid", poly.ctypes.data, len(poly), xs.ctypes.data,
+              len(xs), ys.ctypes.data, len(ys), out.ctypes.data)
+    return out.astype(bool)
 
 
 def _unit_normals(points: np.ndarray) -> np.ndarray:
@@ -110,8 +124,8 @@ def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payl
     gx, gy = np.meshgrid(xs, ys)
     cand = np.vstack([gx.ravel(), gy.ravel()]).T
 
-    in_bell = Path(body_poly).contains_points(cand)
-    in_muscle = Path(muscle_poly).contains_points(cand)
+    in_bell = points_in_polygon_grid(body_poly, xs, ys)
+    in_muscle = points_in_polygon_grid(muscle_poly, xs, ys)
     payload_box = (cand[:, 0] <= PAYLOAD_WIDTH / 2.0) & (cand[:, 1] <= PAYLOAD_HEIGHT)
 
     # Collar around the payload socket (make_jelly.py:164-223)
@@ -135,8 +149,8 @@ def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payl
         collar_poly = np.vstack([collar_out, collar_in[::-1]])
         collar_mus = collar_in + muscle_ratio * (collar_out - collar_in)
         collar_mus_poly = np.vstack([collar_mus, collar_in[::-1]])
-        in_collar = Path(collar_poly).contains_points(cand) & ~payload_box
-        in_collar_mus = Path(collar_mus_poly).contains_points(cand) & ~payload_box
+        in_collar = points_in_polygon_grid(collar_poly, xs, ys) & ~payload_box
+        in_collar_mus = points_in_polygon_grid(collar_mus_poly, xs, ys) & ~payload_box
         bridge_hw = max(outer0[0], PAYLOAD_WIDTH / 2.0 + collar_thick)
     else:
         in_collar = np.zeros(len(cand), dtype=bool)

@@ -111,3 +111,30 @@ def test_polygon_self_intersects_matches_python():
     square = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
     bow = np.array([[0.0, 0.0], [1.0, 1.0], [1.0, 0.0], [0.0, 1.0]])
     assert not polygon_self_intersects(square) and polygon_self_intersects(bow)
+
+
+def test_points_in_polygon_grid_matches_matplotlib():
+    """The native lattice fill against matplotlib's Path.contains_points (the reference's
+    call, make_jelly.py:150-157,209-215) bit for bit: star-shaped and self-intersecting
+    polygons, vertices and edges on lattice lines, lattices of 1 to 90 points per side."""
+    from matplotlib.path import Path
+    rng = np.random.default_rng(0)
+    total = 0
+    for trial in range(200):
+        n = int(rng.integers(3, 60))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        r = rng.uniform(0.2, 1, n)
+        poly = np.stack([r * np.cos(ang), r * np.sin(ang)], 1)
+        if trial % 3 == 0:
+            poly = rng.uniform(-1, 1, (n, 2))
+        if trial % 5 == 0:
+            poly = np.round(poly * 8) / 8
+        xs = np.linspace(-1.1, 1.1, int(rng.integers(1, 90)))
+        ys = np.linspace(-1.1, 1.1, int(rng.integers(1, 90)))
+        if trial % 7 == 0:
+            xs, ys = np.unique(np.round(xs * 8) / 8), np.unique(np.round(ys * 8) / 8)
+        gx, gy = np.meshgrid(xs, ys)
+        ref = Path(poly).contains_points(np.vstack([gx.ravel(), gy.ravel()]).T)
+        np.testing.assert_array_equal(phenotype.points_in_polygon_grid(poly, xs, ys), ref)
+        total += ref.size
+    assert total > 100000
