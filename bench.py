@@ -23,7 +23,9 @@ Also reported on the same line:
 * render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
   map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back;
 * axisym: the axisymmetric variant's rate on the same batch and on the tall view
-  configuration (windows of 8 substeps from fresh loads).
+  configuration (windows of 8 substeps from fresh loads);
+* tank_build_s: host seconds to build the 32 tanks (phenotype.population, the fill_tank
+  restatement with its native steps).
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -376,7 +378,9 @@ def main() -> int:
         return 0
     M, N = args.instances, args.slots
     log(f"building workload: {M} genomes x {N} slots")
+    t_build = time.perf_counter()
     pos, mat, fib, genomes, infos = build_workload(M, N, range(M))
+    t_build = time.perf_counter() - t_build
     live = int((mat >= 0).sum())
     log(f"live particles {live} (slots {M * N})")
 
@@ -506,6 +510,7 @@ def main() -> int:
         "diff": diff,
         "render": rend,
         "axisym": axisym,
+        "tank_build_s": t_build,
     }
     print(json.dumps(out), flush=True)
     return 0
