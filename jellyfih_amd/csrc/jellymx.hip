@@ -115,6 +115,7 @@ struct jm_handle {
   int acc_base = 0;
   int last_read = -1;        // grid buffer the last G2P gathered from (-1: none yet)
   bool grid_is_velocity = false;  // plain path leaves velocity in the planes
+  bool tuning_user = false;  // jm_set_tuning was called: keep its launch shape
   int64_t since_sort = 0;
 
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
@@ -276,6 +277,16 @@ int upload_params(jm_handle* h) {
 // instances are spread over XCDs longest-first to balance the chunk counts.
 int build_chunks(jm_handle* h) {
   const int M = h->M;
+  if (!h->tuning_user) {
+    // Default launch shape by batch size: with fewer than 512 chunks of 1024 live particles
+    // (two per CU), 256 x 2 chunks give the small batch twice the workgroups (one tank:
+    // 16.1 -> 10.8 us per substep; four: 17.2 -> 12.8 us); larger batches keep 256 x 4
+    // (the autotuner's choice at 32 x 80K).
+    int64_t chunks = 0;
+    for (int m = 0; m < M; ++m) chunks += (h->n_live[m] + 1023) / 1024;
+    h->tun.threads = 256;
+    h->tun.ppt = chunks < 512 ? 2 : 4;
+  }
   const int cs = h->tun.threads * h->tun.ppt;
   h->chunk_size = cs;
   std::vector<int> nch(M);
@@ -916,6 +927,7 @@ int jm_set_tuning(jm_handle* h, const jm_tuning* t) {
                         t->instance_groups != h->tun.instance_groups;
   if (t->sort_strip != h->tun.sort_strip) h->since_sort = t->sort_interval;  // re-sort next
   h->tun = *t;
+  h->tuning_user = true;
   h->key_bits = sort_key_bits(h->nx, h->ny, t->sort_strip);
   h->drop_graphs();
   if (relayout && !h->layout_dirty) JM_TRY(build_chunks(h));

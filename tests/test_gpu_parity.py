@@ -292,6 +292,7 @@ def test_variants_agree(require_gpu):
     base = None
     variants = [dict(), dict(use_graphs=0), dict(lazy_vc=0), dict(xcd_affinity=0),
                 dict(threads=256, ppt=4), dict(threads=512, ppt=8), dict(threads=128, ppt=16),
+                dict(threads=256, ppt=2), dict(threads=512, ppt=2),
                 dict(sort_interval=7, block_substeps=5), dict(instance_groups=1),
                 dict(instance_groups=3, use_graphs=0), dict(instance_groups=4)]
     for var in variants:
