@@ -71,8 +71,26 @@ res_sub = video_res // grid_side
 _engine = None
 
 
+def _tuning_for(instances: int, slots: int, path: str | None = None):
+    """The autotuner's launch parameters (``tuning.json`` beside this module, written by
+    ``python -m jellyfih_amd.autotune --write``) when they were measured on this batch
+    shape, else None (the engine then picks its launch shape by batch size)."""
+    import json
+    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = d.get("workload", {})
+    if w.get("instances") != instances or w.get("slots") != slots:
+        return None
+    return dict(d.get("tuning", {})) or None
+
+
 def engine():
-    """The device engine behind this module (created on first use)."""
+    """The device engine behind this module (created on first use), with the autotuner's
+    launch parameters when they were measured on this batch shape."""
     global _engine
     if _engine is None:
         from .engine import Engine
@@ -81,6 +99,10 @@ def engine():
                          n_grid_y=n_grid_y, domain_height=domain_height,
                          track_glow=os.environ.get("JELLY_GLOW", "0") == "1",
                          path="plain" if axisym else "fused", axisym=axisym)
+        tuned = None if (axisym or n_grid_y != n_grid) else _tuning_for(n_instances,
+                                                                         n_particles)
+        if tuned:
+            _engine.set_tuning(**{k: v for k, v in tuned.items() if k in _engine.tuning()})
     return _engine
 
 

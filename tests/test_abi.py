@@ -125,3 +125,11 @@ def test_facade_constants_match_reference():
                  "render_vorticity_overlay", "render_vorticity_rdbu",
                  "render_vorticity_hueshift", "render_vorticity_white"):
         assert callable(getattr(s, name))
+
+
+def test_facade_uses_the_autotuned_parameters_for_their_batch_shape(tmp_path):
+    from jellyfih_amd import mpm_sim as s
+    t = s._tuning_for(32, 80000)
+    assert t is not None and t["threads"] * t["ppt"] > 0 and t["sort_interval"] >= 1
+    assert s._tuning_for(16, 80000) is None and s._tuning_for(32, 160000) is None
+    assert s._tuning_for(32, 80000, path=str(tmp_path / "missing.json")) is None
