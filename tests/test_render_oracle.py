@@ -337,3 +337,48 @@ def test_vorticity_overlay_single_node():
     want = np.zeros((64, 64, 3))
     want[cy - 2:cy + 3, cx - 2:cx + 3] = np.array([1.0, 0.4, 0.0]) * vi * 0.5
     np.testing.assert_allclose(fb, want, rtol=1e-6, atol=1e-8)
+
+
+def _literal_overlay(fb, gv, gm, dx, scale, res_sub, grid_side, H):
+    """mpm_sim.py:537-558 as written: per node, fp32 scalars."""
+    res = fb.shape[0]
+    M, nx, ny, _ = gv.shape
+    for m in range(M):
+        for i in range(nx):
+            for j in range(ny):
+                if i < 1 or i >= nx - 1 or j < 1 or j >= ny - 1 or not gm[m, i, j] > 0:
+                    continue
+                dvx_dy = (gv[m, i, j + 1, 0] - gv[m, i, j - 1, 0]) / (f32(2.0) * f32(dx))
+                dvy_dx = (gv[m, i + 1, j, 1] - gv[m, i - 1, j, 1]) / (f32(2.0) * f32(dx))
+                curl = f32(dvx_dy - dvy_dx)
+                vi = min(abs(curl) * f32(scale), f32(1.0))
+                if vi < 0.01:
+                    continue
+                col = (np.array([1.0, 0.4, 0.0], f32) if curl > 0
+                       else np.array([0.0, 0.4, 1.0], f32)) * f32(vi)
+                row, c = m // grid_side, m % grid_side
+                cx = int((f32(i) / f32(nx) / f32(H) + f32(c)) * f32(res_sub))
+                cy = int(((f32(H) - f32(j) / f32(nx)) / f32(H) + f32(row)) * f32(res_sub))
+                cpx = max(1, res_sub // nx)
+                for px in range(cx - cpx // 2, cx + cpx // 2 + 1):
+                    for py in range(cy - cpx // 2, cy + cpx // 2 + 1):
+                        if 0 <= px < res and 0 <= py < res:
+                            fb[py, px] += col * f32(0.5)
+    return fb
+
+
+@pytest.mark.parametrize("H", [1.0, 2.0])
+def test_vorticity_overlay_matches_literal_loops(H):
+    rng = np.random.default_rng(5)
+    M, nx = 3, 12
+    ny = int(nx * H)
+    gv = rng.normal(0, 0.05, (M, nx, ny, 2)).astype(f32)
+    gm = (rng.uniform(size=(M, nx, ny)) < 0.7).astype(f32)
+    res, grid_side = 72, 2
+    res_sub = res // grid_side
+    got = render.render_vorticity_overlay(np.zeros((res, res, 3), f32), gv, gm, 1 / 128, 0.02,
+                                          res_sub, grid_side, H)
+    want = _literal_overlay(np.zeros((res, res, 3), f32), gv, gm, 1 / 128, 0.02, res_sub,
+                            grid_side, H)
+    assert want.any()
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7)
