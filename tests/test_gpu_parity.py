@@ -284,6 +284,51 @@ def test_sixteen_uneven_instances(require_gpu):
     fp32_like("x", eng.read("x")[live_mask], o32.x[live_mask], o64.x[live_mask])
 
 
+def test_generation_batch_with_bench_tuning(require_gpu):
+    """The measured configuration itself: the 32 seeded genomes x 80K slots of the
+    throughput benchmark, with the autotuned launch parameters it runs (tuning.json), 600
+    substeps through run_batch_headless and the fitness of evolve.py:90-124: validity flags
+    equal, payload CoM and x within what the fp32 oracle shows against the fp64 oracle
+    (2x, + 1e-6), fitness within 1e-4 of the fp64 oracle's."""
+    import json
+    import os
+    from jellyfih_amd import phenotype
+    say("32-genome generation batch with the bench tuning, 600 substeps")
+    pos, mat, fib, genomes, infos = phenotype.population(range(32), 80000, 128)
+    cases = [(pos[m], mat[m], fib[m]) for m in range(32)]
+    eng, o32 = make_pair(cases)
+    with open(os.path.join(os.path.dirname(oracle.__file__), "..", "jellyfih_amd",
+                           "tuning.json")) as f:
+        tuning = json.load(f)["tuning"]
+    eng.set_tuning(**{k: v for k, v in tuning.items() if k in eng.tuning()})
+    say(f"tuning {eng.tuning()}")
+    o64 = oracle.OracleSim(32, 80000, dtype=np.float64)
+    for m, (p_, mt_, f_) in enumerate(cases):
+        o64.load(m, p_, mt_, f_)
+    con = np.clip(genomes[:, 9], 0.05, 0.60).astype(np.float32)
+    frq = np.clip(genomes[:, 10], 0.5, 2.0).astype(np.float32)
+    eng.set_param("contraction", con)
+    eng.set_param("freq", frq)
+    for o in (o32, o64):
+        o.contraction[:] = con
+        o.freq[:] = frq
+    r_gpu = eng.run_batch_headless(600)
+    r_32 = o32.run_batch_headless(600, nthreads=16)
+    r_64 = o64.run_batch_headless(600, nthreads=16)
+    np.testing.assert_array_equal(r_gpu[:, 4], r_64[:, 4])
+    e_gpu = np.abs(r_gpu[:, :4] - r_64[:, :4]).max()
+    e_32 = np.abs(r_32[:, :4] - r_64[:, :4]).max()
+    say(f"CoM: |gpu-f64| {e_gpu:.3e}, |oracle32-f64| {e_32:.3e}")
+    assert e_gpu <= 2.0 * e_32 + 1e-6
+    muscle = [info["muscle_count"] for info in infos]
+    f_gpu = np.array(oracle.compute_fitness(r_gpu, muscle, genomes), dtype=np.float64)
+    f_64 = np.array(oracle.compute_fitness(r_64, muscle, genomes), dtype=np.float64)
+    say(f"fitness max |gpu-f64| {np.abs(f_gpu - f_64).max():.3e}")
+    np.testing.assert_allclose(f_gpu, f_64, atol=1e-4)
+    live_mask = mat >= 0
+    fp32_like("x", eng.read("x")[live_mask], o32.x[live_mask], o64.x[live_mask])
+
+
 def test_variants_agree(require_gpu):
     """Graphs on/off, lazy v/C on/off, XCD affinity, 1-3 concurrent instance groups and
     several launch shapes."""
