@@ -277,3 +277,38 @@ def test_gradients_with_resorting_between_blocks(require_gpu):
         say(f"re-sorted vs not, {n}: {err:.2e} (repeat: {floor[n]:.2e})")
         if n != "body v":
             assert err <= max(10 * floor[n], 1e-9) and err <= 1e-6, (n, err)
+
+
+def test_batched_gradients_match_per_instance(require_gpu):
+    """The batched adjoint keeps instances apart: three tall tanks (128x256, H = 2, seeds
+    0-2) in one DiffTank give, per instance, the payload CoM and the gradients w.r.t.
+    amplitude, contraction and frequency of the same tank run alone (fp64, 200 substeps,
+    checkpoint 100), up to fp64 summation order."""
+    import torch
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.diff import DiffTank
+    say("3 tall tanks batched vs one at a time, fp64, 200 substeps")
+    pos, mat, fib, genomes, infos = phenotype.population(
+        range(3), 160000, 128, domain_height=2.0, spawn_offset=phenotype.TALL_SPAWN)
+
+    def run(P, Mt, Fb, G):
+        tank, st = DiffTank.from_tanks(P, Mt, Fb, n_grid_y=256, domain_height=2.0,
+                                       dtype=torch.float64)
+        dev = tank.device
+        vec = lambda a: torch.tensor(np.asarray(a, np.float64), device=dev).requires_grad_(True)
+        amp = vec(np.full(len(G), 500.0))
+        con, frq = vec(np.clip(G[:, 9], 0.05, 0.6)), vec(np.clip(G[:, 10], 0.5, 2.0))
+        out = tank.run(st, None, amp, con, frq, 700.0, 2.5, 0.0, 200, 100)
+        com = tank.payload_com_y(out)
+        com.sum().backward()
+        res = [t.detach().cpu().numpy() for t in (com, amp.grad, con.grad, frq.grad)]
+        tank.close()
+        return res
+
+    both = run(pos, mat, fib, genomes)
+    for m in range(3):
+        one = run(pos[m:m + 1], mat[m:m + 1], fib[m:m + 1], genomes[m:m + 1])
+        for name, a, b in zip(("com", "d/d amp", "d/d contraction", "d/d freq"), both, one):
+            err = abs(a[m] - b[0]) / max(abs(b[0]), 1e-30)
+            say(f"instance {m} {name}: batched {a[m]:.10e} alone {b[0]:.10e} rel {err:.1e}")
+            assert err <= 1e-8, (m, name)
