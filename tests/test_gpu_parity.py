@@ -329,6 +329,34 @@ def test_generation_batch_with_bench_tuning(require_gpu):
     fp32_like("x", eng.read("x")[live_mask], o32.x[live_mask], o64.x[live_mask])
 
 
+def test_quality_two_grid(require_gpu):
+    """quality = 2 (mpm_sim.py:12,14,19: a 256x256 grid, dt = 1e-5; the reference fixes
+    quality = 1, the engine takes it as a parameter): two small tanks at 4 particles per
+    cell of the finer grid, 300 substeps, fused path against the oracle in fp32 and fp64."""
+    say("quality 2 (256x256, dt 1e-5), 300 substeps, fused vs oracle f32/f64")
+    cases = [small_tank(5, 12000, region=(0.30, 0.50, 0.30, 0.45), ppc_spacing=1 / 512),
+             small_tank(6, 12000, region=(0.55, 0.75, 0.10, 0.25), ppc_spacing=1 / 512)]
+    consts = oracle.reference_consts(quality=2)
+    eng = Engine(2, 12000, quality=2)
+    o32 = oracle.OracleSim(2, 12000, consts, dtype=np.float32)
+    o64 = oracle.OracleSim(2, 12000, consts, dtype=np.float64)
+    for m, (p_, mt_, f_) in enumerate(cases):
+        eng.load_particles(m, p_, mt_, f_)
+        o32.load(m, p_, mt_, f_)
+        o64.load(m, p_, mt_, f_)
+    eng.set_time(0.05)
+    for o in (o32, o64):
+        o.t[0] = np.float32(0.05)
+    eng.substep(300)
+    o32.run(300, nthreads=2)
+    o64.run(300, nthreads=2)
+    live = o32.material >= 0
+    for n, a32, a64 in (("x", o32.x, o64.x), ("v", o32.v, o64.v), ("F", o32.F, o64.F),
+                        ("C", o32.C, o64.C)):
+        fp32_like(n, eng.read(n)[live], a32[live], a64[live])
+    assert eng.get_time() == o32.t[0]
+
+
 def test_variants_agree(require_gpu):
     """Graphs on/off, lazy v/C on/off, XCD affinity, 1-3 concurrent instance groups and
     several launch shapes."""
