@@ -357,6 +357,32 @@ def test_quality_two_grid(require_gpu):
     assert eng.get_time() == o32.t[0]
 
 
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_glow_pass(require_gpu, path):
+    """The render-only glow pass (mpm_sim.py:335-338, engine option track_glow): glow =
+    max(0.9 glow, |1 - Jp|) after every substep, on tanks with a stiff payload whose Jp
+    moves; 200 substeps against the oracle with its glow stage, in fp32 and fp64: glow and
+    Jp within what the fp32 oracle shows against the fp64 oracle (fp32_like)."""
+    say(f"glow pass path={path}, 200 substeps")
+    cases = ragged_cases(12000)[:2]
+    M, N = len(cases), cases[0][0].shape[0]
+    eng = Engine(M, N, path=path, track_glow=True)
+    o32 = oracle.OracleSim(M, N)
+    o64 = oracle.OracleSim(M, N, dtype=np.float64)
+    for m, (p_, mt_, f_) in enumerate(cases):
+        eng.load_particles(m, p_, mt_, f_)
+        o32.load(m, p_, mt_, f_)
+        o64.load(m, p_, mt_, f_)
+    perturb(eng, o32, seed=4, sv=0.05, sc=0.5, sf=0.02, orcs=(o64,))
+    eng.substep(200)
+    for o in (o32, o64):
+        o.run(200, stages=oracle.ST_SUBSTEP | oracle.ST_GLOW, nthreads=2)
+    live = o32.material >= 0
+    assert o64.glow[live].max() > 1e-3
+    fp32_like("glow", eng.read("visual_glow")[live], o32.glow[live], o64.glow[live])
+    fp32_like("Jp", eng.read("Jp")[live], o32.Jp[live], o64.Jp[live])
+
+
 def test_variants_agree(require_gpu):
     """Graphs on/off, lazy v/C on/off, XCD affinity, 1-3 concurrent instance groups and
     several launch shapes."""
