@@ -287,6 +287,14 @@ def render_frames(eng, live: int, reps: int = 20) -> dict:
     for _ in range(reps):
         eng.read_frame()
     out["read_frame_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.read_frame_u8()
+    out["read_frame_u8_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        (np.clip(eng.read_frame(), 0, 1) * 255).astype(np.uint8)
+    out["read_frame_host_encode_ms"] = (time.perf_counter() - t0) / reps * 1e3
     return out
 
 

@@ -212,6 +212,13 @@ int jm_render_vorticity(jm_handle* h, int32_t mode, int32_t res_sub, int32_t gri
 int jm_read_frame(jm_handle* h, float* out, int64_t n);
 int jm_read_vorticity(jm_handle* h, float* out, int64_t n);
 
+/* The frame buffer's 8-bit encode as every viewer caller writes it on the host,
+ * (np.clip(img, 0, 1) * 255).astype(np.uint8) (evolve.py:396-397, helpers/view_single.py:257,
+ * helpers/view_generation.py:210-211): clipped to [0, 1], multiplied by 255 in fp32,
+ * truncated; NaN encodes as 0. Encoded on the device, so res*res*3 bytes cross to the host
+ * instead of res*res*3 floats. n is the number of bytes (res*res*3). */
+int jm_read_frame_u8(jm_handle* h, uint8_t* out, int64_t n);
+
 /* ---- Tank building (host code, no GPU) -------------------------------------------------
  * Replaces the k-d tree query of make_jelly.py:359-366 (fill_tank, step 3): keep[i] = 1 when
  * every body point lies farther than `margin` from candidate i (Euclidean distance in

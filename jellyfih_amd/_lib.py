@@ -39,6 +39,7 @@ EXPORTS = (
     "jm_fwd_block", "jm_bwd_block",
     "jm_render_clear", "jm_render_abyss", "jm_render_flat", "jm_render_tone_map",
     "jm_render_vorticity_grid", "jm_render_vorticity", "jm_read_frame", "jm_read_vorticity",
+    "jm_read_frame_u8",
 )
 
 
@@ -148,6 +149,7 @@ def load() -> ctypes.CDLL:
                                 ctypes.c_float],
         "jm_read_frame": [H, f32p, ctypes.c_int64],
         "jm_read_vorticity": [H, f32p, ctypes.c_int64],
+        "jm_read_frame_u8": [H, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

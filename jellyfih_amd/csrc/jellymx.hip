@@ -86,6 +86,7 @@ struct jm_handle {
   float* damp = nullptr;      // nx + ny floats: boundary damping factors (k.damp)
   float* frame = nullptr;     // renderer: frame_res^2 x 3 floats (allocated on first use)
   int frame_res = 0;
+  unsigned char* frame_u8 = nullptr;  // its 8-bit encode (allocated on first jm_read_frame_u8)
   float* vort = nullptr;      // renderer: M * G floats, node (i, j) at j * nx + i
   float* rhue = nullptr;      // renderer: 2 * M floats, instance_hue then instance_muscle_hue
   std::vector<float> rhue_host;  // what rhue holds
@@ -131,12 +132,14 @@ struct jm_handle {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {st.base, tmp.base, sortbuf, hist, acc, t_buf, params, n_live_d,
-                    chunks_d, nchunks_d, stats_d, staging, damp, frame, vort, rhue, hoop};
+                    chunks_d, nchunks_d, stats_d, staging, damp, frame, frame_u8, vort, rhue,
+                    hoop};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     st.base = tmp.base = nullptr;
     sortbuf = hist = nullptr;
     acc = t_buf = params = stats_d = staging = damp = frame = vort = rhue = hoop = nullptr;
+    frame_u8 = nullptr;
     frame_res = 0;
     rhue_host.clear();
     n_live_d = nullptr;
@@ -1341,7 +1344,9 @@ int jm_render_clear(jm_handle* h, int32_t res, float value) {
   if (res != h->frame_res) {
     JM_TRY(jm_sync(h));
     if (h->frame) (void)hipFree(h->frame);
+    if (h->frame_u8) (void)hipFree(h->frame_u8);
     h->frame = nullptr;
+    h->frame_u8 = nullptr;
     h->frame_res = 0;
     if (hipMalloc(&h->frame, (size_t)res * res * 3 * sizeof(float)) != hipSuccess)
       return fail(JM_ERR_NOMEM, "hipMalloc(frame buffer)");
@@ -1457,6 +1462,21 @@ int jm_read_frame(jm_handle* h, float* out, int64_t n) {
   if (!h->frame) return fail(JM_ERR_STATE, "no frame buffer: call jm_render_clear first");
   if (n != (int64_t)h->frame_res * h->frame_res * 3) return fail(JM_ERR_ARG, "n != res*res*3");
   HIP_TRY(hipMemcpyAsync(out, h->frame, n * 4, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return JM_OK;
+}
+
+int jm_read_frame_u8(jm_handle* h, uint8_t* out, int64_t n) {
+  JM_TRY(check_handle(h));
+  if (!out) return fail(JM_ERR_ARG, "null out");
+  if (!h->frame) return fail(JM_ERR_STATE, "no frame buffer: call jm_render_clear first");
+  if (n != (int64_t)h->frame_res * h->frame_res * 3) return fail(JM_ERR_ARG, "n != res*res*3");
+  if (!h->frame_u8 && hipMalloc(&h->frame_u8, (size_t)n) != hipSuccess)
+    return fail(JM_ERR_NOMEM, "hipMalloc(8-bit frame)");
+  hipLaunchKernelGGL(jm::k_encode_u8, dim3(blocks_for(n, 256 * 4)), dim3(256), 0, h->stream,
+                     h->frame, h->frame_u8, n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, h->frame_u8, n, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return JM_OK;
 }

@@ -257,6 +257,30 @@ __global__ void k_tone_map(float* fb, int64_t n) {
       tone_map(fb[4 * n4 + threadIdx.x]);
 }
 
+// The callers' 8-bit encode of a frame, (np.clip(img, 0, 1) * 255).astype(np.uint8)
+// (evolve.py:397, helpers/view_single.py:257): clip, one fp32 product by 255, truncation
+// toward zero. NaN encodes as 0 (fmaxf returns the number; numpy's cast of NaN is
+// unspecified and gives 0 on x86-64).
+__device__ __forceinline__ unsigned char encode_u8(float v) {
+  const float c = fminf(fmaxf(v, 0.0f), 1.0f);
+  return (unsigned char)(unsigned)(c * 255.0f);
+}
+
+// Four values per thread: one float4 load, one 4-byte store (both buffers are hipMalloc
+// allocations); the n % 4 tail one at a time.
+__global__ void k_encode_u8(const float* fb, unsigned char* out, int64_t n) {
+  const int64_t n4 = n / 4;
+  const float4* fb4 = reinterpret_cast<const float4*>(fb);
+  uchar4* out4 = reinterpret_cast<uchar4*>(out);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 q = fb4[i];
+    out4[i] = make_uchar4(encode_u8(q.x), encode_u8(q.y), encode_u8(q.z), encode_u8(q.w));
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4)
+    out[4 * n4 + threadIdx.x] = encode_u8(fb[4 * n4 + threadIdx.x]);
+}
+
 // Curl of the post-grid-op velocity at interior node (i, j) of instance m with mass, from
 // the grid planes gp the last G2P gathered: momentum x, momentum y, mass (the grid op is
 // applied per node), or with vel_planes velocities already (mpm_sim.py:528-533,575-579);

@@ -202,6 +202,17 @@ class Engine:
         call("jm_read_frame", self._h, _fptr(out), out.size)
         return out
 
+    def read_frame_u8(self) -> np.ndarray:
+        """The frame encoded to uint8 on the device, as the viewers encode it on the host:
+        (np.clip(frame, 0, 1) * 255).astype(np.uint8) (evolve.py:396-397)."""
+        res = getattr(self, "_frame_res", 0)
+        if not res:
+            raise RuntimeError("no frame buffer: call render_clear first")
+        out = np.empty((res, res, 3), np.uint8)
+        call("jm_read_frame_u8", self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+             out.size)
+        return out
+
     def read_vorticity(self) -> np.ndarray:
         out = np.empty((self.M, self.nx, self.ny), np.float32)
         call("jm_read_vorticity", self._h, _fptr(out), out.size)

@@ -374,12 +374,18 @@ def render_vorticity_white(p_res_sub: int, p_grid_side: int, alpha_max: float) -
 
 
 class _FrameBuffer:
-    """frame_buffer: (video_res, video_res, 3) fp32, read back with to_numpy()."""
+    """frame_buffer: (video_res, video_res, 3) fp32, read back with to_numpy() (or
+    to_uint8(), already encoded to 8 bits)."""
 
     shape = (video_res, video_res, 3)
 
     def to_numpy(self) -> np.ndarray:
         return _frame().read_frame()
+
+    def to_uint8(self) -> np.ndarray:
+        """(np.clip(frame_buffer.to_numpy(), 0, 1) * 255).astype(np.uint8), the viewers'
+        encode (evolve.py:396-397), done on the device: a quarter of the bytes cross."""
+        return _frame().read_frame_u8()
 
 
 class _VortGrid:

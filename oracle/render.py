@@ -163,6 +163,15 @@ def tone_map_and_encode(fb):
     return fb
 
 
+def encode_u8(fb):
+    """The viewers' 8-bit encode of a frame, (np.clip(img, 0, 1) * 255).astype(np.uint8)
+    (evolve.py:396-397, helpers/view_single.py:257): fp32 clip, fp32 product, truncation.
+    NaN is given the value 0 here (numpy's cast of NaN is unspecified; 0 on x86-64)."""
+    img = np.asarray(fb, f32)
+    c = np.clip(np.nan_to_num(img, nan=0.0, posinf=1.0, neginf=0.0), f32(0), f32(1))
+    return (c * f32(255)).astype(np.uint8)
+
+
 def grid_curl(grid_v, grid_m, dx):
     """curl = dvx/dy - dvy/dx at interior nodes with mass (mpm_sim.py:528-533,575-579);
     grid_v (M, nx, ny, 2) post-grid-op velocity, grid_m (M, nx, ny). NaN where skipped."""

@@ -90,6 +90,8 @@ def test_renderer_entry_points_reject_a_null_handle():
     assert L.jm_render_vorticity(None, 1, 32, 2, 0.5) == _lib.JM_ERR_ARG
     assert L.jm_read_frame(None, f, 4) == _lib.JM_ERR_ARG
     assert L.jm_read_vorticity(None, f, 4) == _lib.JM_ERR_ARG
+    b = (ctypes.c_uint8 * 4)()
+    assert L.jm_read_frame_u8(None, b, 4) == _lib.JM_ERR_ARG
 
 
 def test_engine_fails_loudly_without_a_device():

@@ -245,3 +245,29 @@ def test_render_edge_cases():
                                         radius, eng.domain_height)
         assert (eng.read_frame() == wflat).all()
     eng.close()
+
+
+def test_frame_encode_u8():
+    """jm_read_frame_u8 against the oracle's encode of the fp32 frame read back, bit-exact:
+    constant frames on both sides of [0, 1] and NaN, a raw abyss frame (values above 1), its
+    tone-mapped frame, and a resolution whose byte count is not a multiple of 4."""
+    say("frame encode u8")
+    eng = _engine(M=3, N=12000, substeps=100)
+    hue = np.full(eng.M, 0.3, np.float32)
+    for res in (97, 256):
+        for value in (-0.5, 0.0, 1 / 255, 0.3, 0.5, 1.0, 7.0, float("nan")):
+            eng.render_clear(res, value)
+            got = eng.read_frame_u8()
+            assert (got == render.encode_u8(eng.read_frame())).all(), (res, value)
+        grid_side = 2
+        eng.render_clear(res, 0.0)
+        eng.render_abyss(res // grid_side, grid_side, 2.0, hue, hue, 1)
+        raw = eng.read_frame()
+        assert raw.max() > 1.0 and (raw == 0).any()
+        assert (eng.read_frame_u8() == render.encode_u8(raw)).all(), res
+        eng.render_tone_map()
+        tm = eng.read_frame()
+        enc = eng.read_frame_u8()
+        say(f"  res {res}: {int((enc > 0).sum())} nonzero bytes, max {int(enc.max())}")
+        assert (enc == render.encode_u8(tm)).all(), res
+    eng.close()

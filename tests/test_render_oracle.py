@@ -226,6 +226,28 @@ def test_tone_map_closed_forms():
     assert np.all(np.diff(out[[0, 3, 1, 4, 2]]) > 0)
 
 
+def test_encode_u8_against_exact_floor():
+    """encode_u8 is floor(255 v) of the clipped value: checked in float64 arithmetic, where
+    the fp32 product may only move a value lying within one fp32 ulp of an integer to its
+    neighbour; out-of-range values and NaN saturate; the map is monotone."""
+    rng = np.random.default_rng(7)
+    k = np.arange(256, dtype=np.float64)
+    edges = (k / 255).astype(f32)
+    v = np.concatenate([rng.random(200000).astype(f32), edges, np.nextafter(edges, f32(2)),
+                        np.nextafter(edges, f32(-1))])
+    e = render.encode_u8(v.reshape(1, -1, 1)).ravel().astype(np.int64)
+    c = np.clip(v.astype(np.float64), 0, 1) * 255
+    exact = np.floor(c).astype(np.int64)
+    near = np.abs(c - np.round(c)) <= np.spacing(np.float32(255))
+    assert np.all((e == exact) | (near & (np.abs(e - exact) <= 1)))
+    assert (e[v >= 1] == 255).all() and (e[v <= 0] == 0).all()
+    assert e[(v > 0) & (v < 1)].max() <= 254
+    order = np.argsort(v, kind="stable")
+    assert np.all(np.diff(e[order]) >= 0)
+    special = np.array([np.nan, -np.inf, np.inf, -0.0, 1e30, -3.0, 0.5], f32)
+    assert render.encode_u8(special).tolist() == [0, 0, 255, 0, 255, 0, 127]
+
+
 def test_curl_of_linear_fields():
     """v = (a y + c, b x + d) has curl dvx/dy - dvy/dx = a - b at every interior node with mass;
     boundary nodes and massless nodes are skipped (vorticity 0)."""
