@@ -52,6 +52,7 @@ def main(out_path: str) -> int:
     hdr = sim.frame_buffer.to_numpy()
     sim.tone_map_and_encode()
     ldr = sim.frame_buffer.to_numpy()
+    ldr_u8 = sim.frame_buffer.to_uint8()       # evolve.py:396-397's encode, on the device
     sim.clear_frame_buffer_white()
     for mat_id, r, g, b in sim.WEB_PALETTE:
         sim.render_flat_pass(sim.res_sub, sim.grid_side, radius, mat_id, r, g, b)
@@ -68,7 +69,8 @@ def main(out_path: str) -> int:
     rdbu = sim.frame_buffer.to_numpy()
     print("[driver] frames done", flush=True)
     np.savez(out_path, results=results, muscle_counts=np.array(muscle_counts),
-             genomes=np.array(genomes), hdr=hdr, ldr=ldr, web=web, t=t, radius=radius,
+             genomes=np.array(genomes), hdr=hdr, ldr=ldr, ldr_u8=ldr_u8, web=web, t=t,
+             radius=radius,
              grid_v=grid_v, grid_m=grid_m, vort=vort, base=base, rdbu=rdbu,
              hue=sim.instance_hue.to_numpy(), muscle_hue=sim.instance_muscle_hue.to_numpy(),
              freq=sim.instance_freq.to_numpy(), contraction=sim.instance_act_contraction.to_numpy(),

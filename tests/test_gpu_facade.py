@@ -6,7 +6,8 @@ environment), checked against the oracle on the same population:
   headless test), fitness (evolve.py:90-124) from both within 1e-4;
 * the frames the facade read back: the abyss HDR frame against the renderer oracle on the
   state the facade read back (splat tolerance of tests/test_gpu_render.py), the tone-mapped
-  frame against the oracle's tone map of that HDR frame (8 ulps), the web palette bit-exact;
+  frame against the oracle's tone map of that HDR frame (8 ulps) and its 8-bit encode
+  (frame_buffer.to_uint8) bit-exact, the web palette bit-exact;
   ``vort_grid`` against the oracle's vorticity of the ``grid_v``/``grid_m`` read back (1e-6)
   and the RdBu composite against the oracle's over the same tone-mapped frame (4 ulps).
 """
@@ -64,6 +65,7 @@ def test_evolve_style_driver(tmp_path):
     tm = render.tone_map_and_encode(d["hdr"].copy())
     ulps = np.abs(d["ldr"].astype(np.float64) - tm) / np.spacing(np.maximum(np.abs(tm), 1e-30))
     assert ulps.max() <= 8, ulps.max()
+    np.testing.assert_array_equal(d["ldr_u8"], render.encode_u8(d["ldr"]))
     web = np.full((res, res, 3), 0.98, np.float32)
     for mat_id, r, g, b in [(0, 0.933, 0.949, 0.957), (1, 0.678, 0.882, 0.875),
                             (3, 0.976, 0.757, 0.765), (2, 0.961, 0.824, 0.576)]:
