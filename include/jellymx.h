@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JM_ABI_VERSION 3
+#define JM_ABI_VERSION 4
 
 typedef enum jm_status {
   JM_OK = 0,
@@ -86,6 +86,14 @@ typedef struct jm_config {
                             (ring-weighted P2G, hoop stress, axis BC; mpm_sim.py:127-133,
                             163-165,236-304); runs on JM_PATH_PLAIN only */
   double domain_height;  /* JELLY_DOMAIN_H, mpm_sim.py:16 */
+  int32_t nan_invalid;   /* 1: jm_run_batch_headless marks an instance invalid (valid = 0)
+                            when its final CoM is not finite or any of its live particles
+                            ends with a non-finite v, C or F (a diverged state: advection
+                            clamps positions into the domain, mpm_sim.py:331-333, so a
+                            diverged instance's CoM stays finite). 0 (default): the
+                            reference's gate `final_y < 0.01` alone (mpm_sim.py:830-833),
+                            under which a NaN CoM counts as valid */
+  int32_t reserved;      /* must be 0 */
 } jm_config;
 
 /* Launch tuning (chosen by jellyfih_amd/autotune.py; defaults are sensible). */

@@ -12,7 +12,7 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libjellymx.so")
 
-JM_ABI_VERSION = 3
+JM_ABI_VERSION = 4
 
 # jm_status
 JM_OK, JM_ERR_ARG, JM_ERR_HIP, JM_ERR_STATE, JM_ERR_NOMEM = range(5)
@@ -54,6 +54,8 @@ class JmConfig(ctypes.Structure):
         ("path", ctypes.c_int32),
         ("axisym", ctypes.c_int32),
         ("domain_height", ctypes.c_double),
+        ("nan_invalid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

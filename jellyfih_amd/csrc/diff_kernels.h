@@ -187,7 +187,7 @@ __device__ __forceinline__ double next_lane(double v) {
 // evaluated in the separable form wy_j P_i + ey_j Q_i (P_i = wx_i (m v + A_0 dx_i),
 // Q_i = wx_i A_1, ey_j = wy_j dy_j), then summed over same-cell groups of 4 lanes
 // (jm::lane_groups / presum3) so that one lane per group issues the atomics. A particle
-// whose contributions may reach 1 in magnitude (or are NaN), or any particle when the tile
+// whose contribution bounds reach 0.75 (or are NaN), or any particle when the tile
 // is not used, scatters to the global grid with fp32 atomics and then carries zeros; lanes
 // without a particle carry zeros computed from finite stand-ins, so every lane's values are
 // finite in the pre-sums. Updates p.F for in-range particles.
@@ -218,11 +218,14 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
   float wx[3], wy[3], dw[3], ey[3], P0[3], Q0[3], P1[3], Q1[3], Mi[3];
   weights(f0, wx, dw);
   weights(f1, wy, dw);
+  // Each of the 27 contributions is at most 0.5625 times these bounds and up to 4 lanes add
+  // before one encode, so values stay inside fixed_bits' exact range (|c| < 2) with a
+  // bound of 0.75. No fmaxf: it drops NaN operands, and each comparison fails on NaN.
   const float bnd = 1.5f * k.dx;
-  const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(A.a) + fabsf(A.b)),
-                                fabsf(mv1) + bnd * (fabsf(A.c) + fabsf(A.d))),
-                          mass);
-  const bool lds_ok = active && tl.use && lim < 1.0f;
+  const bool lds_ok = active && tl.use &&
+                      fabsf(mv0) + bnd * (fabsf(A.a) + fabsf(A.b)) < 0.75f &&
+                      fabsf(mv1) + bnd * (fabsf(A.c) + fabsf(A.d)) < 0.75f && mass < 0.75f &&
+                      f0 + f1 < 4.0f;
   const float sc = lds_ok ? jm::kFixPre : (active ? 1.0f : 0.0f);
 #pragma unroll
   for (int ii = 0; ii < 3; ++ii) {

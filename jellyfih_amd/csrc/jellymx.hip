@@ -82,6 +82,7 @@ struct jm_handle {
   int4* chunks_d = nullptr;
   int* nchunks_d = nullptr;
   float* stats_d = nullptr;   // M * 6
+  int* nonfinite_d = nullptr; // M: live particles with a non-finite v, C or F (nan_invalid)
   float* staging = nullptr;   // M * 2 * G floats (grid velocity readout)
   float* damp = nullptr;      // nx + ny floats: boundary damping factors (k.damp)
   float* frame = nullptr;     // renderer: frame_res^2 x 3 floats (allocated on first use)
@@ -133,7 +134,7 @@ struct jm_handle {
     if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {st.base, tmp.base, sortbuf, hist, acc, t_buf, params, n_live_d,
                     chunks_d, nchunks_d, stats_d, staging, damp, frame, frame_u8, vort, rhue,
-                    hoop};
+                    hoop, nonfinite_d};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     st.base = tmp.base = nullptr;
@@ -143,6 +144,7 @@ struct jm_handle {
     frame_res = 0;
     rhue_host.clear();
     n_live_d = nullptr;
+    nonfinite_d = nullptr;
     chunks_d = nullptr;
     nchunks_d = nullptr;
     for (int g = 1; g < kMaxGroups; ++g) {
@@ -639,6 +641,13 @@ int group_graph(jm_handle* h, int g, int n, int b, hipGraphExec_t* out) {
 // joined back into the main stream. With graphs, each group's kernels of a block are one
 // captured graph per (n, grid-buffer base, group), captured before the block is enqueued.
 int run_fused(jm_handle* h, int64_t steps) {
+  if (h->total_chunks == 0) {
+    // no live particle: only the clock moves, as the reference's substep() still advances it
+    hipLaunchKernelGGL(jm::k_time_advance_n, dim3(1), dim3(64), 0, h->stream, h->t_buf,
+                       h->k.dt, steps);
+    HIP_TRY(hipGetLastError());
+    return JM_OK;
+  }
   int64_t left = steps;
   while (left > 0) {
     const int n = (int)std::min<int64_t>(left, h->tun.block_substeps);
@@ -826,6 +835,9 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
     return fail(JM_ERR_ARG, "unknown path");
   if (!(cfg->domain_height > 0.002)) return fail(JM_ERR_ARG, "domain_height too small");
   if (cfg->axisym != 0 && cfg->axisym != 1) return fail(JM_ERR_ARG, "axisym must be 0 or 1");
+  if (cfg->nan_invalid != 0 && cfg->nan_invalid != 1)
+    return fail(JM_ERR_ARG, "nan_invalid must be 0 or 1");
+  if (cfg->reserved != 0) return fail(JM_ERR_ARG, "reserved must be 0");
   if (cfg->axisym && cfg->path != JM_PATH_PLAIN)
     return fail(JM_ERR_ARG, "the axisymmetric variant runs on JM_PATH_PLAIN");
   const int nx = 128 * cfg->quality;
@@ -884,6 +896,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
             dm((void**)&h->acc, (size_t)3 * M * 3 * h->G * 4) && dm((void**)&h->t_buf, 8 * kMaxGroups) &&
             dm((void**)&h->params, (size_t)6 * M * 4) && dm((void**)&h->n_live_d, M * 4) &&
             dm((void**)&h->nchunks_d, M * 4) && dm((void**)&h->stats_d, (size_t)M * 6 * 4) &&
+            dm((void**)&h->nonfinite_d, M * 4) &&
             dm((void**)&h->staging, (size_t)M * 2 * h->G * 4) &&
             dm((void**)&h->damp, (size_t)(nx + ny) * 4) &&
             (!axisym || dm((void**)&h->hoop, (size_t)M * 2 * h->G * 4));
@@ -1059,17 +1072,34 @@ int jm_run_batch_headless(jm_handle* h, int64_t steps, double* out) {
   JM_TRY(jm_compute_stats(h, JM_STATS_ANY, r0.data()));  // :811
   JM_TRY(jm_substep(h, steps));                            // :814-815
   JM_TRY(jm_compute_stats(h, JM_STATS_ANY, r1.data()));  // :818
-  // mpm_sim.py:793-800 (host division in double) and :821-834
+  std::vector<int> nonfinite(M, 0);
+  if (h->cfg.nan_invalid) {
+    hipLaunchKernelGGL(jm::k_fill_i32, dim3(1), dim3(256), 0, h->stream, h->nonfinite_d,
+                       (int64_t)M, 0);
+    hipLaunchKernelGGL(jm::k_count_nonfinite, dim3(blocks_for(max_live(h), 256), M), dim3(256),
+                       0, h->stream, h->st.s, h->n_live_d, h->cap, h->nonfinite_d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(nonfinite.data(), h->nonfinite_d, M * sizeof(int),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  // mpm_sim.py:793-800: raw[i, 0] / count divides two np.float32 values, so the quotient is
+  // rounded to fp32 before it is stored in the float64 table; then :821-834
   for (int m = 0; m < M; ++m) {
     double* o = out + 5 * m;
     for (int c = 0; c < 5; ++c) o[c] = 0.0;
-    const double c0 = r0[3 * m + 1], c1 = r1[3 * m + 1];
-    if (c0 > 0 && c1 > 0) {
-      o[0] = (double)r0[3 * m] / c0;
-      o[1] = (double)r0[3 * m + 2] / c0;
-      o[2] = (double)r1[3 * m] / c1;
-      o[3] = (double)r1[3 * m + 2] / c1;
-      o[4] = o[2] < 0.01 ? 0.0 : 1.0;
+    const float c0 = r0[3 * m + 1], c1 = r1[3 * m + 1];
+    if (c0 > 0.0f && c1 > 0.0f) {
+      o[0] = (double)(r0[3 * m] / c0);
+      o[1] = (double)(r0[3 * m + 2] / c0);
+      o[2] = (double)(r1[3 * m] / c1);
+      o[3] = (double)(r1[3 * m + 2] / c1);
+      // The reference's gate `final_y < 0.01` lets a NaN CoM through as valid (H12);
+      // nan_invalid = 1 marks an instance invalid when its CoM is not finite or any of its
+      // live particles ended with a non-finite v, C or F.
+      const bool finite = std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]) &&
+                          std::isfinite(o[3]) && nonfinite[m] == 0;
+      o[4] = (o[2] < 0.01 || (h->cfg.nan_invalid && !finite)) ? 0.0 : 1.0;
     }
   }
   return JM_OK;

@@ -330,6 +330,15 @@ __global__ void k_time_advance(float* t_buf, float dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) t_buf[0] = t_buf[0] + dt;
 }
 
+// n substeps' worth of sim_time += dt, one fp32 addition each (mpm_sim.py:340): the clock of
+// a fused-path batch with no live particles, where no substep kernel runs.
+__global__ void k_time_advance_n(float* t_buf, float dt, int64_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float t = t_buf[0];
+  for (int64_t s = 0; s < n; ++s) t = t + dt;
+  t_buf[0] = t;
+}
+
 // ---------------------------------------------------------------------------------------
 // Fused path. One launch per substep: gather the previous substep's grid through LDS (the
 // grid op is applied on the fly while staging), G2P + advection, then P2G of the next
@@ -387,11 +396,16 @@ __device__ __forceinline__ void block_bbox(int& mn0, int& mx0, int& mn1, int& mx
 
 constexpr int kBoxEmpty = 1 << 29;
 
-// P2G accumulates in LDS as 64-bit fixed point (fixed_bits, lanes.h). A window gathers at
-// most 4096 particles, so with |c| < 1 every node sum fits int64. Particles with a
-// contribution outside that range (or NaN) scatter straight to the global fp32 grid
-// instead, so non-finite values propagate as they do in fp32.
-constexpr float kFixMax = 1.0f;
+// P2G accumulates in LDS as 64-bit fixed point (fixed_bits, lanes.h). A particle's 27
+// contributions are w_ij (m v + A d) and w_ij m with w_ij <= 0.5625 and |d| <= 1.5 dx, so
+// bounding |m v_c| + 1.5 dx (|A_c0| + |A_c1|) and m by kFixMax bounds each contribution by
+// 0.5625 kFixMax. The same-cell pre-sums add up to 4 lanes before one encode, so an encoded
+// value stays below 2.25 kFixMax = 1.6875 < 2, inside fixed_bits' exact range (|c| < 2);
+// a window gathers at most 4096 particles, so every node sum fits int64. A particle outside
+// that bound, or with a NaN or infinite term (the comparisons below fail on NaN), scatters
+// straight to the global fp32 grid instead, so non-finite values reach exactly its nine
+// nodes, as in fp32.
+constexpr float kFixMax = 0.75f;
 
 __device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float y) {
   atomicAdd(slot, fixed_bits(y));
@@ -724,10 +738,13 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       bspline(fx0, wx);
       bspline(fx1, wy);
       const float bnd = 1.5f * k.dx;
-      const float lim = fmaxf(fmaxf(fabsf(mv0) + bnd * (fabsf(aff.a00) + fabsf(aff.a01)),
-                                    fabsf(mv1) + bnd * (fabsf(aff.a10) + fabsf(aff.a11))),
-                              mass);
-      const bool fixed_ok = inr && use_lds && lim < kFixMax;
+      // No fmaxf here: it drops NaN operands. Each comparison fails on NaN, and the
+      // fractional offsets are tested too (a NaN position truncates to cell 0 with NaN
+      // weights).
+      const bool fixed_ok = inr && use_lds &&
+                            fabsf(mv0) + bnd * (fabsf(aff.a00) + fabsf(aff.a01)) < kFixMax &&
+                            fabsf(mv1) + bnd * (fabsf(aff.a10) + fabsf(aff.a11)) < kFixMax &&
+                            mass < kFixMax && fx0 + fx1 < 4.0f;
       const float sc = fixed_ok ? kFixPre : (inr ? 1.0f : 0.0f);
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
@@ -984,6 +1001,28 @@ __global__ void __launch_bounds__(256) k_sort_gather(SortArgs a, const int* perm
   a.s.slot[i] = a.tmp.slot[src];
   if (a.s.glow) a.s.glow[i] = a.tmp.glow[src];
   if (a.s.r_ref) a.s.r_ref[i] = a.tmp.r_ref[src];
+}
+
+// ---------------------------------------------------------------------------------------
+// Live particles of each instance whose v, C or F holds a NaN or an infinity (a diverged
+// state; its positions stay finite because advection clamps them into the domain,
+// mpm_sim.py:331-333). out[m] must start at 0. blockIdx.y = instance.
+__global__ void __launch_bounds__(256) k_count_nonfinite(State s, const int* n_live, int cap,
+                                                         int* out) {
+  const int m = blockIdx.y;
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (local < n_live[m]) {
+    const size_t i = (size_t)m * cap + local;
+    const float2 v = s.v[i];
+    const float4 C = s.C[i], F = s.F[i];
+    // x * 0 is 0 for every finite x and NaN for NaN and +-inf
+    const float z = ((v.x * 0.f + v.y * 0.f) + (C.x * 0.f + C.y * 0.f) + (C.z * 0.f + C.w * 0.f)) +
+                    ((F.x * 0.f + F.y * 0.f) + (F.z * 0.f + F.w * 0.f));
+    bad = !(z == 0.f);
+  }
+  const unsigned long long b = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && b != 0ull) atomicAdd(&out[m], __popcll(b));
 }
 
 // ---------------------------------------------------------------------------------------

@@ -70,9 +70,9 @@ __device__ __forceinline__ void presum3(float& c0, float& c1, float& c2, float f
 
 // LDS scatter sums are 64-bit fixed point with 50 fractional bits: integer LDS atomics
 // run at full rate on gfx950 where fp32 LDS atomics do not, and integer sums are exact and
-// independent of the order of the contributions. With |c| < 1 per contribution and at most
-// 4096 contributions per node, every node sum fits int64 (partial sums may wrap; the final
-// two's-complement sum is still exact).
+// independent of the order of the contributions. fixed_bits encodes |c| < 2 exactly (below);
+// with encoded values below 2 and at most 4096 of them per node, every node sum fits int64
+// (partial sums may wrap; the final two's-complement sum is still exact).
 constexpr double kFixInv = 1.0 / 1125899906842624.0;   // 2^-50
 constexpr float kFixPre = 1125899906842624.0f;          // 2^50
 

@@ -52,7 +52,8 @@ class Engine:
 
     def __init__(self, n_instances: int = 16, n_particles: int = 80000, quality: int = 1,
                  n_grid_y: int | None = None, domain_height: float = 1.0,
-                 track_glow: bool = False, path: str = "fused", axisym: bool = False):
+                 track_glow: bool = False, path: str = "fused", axisym: bool = False,
+                 nan_invalid: bool = False):
         L = load()
         cfg = JmConfig()
         call("jm_default_config", ctypes.byref(cfg))
@@ -64,6 +65,7 @@ class Engine:
         cfg.path = {"fused": _lib.JM_PATH_FUSED, "plain": _lib.JM_PATH_PLAIN}[path]
         cfg.domain_height = float(domain_height)
         cfg.axisym = int(bool(axisym))
+        cfg.nan_invalid = int(bool(nan_invalid))
         h = ctypes.c_void_p()
         call("jm_create", ctypes.byref(cfg), ctypes.byref(h))
         self._L = L
@@ -166,8 +168,8 @@ class Engine:
     def render_clear(self, res: int = 1024, value: float = 0.0) -> None:
         """Fill the res x res x 3 frame buffer with ``value`` (clear_frame_buffer: 0,
         clear_frame_buffer_white: 0.98), allocating it on first use."""
-        self._frame_res = int(res)
         call("jm_render_clear", self._h, int(res), float(value))
+        self._frame_res = int(res)  # only once the library has accepted res
 
     def render_abyss(self, res_sub: int, grid_side: int, radius: float, hue, muscle_hue,
                      water_mode: int = 0) -> None:

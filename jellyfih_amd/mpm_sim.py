@@ -33,6 +33,10 @@ n_grid = 128 * quality
 n_grid_y = int(os.environ.get("JELLY_GRID_Y", str(n_grid)))
 domain_height = float(os.environ.get("JELLY_DOMAIN_H", "1.0"))
 axisym = os.environ.get("JELLY_AXISYM", "0") == "1"
+# Opt-in: run_batch_headless marks an instance whose CoM is not finite invalid. Off by
+# default, where the reference's `final_y < 0.01` gate lets a NaN CoM count as valid
+# (mpm_sim.py:830-833).
+nan_invalid = os.environ.get("JELLY_NAN_INVALID", "0") == "1"
 dx, inv_dx = 1 / n_grid, float(n_grid)
 dt = 2e-5 / quality
 p_vol = (dx * 0.5) ** 2
@@ -98,7 +102,8 @@ def engine():
         _engine = Engine(n_instances=n_instances, n_particles=n_particles, quality=quality,
                          n_grid_y=n_grid_y, domain_height=domain_height,
                          track_glow=os.environ.get("JELLY_GLOW", "0") == "1",
-                         path="plain" if axisym else "fused", axisym=axisym)
+                         path="plain" if axisym else "fused", axisym=axisym,
+                         nan_invalid=nan_invalid)
         tuned = None if (axisym or n_grid_y != n_grid) else _tuning_for(n_instances,
                                                                          n_particles)
         if tuned:
@@ -299,6 +304,19 @@ def run_batch_headless(steps: int) -> np.ndarray:
 def sync() -> None:
     """ti.sync()"""
     engine().sync()
+
+
+class _TaichiNamespace:
+    """The ``ti`` name callers reach through this module: ``sim.ti.sync()``
+    (run_population.py:37,47; evolve.py's viewers) waits for all queued device work."""
+
+    @staticmethod
+    def sync() -> None:
+        if _engine is not None:  # nothing can be queued before the engine exists
+            _engine.sync()
+
+
+ti = _TaichiNamespace()
 
 
 # --- renderer (mpm_sim.py:343-712) -----------------------------------------------------------

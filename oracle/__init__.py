@@ -303,17 +303,28 @@ class OracleSim:
             raw[no_payload] = self.raw_stats(True)[no_payload]
         return finish_stats(raw)
 
-    def run_batch_headless(self, steps: int, nthreads: int = 1) -> np.ndarray:
+    def run_batch_headless(self, steps: int, nthreads: int = 1,
+                           nan_invalid: bool = False) -> np.ndarray:
         """mpm_sim.py:802-834."""
         self.t[0] = 0.0
         initial = self.get_stats_any()
         self.run(steps, nthreads=nthreads)
         final = self.get_stats_any()
-        return assemble_results(initial, final)
+        return assemble_results(initial, final, nan_invalid=nan_invalid,
+                                diverged=self.diverged())
+
+    def diverged(self) -> np.ndarray:
+        """Per instance: whether a live particle holds a non-finite v, C or F."""
+        live = self.material >= 0
+        bad = ~(np.isfinite(self.v).all(-1) & np.isfinite(self.C).all((-2, -1)) &
+                np.isfinite(self.F).all((-2, -1)))
+        return (bad & live).any(axis=1)
 
 
 def finish_stats(raw: np.ndarray) -> np.ndarray:
-    """Host division of the fitness buffer into [com_y, com_x, count] (mpm_sim.py:793-800)."""
+    """Host division of the fitness buffer into [com_y, com_x, count] (mpm_sim.py:793-800).
+    The reference's buffer is an fp32 array, so each quotient is rounded to the buffer's
+    precision before it is stored in the float64 table."""
     M = raw.shape[0]
     stats = np.zeros((M, 3))
     for i in range(M):
@@ -325,8 +336,12 @@ def finish_stats(raw: np.ndarray) -> np.ndarray:
     return stats
 
 
-def assemble_results(initial: np.ndarray, final: np.ndarray) -> np.ndarray:
-    """(M,5) = [init_y, init_x, final_y, final_x, valid] (mpm_sim.py:821-834)."""
+def assemble_results(initial: np.ndarray, final: np.ndarray, nan_invalid: bool = False,
+                     diverged=None) -> np.ndarray:
+    """(M,5) = [init_y, init_x, final_y, final_x, valid] (mpm_sim.py:821-834). The
+    reference's gate ``final_y < 0.01`` passes a NaN CoM as valid; ``nan_invalid`` (the
+    engine's opt-in flag) marks an instance invalid when its CoM is not finite or
+    ``diverged[i]`` (a live particle with a non-finite v, C or F) holds."""
     M = initial.shape[0]
     results = np.zeros((M, 5))
     for i in range(M):
@@ -335,7 +350,9 @@ def assemble_results(initial: np.ndarray, final: np.ndarray) -> np.ndarray:
             results[i, 1] = initial[i, 1]
             results[i, 2] = final[i, 0]
             results[i, 3] = final[i, 1]
-            results[i, 4] = 0.0 if final[i, 0] < 0.01 else 1.0
+            bad = nan_invalid and (not np.all(np.isfinite(results[i, :4])) or
+                                   (diverged is not None and bool(diverged[i])))
+            results[i, 4] = 0.0 if (final[i, 0] < 0.01 or bad) else 1.0
     return results
 
 

@@ -42,8 +42,9 @@ def test_struct_layouts_match_header():
     assert [f for f, _ in _lib.JmConfig._fields_] == _header_fields("jm_config")
     assert [f for f, _ in _lib.JmTuning._fields_] == _header_fields("jm_tuning")
     assert [f for f, _ in _lib.JmDiffInputs._fields_] == _header_fields("jm_diff_inputs")
-    assert ctypes.sizeof(_lib.JmConfig) == 40
+    assert ctypes.sizeof(_lib.JmConfig) == 48
     assert _lib.JmConfig.domain_height.offset == 32
+    assert _lib.JmConfig.nan_invalid.offset == 40
     assert ctypes.sizeof(_lib.JmTuning) == 36
 
 
@@ -53,12 +54,14 @@ def test_default_config_is_the_reference_default():
     assert cfg.abi_version == _lib.JM_ABI_VERSION
     assert (cfg.n_instances, cfg.n_particles, cfg.quality) == (16, 80000, 1)   # mpm_sim.py:11-13
     assert cfg.n_grid_y == 0 and cfg.domain_height == 1.0 and cfg.path == _lib.JM_PATH_FUSED
+    assert cfg.nan_invalid == 0 and cfg.reserved == 0  # the reference's validity gate
 
 
 @pytest.mark.parametrize("field,value", [("abi_version", 99), ("n_instances", 0),
                                          ("n_particles", -5), ("quality", 0), ("path", 7),
                                          ("n_grid_y", 6), ("domain_height", 0.0),
-                                         ("axisym", 2)])
+                                         ("axisym", 2), ("nan_invalid", 3),
+                                         ("reserved", 1)])
 def test_create_rejects_bad_config_before_device_work(field, value):
     cfg = _lib.JmConfig()
     _lib.call("jm_default_config", ctypes.byref(cfg))
@@ -127,6 +130,16 @@ def test_facade_constants_match_reference():
                  "render_vorticity_overlay", "render_vorticity_rdbu",
                  "render_vorticity_hueshift", "render_vorticity_white"):
         assert callable(getattr(s, name))
+
+
+def test_facade_ti_sync_without_an_engine():
+    """run_population.py:37,47 call sim.ti.sync(); before the engine exists there is
+    nothing queued, and it must not create one (no device work on import paths)."""
+    from jellyfih_amd import mpm_sim as s
+    had = s._engine
+    s.ti.sync()
+    assert s._engine is had
+    assert s.nan_invalid is False  # the reference's validity gate unless JELLY_NAN_INVALID=1
 
 
 def test_facade_uses_the_autotuned_parameters_for_their_batch_shape(tmp_path):

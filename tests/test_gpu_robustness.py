@@ -1,0 +1,154 @@
+"""Failure handling of the HIP engine, through its C ABI, against the oracle.
+
+* NaN propagation: a NaN velocity injected into a few particles reaches exactly the grid
+  nodes of those particles and the particles that gather from them, on both step paths, as
+  in the oracle (bit-exact NaN masks). The fused path's fixed-point LDS scatter must send
+  such particles to the global grid (a NaN encoded as fixed point would be a large finite
+  number instead).
+* Instance isolation: a NaN in one instance leaves every other instance's positions and
+  headless results bit-identical to a clean run. Each instance fits one fused chunk here, so
+  every grid node receives one fp32 global add per substep and clean runs are bit-for-bit
+  reproducible (larger instances flush several chunks per node in a run-dependent order).
+* The opt-in NaN->invalid flag of jm_run_batch_headless, against the oracle's.
+* The clock of a batch with no live particles still advances one fp32 dt per substep.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from jellyfih_amd.engine import Engine
+from tests.cases import small_tank
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+REGIONS = [(0.10, 0.22, 0.10, 0.22), (0.40, 0.52, 0.30, 0.42), (0.70, 0.82, 0.15, 0.27),
+           (0.45, 0.57, 0.60, 0.72), (0.15, 0.27, 0.55, 0.67), (0.75, 0.87, 0.70, 0.82),
+           (0.30, 0.42, 0.80, 0.92), (0.60, 0.72, 0.45, 0.57)]
+N_SLOTS = 1100
+
+
+def one_chunk_cases():
+    return [small_tank(10 + m, N_SLOTS, region=r) for m, r in enumerate(REGIONS)]
+
+
+def make_engine(cases, path="fused", nan_invalid=False):
+    eng = Engine(len(cases), N_SLOTS, path=path, nan_invalid=nan_invalid)
+    if path == "fused":
+        eng.set_tuning(threads=256, ppt=4, instance_groups=2, sort_interval=100)
+    for m, (p, mt, f) in enumerate(cases):
+        eng.load_particles(m, p, mt, f)
+    return eng
+
+
+def nan_slots(cases, m, picks=(5, 300, 700)):
+    live = np.flatnonzero(cases[m][1] >= 0)
+    return live[list(picks)]
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_nan_propagation_matches_oracle(require_gpu, path):
+    say(f"NaN propagation path={path}")
+    cases = one_chunk_cases()
+    eng = make_engine(cases, path)
+    orc = oracle.OracleSim(len(cases), N_SLOTS)
+    for m, (p, mt, f) in enumerate(cases):
+        orc.load(m, p, mt, f)
+    bad = nan_slots(cases, 3)
+    v = np.zeros((len(cases), N_SLOTS, 2), np.float32)
+    v[3, bad] = np.nan
+    eng.write("v", v)
+    orc.v[...] = v
+    eng.set_time(0.05)
+    orc.t[0] = np.float32(0.05)
+    for steps in (1, 4):
+        eng.substep(steps)
+        orc.run(steps)
+        vg = eng.read("v")
+        nan_g = np.isnan(vg).any(-1)
+        nan_o = np.isnan(orc.v).any(-1)
+        np.testing.assert_array_equal(nan_g, nan_o)
+        np.testing.assert_array_equal(np.isnan(eng.read("C")).any((-2, -1)),
+                                      np.isnan(orc.C).any((-2, -1)))
+        assert nan_g[3].sum() > len(bad) and not nan_g[np.arange(8) != 3].any()
+        x = eng.read("x")
+        assert np.isfinite(x).all()
+        np.testing.assert_array_equal(x[3, nan_g[3]], np.float32([[0.999, 0.999]]) *
+                                      np.ones((nan_g[3].sum(), 1), np.float32))
+        clean = orc.material >= 0
+        clean[3] = False
+        assert float(np.max(np.abs(x[clean] - orc.x[clean]))) < 1e-5
+    if path == "fused":
+        # the grid the last G2P read: NaN at the same nodes as the oracle's grid
+        gv = eng.read("grid_v")
+        np.testing.assert_array_equal(np.isnan(gv).any(-1), np.isnan(orc.grid_v).any(-1))
+    eng.close()
+
+
+def test_nan_instance_is_isolated(require_gpu):
+    say("NaN instance isolation, 8 one-chunk instances, 300 substeps")
+    cases = one_chunk_cases()
+    bad = nan_slots(cases, 3)
+    runs = {}
+    for name, inject, flag in (("clean", False, False), ("clean2", False, False),
+                               ("nan", True, False), ("nan_flag", True, True)):
+        eng = make_engine(cases, "fused", nan_invalid=flag)
+        if inject:
+            v = eng.read("v")
+            v[3, bad] = np.nan
+            eng.write("v", v)
+        res = eng.run_batch_headless(300)
+        runs[name] = (res, eng.read("x"), eng.read("v"))
+        eng.close()
+    others = np.arange(8) != 3
+    # clean runs of one-chunk instances are reproducible bit for bit
+    np.testing.assert_array_equal(runs["clean"][0], runs["clean2"][0])
+    np.testing.assert_array_equal(runs["clean"][1], runs["clean2"][1])
+    for name in ("nan", "nan_flag"):
+        res, x, v = runs[name]
+        np.testing.assert_array_equal(res[others], runs["clean"][0][others])
+        np.testing.assert_array_equal(x[others], runs["clean"][1][others])
+        np.testing.assert_array_equal(v[others], runs["clean"][2][others])
+        assert np.isnan(v[3]).any() and np.isfinite(x[3]).all()
+    # the reference's gate keeps the diverged instance valid (its CoM is finite); the opt-in
+    # flag marks it invalid and leaves the others as they were
+    assert runs["nan"][0][3, 4] == runs["clean"][0][3, 4] == 1.0
+    assert runs["nan_flag"][0][3, 4] == 0.0
+    np.testing.assert_array_equal(runs["nan_flag"][0][others, 4], runs["clean"][0][others, 4])
+    # the oracle agrees on the validity flags, with and without the flag
+    orc = oracle.OracleSim(len(cases), N_SLOTS)
+    for m, (p, mt, f) in enumerate(cases):
+        orc.load(m, p, mt, f)
+    orc.v[3, bad] = np.nan
+    ref = orc.run_batch_headless(300, nan_invalid=True)
+    np.testing.assert_array_equal(ref[:, 4], runs["nan_flag"][0][:, 4])
+    assert float(np.max(np.abs(ref[others, :4] - runs["nan"][0][others, :4]))) < 1e-5
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_empty_batch_clock(require_gpu, path):
+    say(f"empty batch clock path={path}")
+    eng = Engine(2, 100, path=path)
+    eng.set_time(0.25)
+    eng.substep(1000)
+    t = np.float32(0.25)
+    for _ in range(1000):
+        t = np.float32(t + np.float32(2e-5))
+    assert eng.get_time() == t
+    # a run_batch_headless of an empty batch restarts the clock and reports nothing valid
+    res = eng.run_batch_headless(150)
+    assert np.all(res == 0.0)
+    t = np.float32(0.0)
+    for _ in range(150):
+        t = np.float32(t + np.float32(2e-5))
+    assert eng.get_time() == t
+    eng.close()
