@@ -288,6 +288,24 @@ int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const v
                  double t0, int32_t nsteps, const void* grad_out, void* grad_in,
                  void* grad_amp, double* grad_params, void* stream);
 
+/* jm_fwd_block (final-state mode, grid_out = NULL) without any device allocation. No
+ * counterpart in the reference; it is for callers that cannot allocate: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc. There is no handle:
+ * cfg (n_instances, quality, n_grid_y, domain_height) and n_particles / scalar_bits are
+ * what jm_diff_create takes, and the inputs, state_in and out are the caller's device
+ * buffers as for jm_fwd_block (work on `stream`, on the current device). The grid ring and
+ * the states between kernels live in one fixed array of JM_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends: 9*M*nx*ny + 32*P scalars must fit (each part rounded up to
+ * 256 bytes), else JM_ERR_ARG and nothing runs (e.g. fp32: one 128x128 instance up to about
+ * 257K particles). Every call on one GPU uses that GPU's array: calls that overlap on one
+ * GPU (two streams, two threads) race; calls queued on one stream do not. The kernels clear
+ * every grid they accumulate into, so earlier calls' contents do not matter. Returns the
+ * same results as jm_fwd_block (the same kernels). */
+#define JM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int jm_fwd_block_noalloc(const jm_config* cfg, int64_t n_particles, int32_t scalar_bits,
+                         const jm_diff_inputs* in, const void* state_in, double t0,
+                         int32_t nsteps, void* out, void* stream);
+
 /* ti.sync(): wait for all queued work. */
 int jm_sync(jm_handle* h);
 

@@ -13,6 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libjellymx.so")
 
 JM_ABI_VERSION = 4
+JM_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 # jm_status
 JM_OK, JM_ERR_ARG, JM_ERR_HIP, JM_ERR_STATE, JM_ERR_NOMEM = range(5)
@@ -36,7 +37,7 @@ EXPORTS = (
     "jm_stream", "jm_last_error", "jm_water_mask", "jm_polygon_self_intersects",
     "jm_points_in_polygon_grid",
     "jm_diff_create", "jm_diff_destroy",
-    "jm_fwd_block", "jm_bwd_block",
+    "jm_fwd_block", "jm_bwd_block", "jm_fwd_block_noalloc",
     "jm_render_clear", "jm_render_abyss", "jm_render_flat", "jm_render_tone_map",
     "jm_render_vorticity_grid", "jm_render_vorticity", "jm_read_frame", "jm_read_vorticity",
     "jm_read_frame_u8",
@@ -137,6 +138,9 @@ def load() -> ctypes.CDLL:
         "jm_diff_destroy": [H],
         "jm_fwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+        "jm_fwd_block_noalloc": [P(JmConfig), ctypes.c_int64, ctypes.c_int32, P(JmDiffInputs),
+                                 ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                                 ctypes.c_void_p, ctypes.c_void_p],
         "jm_bwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                          ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                          ctypes.c_void_p, ctypes.c_void_p],

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -1770,15 +1771,15 @@ jmd::DiffInputs<T> diff_inputs(const jm_diff_inputs* in) {
           static_cast<const T*>(in->amp_scale), static_cast<const T*>(in->params)};
 }
 
-template <typename T>
-const jmd::Consts<T>& consts_of(const jm_diff* d);
-template <>
-const jmd::Consts<float>& consts_of<float>(const jm_diff* d) { return d->kf; }
-template <>
-const jmd::Consts<double>& consts_of<double>(const jm_diff* d) { return d->kd; }
+// D: jm_diff, or a DiffScratch view (same members) over caller-free device scratch.
+template <typename T, typename D>
+const jmd::Consts<T>& consts_of(const D* d) {
+  if constexpr (sizeof(T) == 4) return d->kf;
+  else return d->kd;
+}
 
-template <typename T>
-int fwd_block(jm_diff* d, const jm_diff_inputs* in, const T* state_in, double t0_in,
+template <typename T, typename D>
+int fwd_block(D* d, const jm_diff_inputs* in, const T* state_in, double t0_in,
               int nsteps, T* out, T* grid_out, hipStream_t st) {
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
   const jmd::Consts<T>& k = consts_of<T>(d);
@@ -1825,8 +1826,8 @@ int fwd_block(jm_diff* d, const jm_diff_inputs* in, const T* state_in, double t0
   return JM_OK;
 }
 
-template <typename T>
-int bwd_block(jm_diff* d, const jm_diff_inputs* in, const T* traj, const T* grids,
+template <typename T, typename D>
+int bwd_block(D* d, const jm_diff_inputs* in, const T* traj, const T* grids,
               double t0_in, int nsteps, const T* grad_out, T* grad_in, T* grad_amp,
               double* grad_params, hipStream_t st) {
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
@@ -1870,6 +1871,48 @@ int bwd_block(jm_diff* d, const jm_diff_inputs* in, const T* traj, const T* grid
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
+
+// ---- fixed device scratch of jm_fwd_block_noalloc ---------------------------------------
+// Every GPU the library runs on holds one copy of this array until the process ends. Host
+// code never passes its name to a kernel (on the host the name is a stand-in, not a device
+// address): the entry point looks up each device's address once, with hipGetSymbolAddress,
+// and uses that.
+static __device__ __align__(256) unsigned char scratch[JM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+int noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static void* ptrs[kMaxDevices] = {nullptr};
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(JM_ERR_STATE, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(JM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptrs[dev] = p;
+  }
+  *out = ptrs[dev];
+  return JM_OK;
+}
+
+// The buffers fwd_block reads from a jm_diff, here carved from `scratch` (no destructor:
+// nothing is owned).
+struct DiffScratch {
+  int M = 0, G = 0;
+  int64_t P = 0;
+  jmd::Consts<float> kf{};
+  jmd::Consts<double> kd{};
+  void* grid = nullptr;  // 3 * M * 3 * G scalars: the forward grid ring
+  void* sbuf = nullptr;  // 2 * P * 12 scalars: states between kernels (ping-pong)
+  void* pbuf = nullptr;  // 8 * P scalars: F_new of the last two substeps
+};
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -1925,6 +1968,53 @@ int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const void* state_in, dou
                              static_cast<double*>(out), static_cast<double*>(grid_out), st);
   return fwd_block<float>(d, in, static_cast<const float*>(state_in), t0, nsteps,
                           static_cast<float*>(out), static_cast<float*>(grid_out), st);
+}
+
+int jm_fwd_block_noalloc(const jm_config* cfg, int64_t n_particles, int32_t scalar_bits,
+                         const jm_diff_inputs* in, const void* state_in, double t0,
+                         int32_t nsteps, void* out, void* stream) {
+  if (!cfg || !in) return fail(JM_ERR_ARG, "null argument");
+  if (cfg->abi_version != JM_ABI_VERSION) return fail(JM_ERR_ARG, "abi_version mismatch");
+  if (cfg->n_instances <= 0 || cfg->quality <= 0 || n_particles <= 0 ||
+      n_particles > (int64_t)INT32_MAX / 16)
+    return fail(JM_ERR_ARG, "bad sizes");
+  if (scalar_bits != 32 && scalar_bits != 64) return fail(JM_ERR_ARG, "scalar_bits: 32 or 64");
+  if (cfg->axisym) return fail(JM_ERR_ARG, "the differentiable path has no axisymmetric variant");
+  if (!in->inst || !in->mat || !in->fiber || !in->amp_scale || !in->params)
+    return fail(JM_ERR_ARG, "jm_diff_inputs has a null pointer");
+  if (!state_in || !out || nsteps < 1) return fail(JM_ERR_ARG, "bad state or step count");
+  const int nx = 128 * cfg->quality;
+  const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
+  if (ny < 8) return fail(JM_ERR_ARG, "n_grid_y too small");
+  DiffScratch d;
+  d.M = cfg->n_instances;
+  d.G = nx * ny;
+  d.P = n_particles;
+  const size_t es = scalar_bits / 8;
+  const size_t gbytes = align256(3 * (size_t)d.M * 3 * d.G * es);
+  const size_t sbytes = align256(2 * (size_t)d.P * 12 * es);
+  const size_t pbytes = align256(8 * (size_t)d.P * es);
+  if (gbytes + sbytes + pbytes > (size_t)JM_NOALLOC_SCRATCH_BYTES)
+    return fail(JM_ERR_ARG, "jm_fwd_block_noalloc: needs " +
+                                std::to_string(gbytes + sbytes + pbytes) +
+                                " bytes of scratch (9*M*nx*ny + 32*P scalars), beyond its " +
+                                std::to_string((size_t)JM_NOALLOC_SCRATCH_BYTES) +
+                                "; use jm_diff_create + jm_fwd_block");
+  void* base = nullptr;
+  JM_TRY(noalloc_scratch(&base));
+  char* p = static_cast<char*>(base);
+  d.grid = p;
+  d.sbuf = p + gbytes;
+  d.pbuf = p + gbytes + sbytes;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (scalar_bits == 64) {
+    d.kd = diff_consts<double>(*cfg, nx, ny);
+    return fwd_block<double>(&d, in, static_cast<const double*>(state_in), t0, nsteps,
+                             static_cast<double*>(out), (double*)nullptr, st);
+  }
+  d.kf = diff_consts<float>(*cfg, nx, ny);
+  return fwd_block<float>(&d, in, static_cast<const float*>(state_in), t0, nsteps,
+                          static_cast<float*>(out), (float*)nullptr, st);
 }
 
 int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const void* grids,
