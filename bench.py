@@ -4,22 +4,35 @@ Workload (the 32 x 80K generation evaluation): 32 seeded 11-gene genomes (seeds 
 uniform within the evolve.py bounds) filled into 80,000-slot water tanks on 128x128 grids
 by the fill_tank restatement (about 62-65K live particles each, ~2.0M total), actuation
 timing genes applied as evolve.load_batch does. One "step" = one 100-substep block of the
-whole batch (HIP-graph replay). The rate counts live particles:
-value = sum(live) * substeps / seconds of the timed region (GPU-synchronised).
+whole batch (HIP-graph replay), a slice of the 150,000-substep fitness episode. After at
+least 1000 warm-up substeps (so the particles have moved off their fill order), K steps
+are timed 5 times, each repetition bracketed by device synchronisation; the line reports
+the median repetition. The rate counts live particles:
+value = sum(live) * 100 * K / seconds of the median repetition.
 
 Also reported on the same line:
 * roofline: algorithmic 108 B per live particle-substep (x, v, C, F read and written, 48 B
-  each way; material and fibre read, 12 B) over the fused kernel's average duration, timed
-  with HIP events on the engine's stream; peak 8.0 TB/s HBM3E;
+  each way; material and fibre read, 12 B) over the fused kernel's average duration with
+  one launch per substep covering every live particle (instance_groups = 1), timed with HIP
+  events on the engine's stream; peak 8.0 TB/s HBM3E. Beside it, event-timed: the wall time
+  of one substep with the tuned concurrent instance-group launches, and the HBM traffic the
+  counters measured for one substep (with the commit it was measured at);
+* episode: one full fitness episode, run_batch_headless(150000) on the freshly loaded batch
+  (mpm_sim.py:802-834, evolve.py:48), its validity flags and finiteness, and the substep
+  time at the end of the episode against the start;
+* batch16: the 16-instance forward configuration (seeds 1-16, 20,000 substeps);
 * cpu_baseline: the oracle (CPU restatement, fp32, one instance per OpenMP thread) on a
-  bounded sample of the same workload on this host;
+  bounded sample of the same workload on this host, and the single-instance configuration
+  (seed 0, 1000 substeps) in full on one thread;
 * plain_value: the same workload stepped by the plain path (one launch per stage per
   instance, global atomics, unsorted);
 * diff: one differentiable episode of one seeded reference tank (seed 0, ~62K live
   particles, 2000 substeps, checkpoint every 100), in fp32 and fp64: forward through the
   autograd blocks, forward + backward of the payload CoM y with recomputation from the
   checkpoints, the same keeping every block's stored trajectory (no recomputation), and the
-  fused (non-differentiable, fp32) forward of the same episode for comparison;
+  fused (non-differentiable, fp32) forward of the same episode for comparison; and the
+  batched configuration: 32 tall tanks (128x256, H = 2, 160K slots), 4000 substeps,
+  checkpoint every 200, per-instance gradients in one adjoint pass;
 * render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
   map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back;
 * axisym: the axisymmetric variant's rate on the same batch and on the tall view
@@ -27,7 +40,7 @@ Also reported on the same line:
 * tank_build_s: host seconds to build the 32 tanks (phenotype.population, the fill_tank
   restatement with its native steps).
 
-Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 10]
 """
 from __future__ import annotations
 
@@ -46,6 +59,9 @@ METRIC = "particle-substeps/s (32x80K batch, fp32)"
 ALGO_BYTES = 108.0          # per live particle-substep
 HBM_PEAK = 8.0e12           # B/s, MI355X HBM3E
 BLOCK = 100                 # substeps per step
+REPS = 5                    # timed repetitions of K steps (the median is reported)
+MIN_WARM = 1000             # warm-up substeps at least
+EPISODE = 150000            # substeps of one fitness episode (evolve.py:48)
 
 
 def log(msg: str) -> None:
@@ -76,12 +92,14 @@ def load_tuning(eng) -> dict:
 
 
 def traffic_from_profiles():
+    """HBM bytes of one fused substep from the FETCH_SIZE / WRITE_SIZE counter runs
+    (rocprofv3 cannot run inside this process), with the commit and command they were
+    measured with."""
     path = os.path.join(ROOT, "profiles", "fused_kernel_traffic.json")
     if not os.path.exists(path):
-        return None, None
+        return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get("bytes_per_launch"), d.get("live_particles")
+        return json.load(f)
 
 
 def cpu_baseline(pos, mat, fib, genomes, budget_s: float = 12.0) -> dict:
@@ -103,10 +121,22 @@ def cpu_baseline(pos, mat, fib, genomes, budget_s: float = 12.0) -> dict:
     t0 = time.perf_counter()
     orc.run(steps, nthreads=cores)
     dt = time.perf_counter() - t0
+    # the single-instance configuration in full: seed 0, 1000 substeps, one thread
+    one = oracle.OracleSim(1, N)
+    one.load(0, pos[0], mat[0], fib[0])
+    one.contraction[:] = orc.contraction[0]
+    one.freq[:] = orc.freq[0]
+    live1 = int((mat[0] >= 0).sum())
+    t1 = time.perf_counter()
+    one.run(1000, nthreads=1)
+    dt1 = time.perf_counter() - t1
     return {"value": live * steps / dt, "unit": "particle-substeps/s", "cores": cores,
             "kind": "port",
             "sample": f"{M} instances of the same workload ({live} live particles), "
-                      f"{steps} substeps, oracle fp32, one instance per OpenMP thread"}
+                      f"{steps} substeps, oracle fp32, one instance per OpenMP thread",
+            "single_instance_full": {"live_particles": live1, "substeps": 1000,
+                                     "seconds": dt1, "value": live1 * 1000 / dt1,
+                                     "cores": 1}}
 
 
 def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
@@ -137,8 +167,11 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
     eng.close()
     res = {"fused_forward_ms": fused * 1e3}
 
-    for dtype in (torch.float32, torch.float64):
-        tank, st0 = DiffTank.from_tanks(pos[None], mat[None], fib[None], dtype=dtype)
+    for key, dtype, math in (("fp32", torch.float32, "fp32"),
+                             ("fp32_buffers_fp64_math", torch.float32, "fp64"),
+                             ("fp64", torch.float64, "fp64")):
+        tank, st0 = DiffTank.from_tanks(pos[None], mat[None], fib[None], dtype=dtype,
+                                        math=math)
         dev = tank.device
 
         def episode(grad: bool, recompute: bool = True) -> float:
@@ -163,7 +196,7 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
         fb_stored = min(episode(True, recompute=False) for _ in range(2))
         res["workload"] = (f"1 reference tank (seed 0, {tank.P} live), {steps} substeps, "
                            f"checkpoint every {checkpoint}, payload CoM y objective")
-        res["fp32" if dtype == torch.float32 else "fp64"] = {
+        res[key] = {
             "forward_ms": fwd * 1e3, "fwd_bwd_ms": fb * 1e3, "fwd_bwd_stored_ms": fb_stored * 1e3,
             "fwd_bwd_over_forward": fb / fwd, "fwd_bwd_stored_over_forward": fb_stored / fwd,
             "fwd_bwd_over_fused_forward": fb / fused,
@@ -173,14 +206,13 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
 
 
 def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
-                 checkpoint: int = 200) -> dict:
+                 checkpoint: int = 200, resort: bool = False) -> dict:
     """Batched forward + backward at the tall-tank shape: M seeded genomes in 128x256 tanks
     of height 2 (spawn (0.5, 0.60), evolve.py:33-36,136), per-instance gradients of the
     payload CoM y w.r.t. actuation amplitude, contraction and frequency in one adjoint
-    pass; beside the fused forward of the same batch."""
-    import torch
+    pass, with fp32 and with fp64 arithmetic over fp32 buffers; beside the fused forward of
+    the same batch."""
     from jellyfih_amd import phenotype
-    from jellyfih_amd.diff import DiffTank
     from jellyfih_amd.engine import Engine
 
     pos, mat, fib, genomes, infos = phenotype.population(
@@ -198,7 +230,21 @@ def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
     eng.sync()
     fused = time.perf_counter() - t0
     eng.close()
-    tank, st0 = DiffTank.from_tanks(pos, mat, fib, n_grid_y=256, domain_height=2.0)
+    out = {"workload": f"{M} tall tanks 128x256 (H=2, {slots} slots, {live} live), {steps} "
+                       f"substeps, checkpoint every {checkpoint}",
+           "live_particles": live, "fused_forward_s": fused, "fused_rate": live * steps / fused}
+    for key, math in (("fp32", "fp32"), ("fp32_buffers_fp64_math", "fp64")):
+        out[key] = _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused,
+                                      math, resort and key == "fp32")
+    return out
+
+
+def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, math,
+                       resort) -> dict:
+    import torch
+    from jellyfih_amd.diff import DiffTank
+    tank, st0 = DiffTank.from_tanks(pos, mat, fib, n_grid_y=256, domain_height=2.0,
+                                    math=math)
     dev = tank.device
 
     def episode(grad: bool, n: int = steps, resort: int = 0) -> float:
@@ -218,23 +264,21 @@ def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
 
     episode(True, checkpoint)   # warm-up: first launches and the allocator's buffers
     fwd = episode(False)
-    log(f"batched diff forward {fwd:.2f} s")
+    log(f"batched diff forward ({math} arithmetic) {fwd:.2f} s")
     fb = episode(True)
-    log(f"batched diff forward + backward {fb:.2f} s")
-    episode(True, 2 * checkpoint, 1)
-    fwd_rs = episode(False, steps, 1)
-    fb_rs = episode(True, steps, 1)
-    log(f"re-sorted every block: forward {fwd_rs:.2f} s, forward + backward {fb_rs:.2f} s")
+    log(f"batched diff forward + backward ({math} arithmetic) {fb:.2f} s")
+    out = {"forward_s": fwd, "fwd_bwd_s": fb, "fwd_bwd_over_forward": fb / fwd,
+           "fwd_bwd_over_fused_forward": fb / fused}
+    if resort:
+        episode(True, 2 * checkpoint, 1)
+        fwd_rs = episode(False, steps, 1)
+        fb_rs = episode(True, steps, 1)
+        log(f"re-sorted every block: forward {fwd_rs:.2f} s, forward + backward {fb_rs:.2f} s")
+        out["resorted_every_block"] = {"forward_s": fwd_rs, "fwd_bwd_s": fb_rs,
+                                       "fwd_bwd_over_forward": fb_rs / fwd_rs,
+                                       "fwd_bwd_over_fused_forward": fb_rs / fused}
     tank.close()
-    return {"workload": f"{M} tall tanks 128x256 (H=2, {slots} slots, {live} live), {steps} "
-                        f"substeps, checkpoint every {checkpoint}, fp32",
-            "live_particles": live, "fused_forward_s": fused, "forward_s": fwd,
-            "fwd_bwd_s": fb, "fwd_bwd_over_forward": fb / fwd,
-            "fwd_bwd_over_fused_forward": fb / fused,
-            "resorted_every_block": {"forward_s": fwd_rs, "fwd_bwd_s": fb_rs,
-                                     "fwd_bwd_over_forward": fb_rs / fwd_rs,
-                                     "fwd_bwd_over_fused_forward": fb_rs / fused},
-            "fused_rate": live * steps / fused}
+    return out
 
 
 WEB_PALETTE = [(0, 0.933, 0.949, 0.957), (1, 0.678, 0.882, 0.875), (3, 0.976, 0.757, 0.765),
@@ -338,18 +382,68 @@ def axisym_rates(pos, mat, fib, genomes, reps: int = 10, n: int = 8) -> dict:
     return out
 
 
+def episode_run(eng, pos, mat, fib, genomes, live: int) -> dict:
+    """One full fitness episode as evolve.py runs it (load, then run_batch_headless(150000),
+    mpm_sim.py:802-834), timed by wall clock around the synchronising call; then the
+    substep time at the episode's end state against the start (HIP events)."""
+    load_engine(eng, pos, mat, fib, genomes)
+    eng.live_count(0)   # commits the load (uploads, chunk tables)
+    eng.set_time(0.0)
+    start_ms = float(np.mean([eng.timed_block(BLOCK)[0] for _ in range(2)]))
+    load_engine(eng, pos, mat, fib, genomes)
+    eng.live_count(0)   # the load's upload stays outside the timed call, as in the reference
+    eng.sync()
+    t0 = time.perf_counter()
+    res = eng.run_batch_headless(EPISODE)
+    secs = time.perf_counter() - t0
+    end_ms = float(np.mean([eng.timed_block(BLOCK)[0] for _ in range(2)]))
+    return {"substeps": EPISODE, "seconds": secs, "rate": live * EPISODE / secs,
+            "valid": int(np.sum(res[:, 4] == 1.0)), "instances": int(res.shape[0]),
+            "finite": bool(np.isfinite(res).all()),
+            "final_y": [round(float(v), 6) for v in res[:, 2]],
+            "substep_us_start": start_ms * 1e3, "substep_us_end": end_ms * 1e3,
+            "clock_end": eng.get_time()}
+
+
+def batch16(reps: int = 3) -> dict:
+    """The 16-instance batched forward configuration: seeds 1-16, 80K slots, 20,000
+    substeps in 100-substep graph blocks after 1000 warm-up substeps; median of reps."""
+    from jellyfih_amd.engine import Engine
+    pos, mat, fib, genomes, _ = build_workload(16, 80000, range(1, 17))
+    live = int((mat >= 0).sum())
+    eng = Engine(16, 80000)
+    load_engine(eng, pos, mat, fib, genomes)
+    eng.substep(MIN_WARM)
+    eng.sync()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        eng.substep(20000)
+        eng.sync()
+        walls.append(time.perf_counter() - t0)
+    eng.close()
+    wall = float(np.median(walls))
+    return {"instances": 16, "seeds": [1, 16], "live_particles": live, "substeps": 20000,
+            "seconds": wall, "value": live * 20000 / wall}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100, help="timed 100-substep blocks")
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed 100-substep blocks per repetition (5 repetitions)")
     ap.add_argument("--warmup", type=int, default=10, help="untimed 100-substep blocks")
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--slots", type=int, default=80000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
-    ap.add_argument("--diff-batched", action="store_true",
-                    help="also time the batched tall-tank forward + backward (~1 minute)")
+    ap.add_argument("--no-diff-batched", action="store_true",
+                    help="skip the batched tall-tank forward + backward")
+    ap.add_argument("--diff-resort", action="store_true",
+                    help="batched leg: also time re-sorting before every block")
+    ap.add_argument("--no-episode", action="store_true")
+    ap.add_argument("--no-batch16", action="store_true")
     ap.add_argument("--only-diff", action="store_true",
                     help="run only the differentiable legs (no throughput line fields)")
     ap.add_argument("--only-batched", action="store_true",
@@ -375,13 +469,15 @@ def main() -> int:
     assert torch.cuda.is_available(), "no GPU visible"
     if args.only_batched:
         print(json.dumps({"diff": {"batched": diff_batched(
-            steps=args.batched_steps, checkpoint=args.batched_checkpoint)}}), flush=True)
+            steps=args.batched_steps, checkpoint=args.batched_checkpoint,
+            resort=args.diff_resort)}}), flush=True)
         return 0
     if args.only_diff:
         out = {"diff": diff_episode()}
-        if args.diff_batched:
+        if not args.no_diff_batched:
             out["diff"]["batched"] = diff_batched(steps=args.batched_steps,
-                                                  checkpoint=args.batched_checkpoint)
+                                                  checkpoint=args.batched_checkpoint,
+                                                  resort=args.diff_resort)
         print(json.dumps(out), flush=True)
         return 0
     M, N = args.instances, args.slots
@@ -408,8 +504,9 @@ def main() -> int:
     load_engine(eng, pos, mat, fib, genomes)
     tuning = load_tuning(eng)
     log(f"tuning {tuning}")
-    log(f"warmup {args.warmup} blocks")
-    eng.substep(args.warmup * BLOCK)
+    warm = max(args.warmup * BLOCK, MIN_WARM)
+    log(f"warmup {warm} substeps")
+    eng.substep(warm)
     eng.sync()
 
     if args.only_render:
@@ -417,39 +514,75 @@ def main() -> int:
         return 0
 
     stream = torch.cuda.ExternalStream(eng.stream)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    eng.sync()
-    log(f"timing {args.steps} blocks")
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        eng.substep(BLOCK)
-    ev1.record(stream)
-    eng.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_s = ev0.elapsed_time(ev1) / 1e3
     substeps = args.steps * BLOCK
+    walls, events = [], []
+    for r in range(REPS):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        eng.sync()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            eng.substep(BLOCK)
+        ev1.record(stream)
+        eng.sync()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        events.append(ev0.elapsed_time(ev1) / 1e3)
+        log(f"rep {r}: {args.steps} blocks wall {walls[-1]:.4f}s events {events[-1]:.4f}s "
+            f"-> {live * substeps / walls[-1]:.3e} particle-substeps/s")
+    med = int(np.argsort(walls)[len(walls) // 2])
+    wall, gpu_s = walls[med], events[med]
     value = live * substeps / wall
-    log(f"wall {wall:.3f}s events {gpu_s:.3f}s -> {value:.3e} particle-substeps/s")
+    log(f"median of {REPS}: wall {wall:.4f}s -> {value:.3e} particle-substeps/s")
 
-    # fused-substep duration with HIP events (3 blocks, not graph-captured): the wall time
-    # of one substep of all instances, whose instance groups' launches run concurrently
-    fused = [eng.timed_block(BLOCK) for _ in range(3)]
-    groups = eng.tuning()["instance_groups"]
-    fused_ms = float(np.mean([f[0] for f in fused]))
-    achieved = ALGO_BYTES * live / (fused_ms * 1e-3)
-    traffic, traffic_live = traffic_from_profiles()
-    if traffic is not None and traffic_live and traffic_live != live:
-        traffic = None
+    # Roofline. One launch per substep over every live particle (instance_groups = 1):
+    # average kernel duration from HIP events on the engine's stream around n - 1 fused
+    # G2P->P2G kernels (3 blocks, launched directly); then the tuned concurrent groups'
+    # wall time per substep, event-timed the same way.
+    eng.set_tuning(instance_groups=1)
+    single = [eng.timed_block(BLOCK)[0] for _ in range(3)]
+    eng.set_tuning(instance_groups=tuning["instance_groups"])
+    concurrent = [eng.timed_block(BLOCK)[0] for _ in range(3)]
+    single_ms = float(np.mean(single))
+    conc_ms = float(np.mean(concurrent))
+    algo = ALGO_BYTES * live
+    achieved = algo / (single_ms * 1e-3)
+    tr = traffic_from_profiles()
+    traffic = tr.get("bytes_per_launch") if tr and tr.get("live_particles") == live else None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "kernel": f"k_fused<G2P,P2G>, one substep = {groups} concurrent "
-                          f"instance-group launches", "kernel_ms": fused_ms,
-                "algorithmic_bytes_per_launch": ALGO_BYTES * live}
-    log(f"fused kernel {fused_ms * 1e3:.1f} us -> {achieved / 1e9:.0f} GB/s algorithmic")
+                "kernel": "k_fused<256,4,G2P,P2G>, one launch per substep over all "
+                          f"{live} live particles (instance_groups=1)",
+                "kernel_ms": single_ms, "algorithmic_bytes_per_launch": algo,
+                "traffic_frac": (traffic / (single_ms * 1e-3) / HBM_PEAK) if traffic else None,
+                "traffic_measured": ({k: tr.get(k) for k in ("commit", "command", "source")}
+                                     if traffic else None),
+                "event_timed_concurrent": {
+                    "instance_groups": tuning["instance_groups"],
+                    "substep_ms": conc_ms, "achieved": algo / (conc_ms * 1e-3) / 1e9,
+                    "frac": algo / (conc_ms * 1e-3) / HBM_PEAK,
+                    "note": "wall time of one substep whose instance groups' launches run "
+                            "concurrently on two streams (HIP events on the main stream)"}}
+    log(f"fused kernel, one launch {single_ms * 1e3:.1f} us -> {achieved / 1e9:.0f} GB/s "
+        f"algorithmic; concurrent groups {conc_ms * 1e3:.1f} us per substep")
+
+    episode = None
+    if not args.no_episode:
+        log(f"one fitness episode: run_batch_headless({EPISODE})")
+        episode = episode_run(eng, pos, mat, fib, genomes, live)
+        log(f"episode {episode['seconds']:.2f} s, {episode['rate']:.3e}, valid "
+            f"{episode['valid']}/{episode['instances']}, substep "
+            f"{episode['substep_us_start']:.1f} -> {episode['substep_us_end']:.1f} us")
+        eng.substep(warm)
+        eng.sync()
+
+    b16 = None
+    if not args.no_batch16:
+        log("16-instance batched forward (seeds 1-16, 20000 substeps)")
+        b16 = batch16()
+        log(f"batch16 {b16['value']:.3e}")
 
     log("renderer (one video frame per pipeline)")
     rend = render_frames(eng, live)
@@ -479,9 +612,10 @@ def main() -> int:
         log("differentiable episode (forward, forward + backward)")
         diff = diff_episode()
         log(f"diff {diff}")
-    if args.diff_batched:
+    if not args.no_diff_batched:
         diff = dict(diff or {}, batched=diff_batched(steps=args.batched_steps,
-                                                     checkpoint=args.batched_checkpoint))
+                                                     checkpoint=args.batched_checkpoint,
+                                                     resort=args.diff_resort))
         log(f"diff batched {diff['batched']}")
 
     cpu = None
@@ -497,6 +631,9 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_substeps": warm,
+        "repetitions": REPS,
+        "rep_values": [live * substeps / w for w in walls],
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -504,8 +641,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic: seeded genomes -> fill_tank restatement (reference 4-per-cell "
                 "lattice, no jitter)",
-        "config": {"workload": f"{M} instances x {N} slots, 128x128 grids, one fitness "
-                               f"episode's substeps in {BLOCK}-substep graph blocks",
+        "config": {"workload": f"{M} instances x {N} slots, 128x128 grids; a step is one "
+                               f"{BLOCK}-substep graph block of the {EPISODE}-substep "
+                               "fitness episode",
                    "instances": M, "slots_per_instance": N, "live_particles": live,
                    "grid": [128, 128], "substeps_per_step": BLOCK, "tuning": tuning},
         "slot_rate": M * N * substeps / wall,
@@ -515,6 +653,8 @@ def main() -> int:
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "episode": episode,
+        "batch16": b16,
         "diff": diff,
         "render": rend,
         "axisym": axisym,

@@ -250,7 +250,8 @@ int jm_points_in_polygon_grid(const double* poly, int64_t n, const double* xs, i
  * No counterpart in the reference (it has no autodiff). A batched substep and its adjoint
  * over caller-owned device buffers (e.g. torch tensors), P particles in any order, in fp32
  * or fp64 (chosen at jm_diff_create; every scalar buffer below is of that type):
- *   state (P,12) per particle: x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11;
+ *   state (P,12) per particle: x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11 (with
+ *   JM_DIFF_MIXED the last four are F - I: F00-1 F01 F10 F11-1);
  *   inputs: inst (P,) int32 instance index, mat (P,) int32, fiber (P,2), amp_scale (P,)
  *   (per-particle multiplier on instance_actuation), params (M,5) = instance_actuation,
  *   instance_act_contraction, instance_freq, instance_payload_density, E_jelly.
@@ -264,8 +265,15 @@ typedef struct jm_diff_inputs {
   const void* params;
 } jm_diff_inputs;
 
-/* Grid buffers for cfg's grid and n_instances, and scratch for n_particles particles;
- * scalar_bits = 32 (fp32, the reference's precision) or 64 (fp64). */
+/* Grid buffers for cfg's grid and n_instances, and scratch for n_particles particles.
+ * scalar_bits = 32: fp32 buffers and arithmetic (the reference's precision); 64: fp64 buffers
+ * and arithmetic; JM_DIFF_MIXED: fp32 buffers (states, grids, inputs, gradients; every
+ * "scalar" below is fp32) with each substep evaluated, and the adjoint carried between
+ * substeps, in fp64, and the states' F columns holding F - I (its adjoint is F's). In fp32
+ * arithmetic the small strains near F = I (the jelly's F - R, the water's J - 1 at
+ * stiffness 1e5) keep only a few digits, which costs gradients percents over a few hundred
+ * substeps; fp32 storage of the results, with F stored as F - I, costs ~1e-5 there. */
+#define JM_DIFF_MIXED 3264
 int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bits,
                    jm_diff** out);
 int jm_diff_destroy(jm_diff* d);

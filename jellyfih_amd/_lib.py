@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(PKG, "libjellymx.so")
 
 JM_ABI_VERSION = 4
 JM_NOALLOC_SCRATCH_BYTES = 32 << 20
+JM_DIFF_MIXED = 3264   # jm_diff_create: fp32 buffers, fp64 arithmetic
 
 # jm_status
 JM_OK, JM_ERR_ARG, JM_ERR_HIP, JM_ERR_STATE, JM_ERR_NOMEM = range(5)

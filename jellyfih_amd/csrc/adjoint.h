@@ -1,8 +1,9 @@
 // Differentiable substep: forward and reverse-mode (adjoint) math per particle and per node.
 //
 // Templated on the scalar type and usable on the host and the device: the HIP kernels in
-// diff_kernels.h instantiate it in fp32; the same functions compiled for the host in fp64
-// let the CPU tests compare the adjoint with central differences of the independent oracle.
+// diff_kernels.h instantiate it in fp32 and fp64 (fp64 also over fp32 buffers: state and
+// grid loads widen, stores round); the same functions compiled for the host in fp64 let
+// the CPU tests compare the adjoint with central differences of the independent oracle.
 //
 // Forward semantics follow the reference substep (c-m-x-r/jellyfih mpm_sim.py:153-340) with
 // two parameters promoted to runtime values (jelly Young's modulus per instance, and a
@@ -475,15 +476,22 @@ struct PState {
   Mat<T> C, F;
 };
 
-template <typename T>
-JMD_HD PState<T> load_state(const T* s) {
-  return {s[0], s[1], s[2], s[3], {s[4], s[5], s[6], s[7]}, {s[8], s[9], s[10], s[11]}};
+// A state of arithmetic type A from a buffer of type S (fp32 buffers with fp64 arithmetic:
+// exact widening on load, one rounding on store).
+template <typename A, typename S>
+JMD_HD PState<A> load_state_as(const S* s) {
+  return {A(s[0]), A(s[1]), A(s[2]), A(s[3]), {A(s[4]), A(s[5]), A(s[6]), A(s[7])},
+          {A(s[8]), A(s[9]), A(s[10]), A(s[11])}};
 }
 template <typename T>
-JMD_HD void store_state(T* s, const PState<T>& p) {
-  s[0] = p.x0; s[1] = p.x1; s[2] = p.v0; s[3] = p.v1;
-  s[4] = p.C.a; s[5] = p.C.b; s[6] = p.C.c; s[7] = p.C.d;
-  s[8] = p.F.a; s[9] = p.F.b; s[10] = p.F.c; s[11] = p.F.d;
+JMD_HD PState<T> load_state(const T* s) {
+  return load_state_as<T>(s);
+}
+template <typename S, typename A>
+JMD_HD void store_state(S* s, const PState<A>& p) {
+  s[0] = S(p.x0); s[1] = S(p.x1); s[2] = S(p.v0); s[3] = S(p.v1);
+  s[4] = S(p.C.a); s[5] = S(p.C.b); s[6] = S(p.C.c); s[7] = S(p.C.d);
+  s[8] = S(p.F.a); s[9] = S(p.F.b); s[10] = S(p.F.c); s[11] = S(p.F.d);
 }
 
 // P2G of one particle (mpm_sim.py:183-271): updates F in `p` and hands the momentum and
@@ -513,11 +521,11 @@ JMD_HD bool p2g_particle(PState<T>& p, int mat, T d0, T d1, T beta, T mu_j, T la
 }
 
 // Velocity of node (i, j) after the grid op, from its momentum and mass planes.
-template <typename T>
-JMD_HD void node_velocity(const T* gp0, const T* gp1, const T* gm, int i, int j,
+template <typename T, typename S>
+JMD_HD void node_velocity(const S* gp0, const S* gp1, const S* gm, int i, int j,
                           const Consts<T>& k, T& a0, T& a1) {
   const int node = j * k.nx + i;
-  gridop_forward(gp0[node], gp1[node], gm[node], i, j, k, a0, a1);
+  gridop_forward(T(gp0[node]), T(gp1[node]), T(gm[node]), i, j, k, a0, a1);
 }
 
 // What the G2P adjoint needs of one node: its velocity after the grid op (v0, v1) and the
@@ -589,9 +597,9 @@ JMD_HD void g2p_particle_v(PState<T>& p, const Consts<T>& k, Vel&& vel) {
 }
 
 // The same, with the grid op evaluated per stencil node from momentum gp0, gp1, mass gm.
-template <typename T>
-JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const T* gp0, const T* gp1,
-                         const T* gm) {
+template <typename T, typename S>
+JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const S* gp0, const S* gp1,
+                         const S* gm) {
   g2p_particle_v(p, k, [&](int i, int j, T& a0, T& a1) {
     node_velocity(gp0, gp1, gm, i, j, k, a0, a1);
   });
@@ -663,15 +671,15 @@ JMD_HD void g2p_backward_c(const PState<T>& s, const PState<T>& b_out, const Con
 }
 
 // The same, with each node's NodeCoef computed from momentum gp0, gp1 and mass gm.
-template <typename T, typename Emit>
+template <typename T, typename S, typename Emit>
 JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
-                         const T* gp0, const T* gp1, const T* gm, PState<T>& b_in,
+                         const S* gp0, const S* gp1, const S* gm, PState<T>& b_in,
                          Emit&& emit) {
   g2p_backward_c(
       s, b_out, k,
       [&](int i, int j) {
         const int node = j * k.nx + i;
-        return node_coef(gp0[node], gp1[node], gm[node], i, j, k);
+        return node_coef(T(gp0[node]), T(gp1[node]), T(gm[node]), i, j, k);
       },
       b_in, emit);
 }

@@ -16,7 +16,15 @@
 // their contributions first (groups of 4 in the fp32 P2G, pairs elsewhere), so only one
 // lane per group issues atomics. The fp32 P2G tile accumulates in 64-bit fixed point
 // (exact); the fp64 P2G tile and the adjoint tile (whose values have no fixed range) in
-// fp64 (gfx950's fp64 LDS atomic add is much faster than its fp32 one). Every
+// fp64 (gfx950's fp64 LDS atomic add is much faster than its fp32 one).
+//
+// Kernels take two scalar types: T, of the buffers (states, grids, inputs, gradients), and
+// A, of the arithmetic inside a substep. T = A = float is the reference's precision;
+// T = A = double is fp64 throughout; T = float, A = double keeps fp32 buffers (fp32 states
+// and forward grids, as in the reference) but evaluates each substep, and carries the
+// adjoint between kernels, in fp64: small strains (F - R of the jelly, J - 1 of the water,
+// whose stiffness is 1e5) are differences of numbers near 1 that fp32 arithmetic resolves
+// to only ~1e-7, while fp32 storage of the results costs the gradients little. Every
 // grid a kernel accumulates into starts zeroed: grids rotate through three buffers and
 // each kernel clears the one the kernel after next accumulates into (grid-stride), or a
 // fill kernel clears them at the start of a block.
@@ -45,23 +53,23 @@ struct DiffInputs {
 // The per-instance parameters a particle of material `mt` uses: jelly/muscle Lame
 // parameters from E_jelly (nu = 0.3), the actuation waveform and amplitude for muscle, the
 // payload density for payload (zeros otherwise: water, the majority, needs none of them).
-template <typename T>
-__device__ __forceinline__ void inst_params(const DiffInputs<T>& in, int m, int mt, T t,
-                                            const Consts<T>& k, Act<T>& act, T& mu, T& la,
-                                            T& amp, T& pdens) {
+template <typename A, typename T>
+__device__ __forceinline__ void inst_params(const DiffInputs<T>& in, int m, int mt, A t,
+                                            const Consts<A>& k, Act<A>& act, A& mu, A& la,
+                                            A& amp, A& pdens) {
   const T* pr = in.params + 5 * m;
-  act = {T(0), T(0), T(0)};
-  mu = la = amp = pdens = T(0);
+  act = {A(0), A(0), A(0)};
+  mu = la = amp = pdens = A(0);
   if (mt == 1 || mt == 3) {
-    const T E = pr[4], nu = k.nu_jelly;
-    mu = E / (T(2) * (T(1) + nu));
-    la = E * nu / ((T(1) + nu) * (T(1) - T(2) * nu));
+    const A E = A(pr[4]), nu = k.nu_jelly;
+    mu = E / (A(2) * (A(1) + nu));
+    la = E * nu / ((A(1) + nu) * (A(1) - A(2) * nu));
   }
   if (mt == 3) {
-    act = activation(t, pr[2], pr[1], k);
-    amp = pr[0];
+    act = activation(t, A(pr[2]), A(pr[1]), k);
+    amp = A(pr[0]);
   }
-  if (mt == 2) pdens = pr[3];
+  if (mt == 2) pdens = A(pr[3]);
 }
 
 template <typename T>
@@ -181,19 +189,19 @@ __device__ __forceinline__ double next_lane(double v) {
 }
 
 
-// fp32 P2G of one particle per lane into the workgroup's LDS tile (3 planes of kTileNodes
-// 64-bit fixed-point sums, jm::fixed_bits) and its flush, called by every thread of the
-// workgroup: the same contributions as p2g_particle, w (m v + A (x_node - x)) and w m,
-// evaluated in the separable form wy_j P_i + ey_j Q_i (P_i = wx_i (m v + A_0 dx_i),
+// fp32 P2G scatter of one particle per lane into the workgroup's LDS tile (3 planes of
+// kTileNodes 64-bit fixed-point sums, jm::fixed_bits) and its flush, called by every thread
+// of the workgroup, from the particle's mass, affine term A and velocity (p2g_forward's
+// results, rounded to fp32): the contributions of p2g_particle, w (m v + A (x_node - x)) and
+// w m, evaluated in the separable form wy_j P_i + ey_j Q_i (P_i = wx_i (m v + A_0 dx_i),
 // Q_i = wx_i A_1, ey_j = wy_j dy_j), then summed over same-cell groups of 4 lanes
 // (jm::lane_groups / presum3) so that one lane per group issues the atomics. A particle
 // whose contribution bounds reach 0.75 (or are NaN), or any particle when the tile
 // is not used, scatters to the global grid with fp32 atomics and then carries zeros; lanes
 // without a particle carry zeros computed from finite stand-ins, so every lane's values are
-// finite in the pre-sums. Updates p.F for in-range particles.
-__device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active, int mt, float d0,
-                                                  float d1, float beta, float mu, float la,
-                                                  float pdens, const Consts<float>& k, int b0,
+// finite in the pre-sums.
+__device__ __forceinline__ void p2g_scatter_fixed(bool active, float mass, const Mat<float>& A,
+                                                  float v0, float v1, float dx, int nx, int b0,
                                                   int b1, float fx0, float fx1, const Tile& tl,
                                                   double* lds, float* gi, float* gtile, int G) {
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds);
@@ -206,22 +214,15 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
     }
     __syncthreads();
   }
-  float mass = 0.f;
-  Mat<float> A = {0.f, 0.f, 0.f, 0.f};
-  if (active) {
-    Mat<float> Fn;
-    p2g_forward(mt, p.F, p.C, d0, d1, beta, mu, la, pdens, k, Fn, A, mass);
-    p.F = Fn;
-  }
   const float f0 = active ? fx0 : 0.5f, f1 = active ? fx1 : 0.5f;
-  const float mv0 = mass * (active ? p.v0 : 0.f), mv1 = mass * (active ? p.v1 : 0.f);
+  const float mv0 = mass * (active ? v0 : 0.f), mv1 = mass * (active ? v1 : 0.f);
   float wx[3], wy[3], dw[3], ey[3], P0[3], Q0[3], P1[3], Q1[3], Mi[3];
   weights(f0, wx, dw);
   weights(f1, wy, dw);
   // Each of the 27 contributions is at most 0.5625 times these bounds and up to 4 lanes add
   // before one encode, so values stay inside fixed_bits' exact range (|c| < 2) with a
   // bound of 0.75. No fmaxf: it drops NaN operands, and each comparison fails on NaN.
-  const float bnd = 1.5f * k.dx;
+  const float bnd = 1.5f * dx;
   const bool lds_ok = active && tl.use &&
                       fabsf(mv0) + bnd * (fabsf(A.a) + fabsf(A.b)) < 0.75f &&
                       fabsf(mv1) + bnd * (fabsf(A.c) + fabsf(A.d)) < 0.75f && mass < 0.75f &&
@@ -230,7 +231,7 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
 #pragma unroll
   for (int ii = 0; ii < 3; ++ii) {
     const float ai = wx[ii] * sc;
-    const float bi = ai * ((float(ii) - f0) * k.dx);
+    const float bi = ai * ((float(ii) - f0) * dx);
     P0[ii] = ai * mv0 + bi * A.a;
     Q0[ii] = ai * A.b;
     P1[ii] = ai * mv1 + bi * A.c;
@@ -238,11 +239,11 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
     Mi[ii] = ai * mass;
   }
 #pragma unroll
-  for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * ((float(jj) - f1) * k.dx);
+  for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * ((float(jj) - f1) * dx);
   if (active && !lds_ok) {
     for (int jj = 0; jj < 3; ++jj)
       for (int ii = 0; ii < 3; ++ii) {
-        const int node = (b1 + jj) * k.nx + (b0 + ii);
+        const int node = (b1 + jj) * nx + (b0 + ii);
         unsafeAtomicAdd(gi + node, wy[jj] * P0[ii] + ey[jj] * Q0[ii]);
         unsafeAtomicAdd(gi + G + node, wy[jj] * P1[ii] + ey[jj] * Q1[ii]);
         unsafeAtomicAdd(gi + 2 * (size_t)G + node, wy[jj] * Mi[ii]);
@@ -254,7 +255,7 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
     }
   }
   const jm::LaneGroups lg =
-      jm::lane_groups(lds_ok ? b1 * k.nx + b0 + 1 : -(1 + (int)(threadIdx.x & 63)), lds_ok);
+      jm::lane_groups(lds_ok ? b1 * nx + b0 + 1 : -(1 + (int)(threadIdx.x & 63)), lds_ok);
   const int q0 = (b1 - tl.oy) * tl.tw + (b0 - tl.ox);
 #pragma unroll
   for (int jj = 0; jj < 3; ++jj) {
@@ -275,7 +276,7 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
     __syncthreads();
     for (int li = threadIdx.x; li < n; li += kDiffBlock) {
       const int jj = li / tl.tw;
-      const int node = (tl.oy + jj) * k.nx + tl.ox + (li - jj * tl.tw);
+      const int node = (tl.oy + jj) * nx + tl.ox + (li - jj * tl.tw);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const float v = jm::from_fixed_bits(acc[c * kTileNodes + li]);
@@ -293,21 +294,57 @@ __device__ __forceinline__ void p2g_scatter_fixed(PState<float>& p, bool active,
 // and F of that state) plus F_new, the F that substep s's P2G computed (the F of state
 // s+1). K_f(0) is P2G only, K_f(n) G2P only.
 
-template <typename T>
-__device__ __forceinline__ void load_xvc(const T* p, PState<T>& s) {
-  s.x0 = p[0]; s.x1 = p[1]; s.v0 = p[2]; s.v1 = p[3];
-  s.C = {p[4], p[5], p[6], p[7]};
+// State buffers of fp32 with fp64 arithmetic (sizeof(T) < sizeof(A)) hold the deformation
+// gradient as F - I: near the identity fp32 keeps F - I to its own relative precision where
+// F itself would round it to ~6e-8 absolute, and a strain of 1e-5 to two digits.
+template <typename T, typename A>
+constexpr bool kShiftF = sizeof(T) < sizeof(A);
+
+template <typename A, typename T>
+__device__ __forceinline__ Mat<A> load_F(const T* f) {
+  const A one = kShiftF<T, A> ? A(1) : A(0);
+  return {A(f[0]) + one, A(f[1]), A(f[2]), A(f[3]) + one};
 }
 
-template <typename T, bool DO_G2P, bool DO_P2G>
+template <typename T, typename A>
+__device__ __forceinline__ void store_F(T* f, const Mat<A>& F) {
+  const A one = kShiftF<T, A> ? A(1) : A(0);
+  f[0] = T(F.a - one);
+  f[1] = T(F.b);
+  f[2] = T(F.c);
+  f[3] = T(F.d - one);
+}
+
+// A particle state from a state buffer (F as stored: see kShiftF), and back.
+template <typename A, typename T>
+__device__ __forceinline__ PState<A> load_particle(const T* p) {
+  PState<A> s = load_state_as<A>(p);
+  s.F = load_F<A>(p + 8);
+  return s;
+}
+
+template <typename T, typename A>
+__device__ __forceinline__ void store_particle(T* p, const PState<A>& s) {
+  store_state(p, s);
+  store_F(p + 8, s.F);
+}
+
+template <typename A, typename T>
+__device__ __forceinline__ void load_xvc(const T* p, PState<A>& s) {
+  s.x0 = A(p[0]); s.x1 = A(p[1]); s.v0 = A(p[2]); s.v1 = A(p[3]);
+  s.C = {A(p[4]), A(p[5]), A(p[6]), A(p[7])};
+}
+
+template <typename T, typename A, bool DO_G2P, bool DO_P2G>
 __global__ void __launch_bounds__(kDiffBlock)
 k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T* dst,
-        T* fnew_out, T* grid_cur, T* zero, int64_t zero_n, int P, int G, T t, Consts<T> k) {
+        T* fnew_out, T* grid_cur, T* zero, int64_t zero_n, int P, int G, A t, Consts<A> k) {
+  static_assert(sizeof(A) >= sizeof(T), "arithmetic at least as wide as the buffers");
   __shared__ double lds[3 * kTileNodes];
   zero_span(zero, zero_n);
   const int p = blockIdx.x * kDiffBlock + threadIdx.x;
   const bool valid = p < P;
-  PState<T> s = {T(0), T(0), T(0), T(0), {T(1), T(0), T(0), T(1)}, {T(1), T(0), T(0), T(1)}};
+  PState<A> s = {A(0), A(0), A(0), A(0), {A(1), A(0), A(0), A(1)}, {A(1), A(0), A(0), A(1)}};
   int mt = -1, m = 0;
   if (valid) {
     mt = in.mat[p];
@@ -315,19 +352,18 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
     if (DO_G2P) {
       // G2P of substep s-1: x, v, C of state s-1, F = F_new of substep s-1
       load_xvc(src + 12 * (size_t)p, s);
-      const T* fn = fnew_in + 4 * (size_t)p;
-      s.F = {fn[0], fn[1], fn[2], fn[3]};
+      s.F = load_F<A>(fnew_in + 4 * (size_t)p);
     } else {
-      s = load_state(src + 12 * (size_t)p);
+      s = load_particle<A>(src + 12 * (size_t)p);
     }
   }
   if constexpr (DO_G2P) {
     // Node velocities of the workgroup's window, grid op applied once per node, in LDS.
     int b0 = 0, b1 = 0;
-    T fx0, fx1;
+    A fx0, fx1;
     const bool ing = valid && mt >= 0 && base_cell(s.x0, s.x1, k, b0, b1, fx0, fx1);
     const Tile wg = block_tile(ing, m, b0, b1);
-    T* vw = reinterpret_cast<T*>(lds);
+    A* vw = reinterpret_cast<A*>(lds);  // 2 x kTileNodes values of A fit the tile's bytes
     const int wn = wg.tw * wg.th;
     if (wg.use) {
       const T* gp = grid_prev + (size_t)wg.inst * 3 * G;
@@ -335,14 +371,14 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
         const int jj = li / wg.tw;
         const int i = wg.ox + (li - jj * wg.tw), j = wg.oy + jj;
         const int node = j * k.nx + i;
-        gridop_forward(gp[node], gp[G + node], gp[2 * (size_t)G + node], i, j, k, vw[li],
-                       vw[wn + li]);
+        gridop_forward(A(gp[node]), A(gp[G + node]), A(gp[2 * (size_t)G + node]), i, j, k,
+                       vw[li], vw[wn + li]);
       }
       __syncthreads();
     }
     if (valid && mt >= 0) {
       if (wg.use) {
-        g2p_particle_v(s, k, [&](int i, int j, T& a0, T& a1) {
+        g2p_particle_v(s, k, [&](int i, int j, A& a0, A& a1) {
           const int li = (j - wg.oy) * wg.tw + (i - wg.ox);
           a0 = vw[li];
           a1 = vw[wn + li];
@@ -352,29 +388,41 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
         g2p_particle(s, k, g, g + G, g + 2 * (size_t)G);
       }
     }
-    if (valid) store_state(dst + 12 * (size_t)p, s);
+    if (valid) store_particle(dst + 12 * (size_t)p, s);
     if (DO_P2G && wg.use) __syncthreads();  // the P2G tile reuses the window's LDS
   }
   if constexpr (DO_P2G) {
     int b0 = 0, b1 = 0;
-    T fx0 = T(0.5), fx1 = T(0.5);
+    A fx0 = A(0.5), fx1 = A(0.5);
     const bool active = valid && mt >= 0 && base_cell(s.x0, s.x1, k, b0, b1, fx0, fx1);
     const Tile tl = block_tile(active, m, b0, b1);
-    Act<T> act = {T(0), T(0), T(0)};
-    T mu = T(0), la = T(0), amp = T(0), pdens = T(0), d0 = T(0), d1 = T(0), as = T(0);
+    Act<A> act = {A(0), A(0), A(0)};
+    A mu = A(0), la = A(0), amp = A(0), pdens = A(0), d0 = A(0), d1 = A(0), as = A(0);
     if (active) {
       inst_params(in, m, mt, t, k, act, mu, la, amp, pdens);
       if (mt == 3) {  // fibre and multiplier matter for muscle only
-        d0 = in.fiber[2 * p];
-        d1 = in.fiber[2 * p + 1];
-        as = in.amp_scale[p];
+        d0 = A(in.fiber[2 * p]);
+        d1 = A(in.fiber[2 * p + 1]);
+        as = A(in.amp_scale[p]);
       }
     }
     const int n = tl.tw * tl.th;
     T* gi = grid_cur + (size_t)m * 3 * G;
     if constexpr (sizeof(T) == 4) {
-      p2g_scatter_fixed(s, active, mt, d0, d1, act.a * amp * as, mu, la, pdens, k, b0, b1,
-                        fx0, fx1, tl, lds, gi, grid_cur + (size_t)tl.inst * 3 * G, G);
+      // the particle update in A, the scatter of its fp32-rounded results
+      float mass = 0.f;
+      Mat<float> Af = {0.f, 0.f, 0.f, 0.f};
+      if (active) {
+        Mat<A> Fn, Aa;
+        A ma;
+        p2g_forward(mt, s.F, s.C, d0, d1, act.a * amp * as, mu, la, pdens, k, Fn, Aa, ma);
+        s.F = Fn;
+        mass = float(ma);
+        Af = {float(Aa.a), float(Aa.b), float(Aa.c), float(Aa.d)};
+      }
+      p2g_scatter_fixed(active, mass, Af, float(s.v0), float(s.v1), float(k.dx), k.nx, b0,
+                        b1, float(fx0), float(fx1), tl, lds, gi,
+                        grid_cur + (size_t)tl.inst * 3 * G, G);
     } else {
       // fp64: an fp64 LDS tile (fixed point would be coarser than fp64 rounding), lane pairs
       const GlobalEmit<T> direct{gi, k.nx, G};
@@ -386,9 +434,9 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
       }
       if (active)
         p2g_particle(s, mt, d0, d1, act.a * amp * as, mu, la, pdens, k,
-                     [&](int i, int j, T c0, T c1, T c2) {
+                     [&](int i, int j, A c0, A c1, A c2) {
                        if (tl.use) {
-                         const T n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
+                         const A n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
                          if (pr.same_next) {
                            c0 += n0;
                            c1 += n1;
@@ -401,7 +449,7 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
                            atomicAdd(&lds[2 * n + li], double(c2));
                          }
                        } else {
-                         direct(i, j, c0, c1, c2);
+                         direct(i, j, T(c0), T(c1), T(c2));
                        }
                      });
       if (tl.use) {
@@ -409,13 +457,7 @@ k_d_fwd(DiffInputs<T> in, const T* src, const T* fnew_in, const T* grid_prev, T*
         tile_flush(lds, tl, grid_cur + (size_t)tl.inst * 3 * G, k.nx, G);
       }
     }
-    if (valid) {
-      T* fo = fnew_out + 4 * (size_t)p;
-      fo[0] = s.F.a;  // p2g_particle leaves F unchanged outside the transfer range
-      fo[1] = s.F.b;
-      fo[2] = s.F.c;
-      fo[3] = s.F.d;
-    }
+    if (valid) store_F(fnew_out + 4 * (size_t)p, s.F);  // unchanged outside the range
   }
 }
 
@@ -457,14 +499,15 @@ __device__ __forceinline__ void add_param_adjoints(const double g[4], bool has, 
 // of a_s (4 scalars). K_b(n) is the G2P adjoint only (driven by grad_out), K_b(0) the P2G
 // adjoint only (writing grad_in).
 
-template <typename T, bool DO_P2GB, bool DO_G2PB>
+template <typename T, typename A, bool DO_P2GB, bool DO_G2PB>
 __global__ void __launch_bounds__(kDiffBlock)
-k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int aF_stride,
-        const T* m_in, const T* agrid_cur, const T* grid_prev, T* agrid_prev,
-        const T* a_full_in, T* m_out, T* aF_out, T* a_full_out, T* zero, int64_t zero_n,
-        int P, int G, T t, Consts<T> k, T* gamp, double* gparams) {
+k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const A* aF_in, int aF_stride,
+        const A* m_in, const A* agrid_cur, const T* grid_prev, A* agrid_prev,
+        const T* a_full_in, A* m_out, A* aF_out, T* a_full_out, A* zero, int64_t zero_n,
+        int P, int G, A t, Consts<A> k, T* gamp, double* gparams) {
+  static_assert(sizeof(A) >= sizeof(T), "arithmetic at least as wide as the buffers");
   __shared__ double acc[3 * kWinB];  // G2P-adjoint scatter tile
-  __shared__ T cw[6 * kWinB];        // staged NodeCoef of the window
+  __shared__ A cw[6 * kWinB];        // staged NodeCoef of the window
   zero_span(zero, zero_n);
   const int p = blockIdx.x * kDiffBlock + threadIdx.x;
   const bool valid = p < P;
@@ -474,37 +517,37 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
     m = in.inst[p];
   }
   // a: the adjoint of the state between the two halves (state s)
-  PState<T> a = {T(0), T(0), T(0), T(0), {T(0), T(0), T(0), T(0)}, {T(0), T(0), T(0), T(0)}};
+  PState<A> a = {A(0), A(0), A(0), A(0), {A(0), A(0), A(0), A(0)}, {A(0), A(0), A(0), A(0)}};
   double gp4[4] = {0.0, 0.0, 0.0, 0.0};  // this particle's share of (E_jelly, amp, c, f)
   if (DO_P2GB) {
     if (valid) {
-      const T* mi = m_in + 8 * (size_t)p;
+      const A* mi = m_in + 8 * (size_t)p;
       a.x0 = mi[0]; a.x1 = mi[1]; a.v0 = mi[2]; a.v1 = mi[3];
       a.C = {mi[4], mi[5], mi[6], mi[7]};
-      const T* fb = aF_in + (size_t)aF_stride * p;
-      const Mat<T> Fnb = {fb[0], fb[1], fb[2], fb[3]};
+      const A* fb = aF_in + (size_t)aF_stride * p;
+      const Mat<A> Fnb = {fb[0], fb[1], fb[2], fb[3]};
       if (mt >= 0) {
-        const PState<T> sc = load_state(st_cur + 12 * (size_t)p);
-        Act<T> act;
-        T mu, la, amp, pdens;
+        const PState<A> sc = load_particle<A>(st_cur + 12 * (size_t)p);
+        Act<A> act;
+        A mu, la, amp, pdens;
         inst_params(in, m, mt, t, k, act, mu, la, amp, pdens);
-        const T* ag = agrid_cur + (size_t)m * 3 * G;
-        T mub = T(0), lab = T(0), betab = T(0);
-        T as = T(0), d0 = T(0), d1 = T(0);
+        const A* ag = agrid_cur + (size_t)m * 3 * G;
+        A mub = A(0), lab = A(0), betab = A(0);
+        A as = A(0), d0 = A(0), d1 = A(0);
         if (mt == 3) {  // fibre and multiplier matter for muscle only
-          as = in.amp_scale[p];
-          d0 = in.fiber[2 * p];
-          d1 = in.fiber[2 * p + 1];
+          as = A(in.amp_scale[p]);
+          d0 = A(in.fiber[2 * p]);
+          d1 = A(in.fiber[2 * p + 1]);
         }
         p2g_backward_particle(sc, mt, d0, d1, act.a * amp * as, mu, la, pdens, k, ag, ag + G,
                               ag + 2 * (size_t)G, Fnb, a, mub, lab, betab);
         if (mt == 1 || mt == 3) {
-          const T nu = k.nu_jelly;
-          gp4[0] = double(mub / (T(2) * (T(1) + nu)) +
-                          lab * nu / ((T(1) + nu) * (T(1) - T(2) * nu)));
+          const A nu = k.nu_jelly;
+          gp4[0] = double(mub / (A(2) * (A(1) + nu)) +
+                          lab * nu / ((A(1) + nu) * (A(1) - A(2) * nu)));
         }
         if (mt == 3) {
-          gamp[p] += betab * act.a * amp;
+          gamp[p] += T(betab * act.a * amp);
           gp4[1] = double(betab * act.a * as);
           gp4[2] = double(betab * amp * as * act.da_dc);
           gp4[3] = double(betab * amp * as * act.da_df);
@@ -516,20 +559,20 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
     }
     add_param_adjoints(gp4, valid && (mt == 1 || mt == 3), m, gparams);
   } else if (valid) {
-    a = load_state(a_full_in + 12 * (size_t)p);
+    a = load_state_as<A>(a_full_in + 12 * (size_t)p);
   }
   if constexpr (DO_G2PB) {
-    PState<T> sp = {T(0), T(0), T(0), T(0), {T(1), T(0), T(0), T(1)}, {T(1), T(0), T(0), T(1)}};
-    if (valid) sp = load_state(st_prev + 12 * (size_t)p);
+    PState<A> sp = {A(0), A(0), A(0), A(0), {A(1), A(0), A(0), A(1)}, {A(1), A(0), A(0), A(1)}};
+    if (valid) sp = load_particle<A>(st_prev + 12 * (size_t)p);
     int b0 = 0, b1 = 0;
-    T fx0, fx1;
+    A fx0, fx1;
     const bool active = valid && mt >= 0 && base_cell(sp.x0, sp.x1, k, b0, b1, fx0, fx1);
     const Tile tl = block_tile(active, m, b0, b1, kWinB);
-    PState<T> mo = {T(0), T(0), T(0), T(0), {T(0), T(0), T(0), T(0)}, {T(0), T(0), T(0), T(0)}};
+    PState<A> mo = {A(0), A(0), A(0), A(0), {A(0), A(0), A(0), A(0)}, {A(0), A(0), A(0), A(0)}};
     if (valid && mt < 0) mo = a;  // dead slots pass the adjoint through
     const T* g = grid_prev + (size_t)m * 3 * G;
     const int n = tl.tw * tl.th;
-    const GlobalEmit<T> direct{agrid_prev + (size_t)m * 3 * G, k.nx, G};
+    const GlobalEmit<A> direct{agrid_prev + (size_t)m * 3 * G, k.nx, G};
     if (tl.use) {
       // the window's NodeCoef once per node, and the zeroed scatter tile, in LDS
       const T* gw = grid_prev + (size_t)tl.inst * 3 * G;
@@ -537,7 +580,8 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
         const int jj = li / tl.tw;
         const int i = tl.ox + (li - jj * tl.tw), j = tl.oy + jj;
         const int node = j * k.nx + i;
-        const NodeCoef<T> c = node_coef(gw[node], gw[G + node], gw[2 * (size_t)G + node], i, j, k);
+        const NodeCoef<A> c = node_coef(A(gw[node]), A(gw[G + node]),
+                                        A(gw[2 * (size_t)G + node]), i, j, k);
         cw[li] = c.v0;
         cw[n + li] = c.v1;
         cw[2 * n + li] = c.A0;
@@ -557,16 +601,16 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
           [&](int i, int j) {
             if (tl.use) {
               const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-              return NodeCoef<T>{cw[li],         cw[n + li],     cw[2 * n + li],
+              return NodeCoef<A>{cw[li],         cw[n + li],     cw[2 * n + li],
                                  cw[3 * n + li], cw[4 * n + li], cw[5 * n + li]};
             }
             const int node = j * k.nx + i;
-            return node_coef(g[node], g[G + node], g[2 * (size_t)G + node], i, j, k);
+            return node_coef(A(g[node]), A(g[G + node]), A(g[2 * (size_t)G + node]), i, j, k);
           },
           mo,
-          [&](int i, int j, T c0, T c1, T c2) {
+          [&](int i, int j, A c0, A c1, A c2) {
             if (tl.use) {
-              const T n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
+              const A n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
               if (pr.same_next) {
                 c0 += n0;
                 c1 += n1;
@@ -587,10 +631,10 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const T* aF_in, int
       tile_flush(acc, tl, agrid_prev + (size_t)tl.inst * 3 * G, k.nx, G);
     }
     if (valid) {
-      T* mo_p = m_out + 8 * (size_t)p;
+      A* mo_p = m_out + 8 * (size_t)p;
       mo_p[0] = mo.x0; mo_p[1] = mo.x1; mo_p[2] = mo.v0; mo_p[3] = mo.v1;
       mo_p[4] = mo.C.a; mo_p[5] = mo.C.b; mo_p[6] = mo.C.c; mo_p[7] = mo.C.d;
-      T* af = aF_out + 4 * (size_t)p;
+      A* af = aF_out + 4 * (size_t)p;
       af[0] = a.F.a; af[1] = a.F.b; af[2] = a.F.c; af[3] = a.F.d;
     }
   }

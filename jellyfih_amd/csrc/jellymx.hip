@@ -1705,13 +1705,15 @@ int jm_points_in_polygon_grid(const double* poly, int64_t n, const double* xs, i
 struct jm_diff {
   int M = 0, nx = 0, ny = 0, G = 0;
   int64_t P = 0;
-  int bits = 32;                 // scalar type of states, grids and parameters
+  int bits = 32;                 // scalar type of the buffers: states, grids, parameters
+  int math = 32;                 // scalar type of the arithmetic and the carried adjoint
   jmd::Consts<float> kf{};
   jmd::Consts<double> kd{};
-  void* grid = nullptr;   // 3 * M * 3 * G scalars (forward grid ring)
-  void* agrid = nullptr;  // 3 * M * 3 * G scalars (adjoint grid ring)
-  void* sbuf = nullptr;   // 2 * P * 12 scalars (forward states, ping-pong)
-  void* pbuf = nullptr;   // 32 * P scalars: F_new (2 x 4), adjoint m (2 x 8), adjoint F (2 x 4)
+  void* grid = nullptr;   // 3 * M * 3 * G buffer scalars (forward grid ring)
+  void* agrid = nullptr;  // 3 * M * 3 * G arithmetic scalars (adjoint grid ring)
+  void* sbuf = nullptr;   // 2 * P * 12 buffer scalars (forward states, ping-pong)
+  void* pbuf = nullptr;   // F_new (2 x 4P buffer scalars), then adjoint m (2 x 8P) and adjoint
+                          // F (2 x 4P) in arithmetic scalars
   ~jm_diff() {
     for (void* p : {grid, agrid, sbuf, pbuf})
       if (p) (void)hipFree(p);
@@ -1758,6 +1760,20 @@ jmd::Consts<T> diff_consts(const jm_config& c, int nx, int ny) {
   return k;
 }
 
+// scalar_bits of jm_diff_create: the buffer and arithmetic scalar widths.
+bool diff_precision(int32_t scalar_bits, int* bits, int* math) {
+  if (scalar_bits == 32 || scalar_bits == 64) {
+    *bits = *math = scalar_bits;
+    return true;
+  }
+  if (scalar_bits == JM_DIFF_MIXED) {
+    *bits = 32;
+    *math = 64;
+    return true;
+  }
+  return false;
+}
+
 int diff_check(jm_diff* d, const jm_diff_inputs* in) {
   if (!d) return fail(JM_ERR_ARG, "null diff handle");
   if (!in || !in->inst || !in->mat || !in->fiber || !in->amp_scale || !in->params)
@@ -1778,11 +1794,14 @@ const jmd::Consts<T>& consts_of(const D* d) {
   else return d->kd;
 }
 
-template <typename T, typename D>
+// T: buffer scalar, A: arithmetic scalar (see diff_kernels.h). The clock advances in T, as
+// the reference's fp32 sim_time does in its fp32 build (mpm_sim.py:340).
+template <typename T, typename A, typename D>
 int fwd_block(D* d, const jm_diff_inputs* in, const T* state_in, double t0_in,
               int nsteps, T* out, T* grid_out, hipStream_t st) {
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
-  const jmd::Consts<T>& k = consts_of<T>(d);
+  const jmd::Consts<A>& k = consts_of<A>(d);
+  const T dt = consts_of<T>(d).dt;
   const int P = (int)d->P, G = d->G, n = nsteps;
   const size_t S = (size_t)P * 12;
   const int64_t MG = (int64_t)d->M * 3 * G;  // one substep's grids
@@ -1805,68 +1824,70 @@ int fwd_block(D* d, const jm_diff_inputs* in, const T* state_in, double t0_in,
                        MG);
   T t = T(t0_in);
   // K_f(0): P2G of substep 0
-  hipLaunchKernelGGL((jmd::k_d_fwd<T, false, true>), pg, tb, 0, st, din, state_in,
+  hipLaunchKernelGGL((jmd::k_d_fwd<T, A, false, true>), pg, tb, 0, st, din, state_in,
                      (const T*)nullptr, (const T*)nullptr, (T*)nullptr, fnew[0], grid(0),
-                     (!store && n > 1) ? ring + (size_t)MG : (T*)nullptr, MG, P, G, t, k);
+                     (!store && n > 1) ? ring + (size_t)MG : (T*)nullptr, MG, P, G, A(t), k);
   for (int s = 1; s < n; ++s) {
-    t = t + k.dt;  // mpm_sim.py:340 (fp32 in the reference)
+    t = t + dt;  // mpm_sim.py:340 (fp32 in the reference)
     // K_f(s): G2P of substep s-1, P2G of substep s
-    hipLaunchKernelGGL((jmd::k_d_fwd<T, true, true>), pg, tb, 0, st, din,
+    hipLaunchKernelGGL((jmd::k_d_fwd<T, A, true, true>), pg, tb, 0, st, din,
                        (const T*)(s == 1 ? state_in : state(s - 1)), (const T*)fnew[(s - 1) % 2],
                        (const T*)grid(s - 1), state(s), fnew[s % 2], grid(s),
                        (!store && s + 1 < n) ? ring + (size_t)((s + 1) % 3) * MG : (T*)nullptr,
-                       MG, P, G, t, k);
+                       MG, P, G, A(t), k);
   }
   // K_f(n): G2P of substep n-1
-  hipLaunchKernelGGL((jmd::k_d_fwd<T, true, false>), pg, tb, 0, st, din,
+  hipLaunchKernelGGL((jmd::k_d_fwd<T, A, true, false>), pg, tb, 0, st, din,
                      (const T*)(n == 1 ? state_in : state(n - 1)), (const T*)fnew[(n - 1) % 2],
                      (const T*)grid(n - 1), state(n), (T*)nullptr, (T*)nullptr, (T*)nullptr,
-                     MG, P, G, t, k);
+                     MG, P, G, A(t), k);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
 
-template <typename T, typename D>
+template <typename T, typename A, typename D>
 int bwd_block(D* d, const jm_diff_inputs* in, const T* traj, const T* grids,
               double t0_in, int nsteps, const T* grad_out, T* grad_in, T* grad_amp,
               double* grad_params, hipStream_t st) {
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
-  const jmd::Consts<T>& k = consts_of<T>(d);
+  const jmd::Consts<A>& k = consts_of<A>(d);
+  const T dt = consts_of<T>(d).dt;
   const int P = (int)d->P, G = d->G, n = nsteps;
   const size_t S = (size_t)P * 12;
   const int64_t MG = (int64_t)d->M * 3 * G;
   const dim3 pg(blocks_for(P, jmd::kDiffBlock)), tb(jmd::kDiffBlock);
-  T* ring = static_cast<T*>(d->agrid);
-  T* pb = static_cast<T*>(d->pbuf);
-  T* mb[2] = {pb + 8 * (size_t)P, pb + 16 * (size_t)P};     // m_s, by s % 2
-  T* aF[2] = {pb + 24 * (size_t)P, pb + 28 * (size_t)P};    // F part of a_s, by s % 2
+  A* ring = static_cast<A*>(d->agrid);
+  A* pb = reinterpret_cast<A*>(static_cast<char*>(d->pbuf) + 8 * (size_t)P * sizeof(T));
+  A* mb[2] = {pb, pb + 8 * (size_t)P};                        // m_s, by s % 2
+  A* aF[2] = {pb + 16 * (size_t)P, pb + 20 * (size_t)P};      // F part of a_s, by s % 2
   auto agrid = [&](int s) { return ring + (size_t)(s % 3) * MG; };
   std::vector<T> times(n + 1);
   times[0] = T(t0_in);
-  for (int s = 0; s < n; ++s) times[s + 1] = times[s] + k.dt;
-  hipLaunchKernelGGL(jmd::k_d_fill<T>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st,
+  for (int s = 0; s < n; ++s) times[s + 1] = times[s] + dt;
+  hipLaunchKernelGGL(jmd::k_d_fill<A>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st,
                      agrid(n - 1), MG);
   // K_b(n): G2P adjoint of substep n-1, driven by grad_out
-  hipLaunchKernelGGL((jmd::k_d_bwd<T, false, true>), pg, tb, 0, st, din, (const T*)nullptr,
-                     traj + (size_t)(n - 1) * S, (const T*)nullptr, 4, (const T*)nullptr,
-                     (const T*)nullptr, grids + (size_t)(n - 1) * MG, agrid(n - 1), grad_out,
+  hipLaunchKernelGGL((jmd::k_d_bwd<T, A, false, true>), pg, tb, 0, st, din, (const T*)nullptr,
+                     traj + (size_t)(n - 1) * S, (const A*)nullptr, 4, (const A*)nullptr,
+                     (const A*)nullptr, grids + (size_t)(n - 1) * MG, agrid(n - 1), grad_out,
                      mb[(n - 1) % 2], aF[n % 2], (T*)nullptr,
-                     n >= 2 ? agrid(n + 1) : (T*)nullptr, MG, P, G, times[n], k, grad_amp,
+                     n >= 2 ? agrid(n + 1) : (A*)nullptr, MG, P, G, A(times[n]), k, grad_amp,
                      grad_params);
   for (int s = n - 1; s >= 1; --s) {
     // K_b(s): P2G adjoint of substep s, G2P adjoint of substep s-1
-    hipLaunchKernelGGL((jmd::k_d_bwd<T, true, true>), pg, tb, 0, st, din, traj + (size_t)s * S,
-                       traj + (size_t)(s - 1) * S, (const T*)aF[(s + 1) % 2], 4,
-                       (const T*)mb[s % 2], (const T*)agrid(s), grids + (size_t)(s - 1) * MG,
-                       agrid(s - 1), (const T*)nullptr, mb[(s - 1) % 2], aF[s % 2],
-                       (T*)nullptr, s >= 2 ? agrid(s + 1) : (T*)nullptr, MG, P, G, times[s],
-                       k, grad_amp, grad_params);
+    hipLaunchKernelGGL((jmd::k_d_bwd<T, A, true, true>), pg, tb, 0, st, din,
+                       traj + (size_t)s * S, traj + (size_t)(s - 1) * S,
+                       (const A*)aF[(s + 1) % 2], 4, (const A*)mb[s % 2], (const A*)agrid(s),
+                       grids + (size_t)(s - 1) * MG, agrid(s - 1), (const T*)nullptr,
+                       mb[(s - 1) % 2], aF[s % 2], (T*)nullptr,
+                       s >= 2 ? agrid(s + 1) : (A*)nullptr, MG, P, G, A(times[s]), k,
+                       grad_amp, grad_params);
   }
   // K_b(0): P2G adjoint of substep 0, writing grad_in
-  hipLaunchKernelGGL((jmd::k_d_bwd<T, true, false>), pg, tb, 0, st, din, traj,
-                     (const T*)nullptr, (const T*)aF[1], 4, (const T*)mb[0],
-                     (const T*)agrid(0), (const T*)nullptr, (T*)nullptr, (const T*)nullptr,
-                     (T*)nullptr, (T*)nullptr, grad_in, (T*)nullptr, MG, P, G, times[0], k,
+  hipLaunchKernelGGL((jmd::k_d_bwd<T, A, true, false>), pg, tb, 0, st, din, traj,
+                     (const T*)nullptr, (const A*)aF[1], 4, (const A*)mb[0],
+                     (const A*)agrid(0), (const T*)nullptr, (A*)nullptr, (const T*)nullptr,
+                     (A*)nullptr, (A*)nullptr, grad_in, (A*)nullptr, MG, P, G, A(times[0]), k,
                      grad_amp, grad_params);
   HIP_TRY(hipGetLastError());
   return JM_OK;
@@ -1926,7 +1947,9 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
   if (cfg->n_instances <= 0 || cfg->quality <= 0 || n_particles <= 0 ||
       n_particles > (int64_t)INT32_MAX / 16)
     return fail(JM_ERR_ARG, "bad sizes");
-  if (scalar_bits != 32 && scalar_bits != 64) return fail(JM_ERR_ARG, "scalar_bits: 32 or 64");
+  int bits = 0, math = 0;
+  if (!diff_precision(scalar_bits, &bits, &math))
+    return fail(JM_ERR_ARG, "scalar_bits: 32, 64 or JM_DIFF_MIXED");
   if (cfg->axisym) return fail(JM_ERR_ARG, "the differentiable path has no axisymmetric variant");
   const int nx = 128 * cfg->quality;
   const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
@@ -1937,15 +1960,17 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
   d->ny = ny;
   d->G = nx * ny;
   d->P = n_particles;
-  d->bits = scalar_bits;
+  d->bits = bits;
+  d->math = math;
   d->kf = diff_consts<float>(*cfg, nx, ny);
   d->kd = diff_consts<double>(*cfg, nx, ny);
-  const size_t es = scalar_bits / 8;
-  const size_t gbytes = 3 * (size_t)d->M * 3 * d->G * es;
+  const size_t es = bits / 8, ea = math / 8;
+  const size_t grid_nodes = 3 * (size_t)d->M * 3 * d->G;
   const size_t sbytes = (size_t)d->P * 12 * es;
-  if (hipMalloc(&d->grid, gbytes) != hipSuccess || hipMalloc(&d->agrid, gbytes) != hipSuccess ||
+  if (hipMalloc(&d->grid, grid_nodes * es) != hipSuccess ||
+      hipMalloc(&d->agrid, grid_nodes * ea) != hipSuccess ||
       hipMalloc(&d->sbuf, 2 * sbytes) != hipSuccess ||
-      hipMalloc(&d->pbuf, 32 * (size_t)d->P * es) != hipSuccess) {
+      hipMalloc(&d->pbuf, 8 * (size_t)d->P * es + 24 * (size_t)d->P * ea) != hipSuccess) {
     delete d;
     return fail(JM_ERR_NOMEM, "hipMalloc(diff buffers)");
   }
@@ -1964,10 +1989,15 @@ int jm_fwd_block(jm_diff* d, const jm_diff_inputs* in, const void* state_in, dou
   if (!state_in || !out || nsteps < 1) return fail(JM_ERR_ARG, "bad state or step count");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d->bits == 64)
-    return fwd_block<double>(d, in, static_cast<const double*>(state_in), t0, nsteps,
-                             static_cast<double*>(out), static_cast<double*>(grid_out), st);
-  return fwd_block<float>(d, in, static_cast<const float*>(state_in), t0, nsteps,
-                          static_cast<float*>(out), static_cast<float*>(grid_out), st);
+    return fwd_block<double, double>(d, in, static_cast<const double*>(state_in), t0, nsteps,
+                                     static_cast<double*>(out),
+                                     static_cast<double*>(grid_out), st);
+  if (d->math == 64)
+    return fwd_block<float, double>(d, in, static_cast<const float*>(state_in), t0, nsteps,
+                                    static_cast<float*>(out), static_cast<float*>(grid_out),
+                                    st);
+  return fwd_block<float, float>(d, in, static_cast<const float*>(state_in), t0, nsteps,
+                                 static_cast<float*>(out), static_cast<float*>(grid_out), st);
 }
 
 int jm_fwd_block_noalloc(const jm_config* cfg, int64_t n_particles, int32_t scalar_bits,
@@ -1978,7 +2008,9 @@ int jm_fwd_block_noalloc(const jm_config* cfg, int64_t n_particles, int32_t scal
   if (cfg->n_instances <= 0 || cfg->quality <= 0 || n_particles <= 0 ||
       n_particles > (int64_t)INT32_MAX / 16)
     return fail(JM_ERR_ARG, "bad sizes");
-  if (scalar_bits != 32 && scalar_bits != 64) return fail(JM_ERR_ARG, "scalar_bits: 32 or 64");
+  int bits = 0, math = 0;
+  if (!diff_precision(scalar_bits, &bits, &math))
+    return fail(JM_ERR_ARG, "scalar_bits: 32, 64 or JM_DIFF_MIXED");
   if (cfg->axisym) return fail(JM_ERR_ARG, "the differentiable path has no axisymmetric variant");
   if (!in->inst || !in->mat || !in->fiber || !in->amp_scale || !in->params)
     return fail(JM_ERR_ARG, "jm_diff_inputs has a null pointer");
@@ -1990,7 +2022,7 @@ int jm_fwd_block_noalloc(const jm_config* cfg, int64_t n_particles, int32_t scal
   d.M = cfg->n_instances;
   d.G = nx * ny;
   d.P = n_particles;
-  const size_t es = scalar_bits / 8;
+  const size_t es = bits / 8;
   const size_t gbytes = align256(3 * (size_t)d.M * 3 * d.G * es);
   const size_t sbytes = align256(2 * (size_t)d.P * 12 * es);
   const size_t pbytes = align256(8 * (size_t)d.P * es);
@@ -2007,14 +2039,16 @@ int jm_fwd_block_noalloc(const jm_config* cfg, int64_t n_particles, int32_t scal
   d.sbuf = p + gbytes;
   d.pbuf = p + gbytes + sbytes;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (scalar_bits == 64) {
-    d.kd = diff_consts<double>(*cfg, nx, ny);
-    return fwd_block<double>(&d, in, static_cast<const double*>(state_in), t0, nsteps,
-                             static_cast<double*>(out), (double*)nullptr, st);
-  }
   d.kf = diff_consts<float>(*cfg, nx, ny);
-  return fwd_block<float>(&d, in, static_cast<const float*>(state_in), t0, nsteps,
-                          static_cast<float*>(out), (float*)nullptr, st);
+  d.kd = diff_consts<double>(*cfg, nx, ny);
+  if (bits == 64)
+    return fwd_block<double, double>(&d, in, static_cast<const double*>(state_in), t0, nsteps,
+                                     static_cast<double*>(out), (double*)nullptr, st);
+  if (math == 64)
+    return fwd_block<float, double>(&d, in, static_cast<const float*>(state_in), t0, nsteps,
+                                    static_cast<float*>(out), (float*)nullptr, st);
+  return fwd_block<float, float>(&d, in, static_cast<const float*>(state_in), t0, nsteps,
+                                 static_cast<float*>(out), (float*)nullptr, st);
 }
 
 int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const void* grids,
@@ -2025,14 +2059,22 @@ int jm_bwd_block(jm_diff* d, const jm_diff_inputs* in, const void* traj, const v
     return fail(JM_ERR_ARG, "bad backward arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d->bits == 64)
-    return bwd_block<double>(d, in, static_cast<const double*>(traj),
-                             static_cast<const double*>(grids), t0, nsteps,
-                             static_cast<const double*>(grad_out), static_cast<double*>(grad_in),
-                             static_cast<double*>(grad_amp), grad_params, st);
-  return bwd_block<float>(d, in, static_cast<const float*>(traj),
-                          static_cast<const float*>(grids), t0, nsteps,
-                          static_cast<const float*>(grad_out), static_cast<float*>(grad_in),
-                          static_cast<float*>(grad_amp), grad_params, st);
+    return bwd_block<double, double>(d, in, static_cast<const double*>(traj),
+                                     static_cast<const double*>(grids), t0, nsteps,
+                                     static_cast<const double*>(grad_out),
+                                     static_cast<double*>(grad_in),
+                                     static_cast<double*>(grad_amp), grad_params, st);
+  if (d->math == 64)
+    return bwd_block<float, double>(d, in, static_cast<const float*>(traj),
+                                    static_cast<const float*>(grids), t0, nsteps,
+                                    static_cast<const float*>(grad_out),
+                                    static_cast<float*>(grad_in),
+                                    static_cast<float*>(grad_amp), grad_params, st);
+  return bwd_block<float, float>(d, in, static_cast<const float*>(traj),
+                                 static_cast<const float*>(grids), t0, nsteps,
+                                 static_cast<const float*>(grad_out),
+                                 static_cast<float*>(grad_in), static_cast<float*>(grad_amp),
+                                 grad_params, st);
 }
 
 }  // extern "C"

@@ -8,7 +8,9 @@ input state (the checkpoint); its backward re-runs the block's forward storing e
 substep's state and grid, then runs the adjoint substeps in reverse.
 
 State layout, one row of 12 fp32 per live particle:
-``x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11``. Particles of all instances share one
+``x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11``; with fp32 buffers and fp64 arithmetic
+(the default for fp32 tanks) the last four hold F - I (``DiffTank.encode_state`` /
+``decode_state`` convert). Particles of all instances share one
 array; ``inst`` gives each particle's instance. Dead slots are dropped when a tank is
 packed, so every row is a live particle.
 
@@ -83,18 +85,26 @@ class DiffTank:
     and scratch the forward and adjoint substeps need.
 
     ``inst`` (P,) instance index, ``mat`` (P,) material (0 water, 1 jelly, 2 payload,
-    3 muscle), ``fiber`` (P, 2) muscle fibre direction. ``dtype`` is the arithmetic of
-    states, grids and gradients: float32 (the reference's precision) or float64 (for
-    gradients that must agree with fp64 finite differences: an fp32 trajectory's positions
-    carry rounding of ~1e-5 after thousands of substeps, a percent of a typical payload
-    displacement)."""
+    3 muscle), ``fiber`` (P, 2) muscle fibre direction. ``dtype`` is the type of the states,
+    grids, inputs and gradients: float32 (the reference's storage) or float64. ``math`` is
+    the arithmetic inside each substep: "fp64" (default; with float32 buffers each substep
+    widens its fp32 inputs, computes and carries the adjoint in fp64, and rounds its results
+    to fp32) or "fp32" (float32 only: the reference's arithmetic). fp32 arithmetic resolves
+    the small strains near F = I (the jelly's F - R, the water's J - 1 at stiffness 1e5) to
+    a few digits only, which moves gradients by percents within a few hundred substeps; fp32
+    storage of the results does not (the fp64 arithmetic over fp32 buffers agrees with fp64
+    finite differences to ~1e-5 over 200 substeps)."""
 
     def __init__(self, inst, mat, fiber, n_instances: int, quality: int = 1,
                  n_grid_y: int | None = None, domain_height: float = 1.0,
-                 device: str | torch.device = "cuda", dtype: torch.dtype = torch.float32):
+                 device: str | torch.device = "cuda", dtype: torch.dtype = torch.float32,
+                 math: str = "fp64"):
         if dtype not in (torch.float32, torch.float64):
             raise ValueError("dtype must be torch.float32 or torch.float64")
+        if math not in ("fp32", "fp64") or (dtype == torch.float64 and math != "fp64"):
+            raise ValueError("math must be 'fp64', or 'fp32' with float32 buffers")
         self.dtype = dtype
+        self.math = math
         load()
         dev = torch.device(device)
         if dev.type == "cuda" and dev.index is None:
@@ -136,8 +146,11 @@ class DiffTank:
         self.ny = int(n_grid_y or self.nx)
         self.G = self.nx * self.ny
         h = ctypes.c_void_p()
-        call("jm_diff_create", ctypes.byref(cfg), ctypes.c_int64(self.P),
-             64 if dtype == torch.float64 else 32, ctypes.byref(h))
+        bits = 64 if dtype == torch.float64 else (_lib.JM_DIFF_MIXED if math == "fp64" else 32)
+        call("jm_diff_create", ctypes.byref(cfg), ctypes.c_int64(self.P), bits, ctypes.byref(h))
+        self.scalar_bits = bits
+        # fp32 buffers with fp64 arithmetic carry F - I in the state's F columns
+        self.f_shift = bits == _lib.JM_DIFF_MIXED
         self._h = h
         self._L = load()
         self._owner = None      # a view (see _view) keeps the tank that owns the handle
@@ -166,9 +179,23 @@ class DiffTank:
         tank.slots = np.stack([inst_all, slot_all], 1)
         state = np.zeros((inst_all.size, STATE_COLS), np.float32)
         state[:, 0:2] = pos[inst_all, slot_all]
-        state[:, 8] = 1.0
-        state[:, 11] = 1.0
+        state[:, 8] = state[:, 11] = 0.0 if tank.f_shift else 1.0   # F = I
         return tank, torch.from_numpy(state).to(tank.device, dtype)
+
+    # -- state format ---------------------------------------------------------------------
+    def _identity_cols(self, like: torch.Tensor) -> torch.Tensor:
+        e = torch.zeros(STATE_COLS, dtype=like.dtype, device=like.device)
+        e[8] = e[11] = 1.0
+        return e
+
+    def encode_state(self, state: torch.Tensor) -> torch.Tensor:
+        """A state with F in columns 8-11 in this tank's buffer format (F - I when
+        ``f_shift``); differentiable, a no-op otherwise."""
+        return state - self._identity_cols(state) if self.f_shift else state
+
+    def decode_state(self, state: torch.Tensor) -> torch.Tensor:
+        """The inverse of :meth:`encode_state`: columns 8-11 hold F."""
+        return state + self._identity_cols(state) if self.f_shift else state
 
     def close(self) -> None:
         if getattr(self, "_owner", None) is not None:   # a view: the owner frees the handle

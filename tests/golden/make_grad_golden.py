@@ -1,5 +1,6 @@
-"""Writes tests/golden/grad_fd_golden.json: fp64 central differences of the payload CoM y
-of one seeded reference tank (seed 0, ~62K live particles) after 2000 substeps, along the
+"""Writes tests/golden/grad_fd_golden.json (2000 substeps) or grad_fd_golden_<steps>.json:
+fp64 central differences of the payload CoM y
+of one seeded reference tank (seed 0, ~62K live particles) after the run, along the
 seeded directions of tests/harness.py (body initial velocities, actuation amplitude,
 contraction fraction, frequency, E_jelly, mixtures) from the oracle, and along a
 per-particle actuation multiplier direction from the fp64 harness (the multiplier has no
@@ -12,7 +13,8 @@ rounding of a 2000-substep trajectory moves the gradient.
 The GPU test recomputes the same directions from their seeds and compares the HIP
 adjoint's directional derivatives with these values. Takes a few minutes on 8 cores:
 
-    python tests/golden/make_grad_golden.py
+    python tests/golden/make_grad_golden.py          # 2000 substeps
+    python tests/golden/make_grad_golden.py 200      # 200 substeps
 """
 from __future__ import annotations
 
@@ -30,9 +32,10 @@ from tests.harness import (amp_scale_direction, gradient_directions,  # noqa: E4
                            harness_amp_scale_fd, harness_payload_gradient, load_harness,
                            oracle_payload_fd, reference_tank_case)
 
-STEPS = 2000
+STEPS = int(sys.argv[1]) if len(sys.argv) > 1 else 2000
 EPS = [1e-4, 1e-5, 1e-6]
-OUT = os.path.join(ROOT, "tests", "golden", "grad_fd_golden.json")
+OUT = os.path.join(ROOT, "tests", "golden", "grad_fd_golden.json" if STEPS == 2000
+                   else f"grad_fd_golden_{STEPS}.json")
 
 
 def main() -> None:
@@ -58,7 +61,7 @@ def main() -> None:
     ad32_amp = float(np.sum(gamp * amp_scale_direction(cs.mat)))
     print(f"fp32 harness adjoint done ({time.time() - t0:.0f} s)", flush=True)
     rec = {
-        "case": "reference tank seed 0, 80000 slots, 2000 substeps from t=0, objective = "
+        "case": f"reference tank seed 0, 80000 slots, {STEPS} substeps from t=0, objective = "
                 "payload CoM y after the run",
         "steps": STEPS,
         "n_live": int(cs.P),

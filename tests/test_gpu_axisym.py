@@ -9,7 +9,7 @@ r >= 0.6 (the oracle shows the same). The runs compared here stop before that.
 Tolerances as for the planar parity tests: one substep's grid mass and momentum within
 1e-5 of their maximum, grid velocity within 1e-5 where the node mass is at least 1e-3 of
 the maximum; after several substeps x, v, C and F within twice the fp32 oracle's own error
-against the fp64 oracle (C: the 99.9th percentile, maximum within 5x; see fp32_like)."""
+against the fp64 oracle (C: mean within 2x, tail quantiles 3x, maximum 5x; see fp32_like)."""
 from __future__ import annotations
 
 import sys

@@ -10,14 +10,13 @@
   (eps = 1e-4, 1e-5, 1e-6; tests/golden/grad_fd_golden.json) of the oracle, along 8 seeded
   directions in the body's initial velocities, actuation amplitude, contraction fraction,
   frequency and E_jelly, and of the fp64 harness along the per-particle actuation
-  multiplier. fp64: within 1e-4. fp32: the objective, a payload displacement of ~1.6e-3, is
-  a difference of positions near 0.4 that fp32 state holds to ~3e-8, and 2000 substeps
-  amplify that rounding; a sequential fp32 build of the same algorithm (in the fixture) is
-  off by up to 5%, and the HIP fp32 result varies from run to run with the order of its
-  atomic adds (2% to 20% for the most sensitive direction). The fp32 kernels are the same
-  templates as the fp64 ones and are checked against the fp64 adjoint on a
-  well-conditioned case above; here they must agree in sign and magnitude (within a factor
-  of 2), and the spread is reported;
+  multiplier; the same at 200 substeps (tests/golden/grad_fd_golden_200.json). fp64: within
+  1e-4. fp32 buffers with fp64 arithmetic inside each substep (DiffTank's default for fp32):
+  within 1e-3 at 200 substeps, 5e-3 at 2000 (GRAD_BOUNDS). fp32 arithmetic (the
+  reference's): small strains near F = I are differences of numbers near 1 that fp32
+  resolves to ~1e-7, so its gradients are off by percents (a sequential fp32 build of the
+  same algorithm, in the fixtures: up to 4% at 200 substeps, 5% at 2000); they must agree in
+  sign and magnitude (within a factor of 2), and the spread is reported;
 * gradients do not depend on the checkpoint interval or on recomputation.
 """
 from __future__ import annotations
@@ -31,37 +30,43 @@ from tests.harness import (Case, amp_scale_direction, gradient_directions, load_
 pytestmark = pytest.mark.gpu
 
 DTYPES = ["float32", "float64"]
+# (buffers, arithmetic): fp32 buffers with fp64 arithmetic (DiffTank's default for fp32),
+# the reference's fp32 arithmetic, fp64 throughout
+PRECISIONS = [("float32", "fp64"), ("float32", "fp32"), ("float64", "fp64")]
 
 
 def say(*a):
     print(*a, flush=True)
 
 
-def _gpu(cs, dtype="float32"):
+def _gpu(cs, dtype="float32", math="fp64"):
     import torch
     from jellyfih_amd.diff import DiffTank
     td = getattr(torch, dtype)
-    tank = DiffTank(cs.inst, cs.mat, cs.fiber, cs.M, dtype=td)
+    tank = DiffTank(cs.inst, cs.mat, cs.fiber, cs.M, dtype=td, math=math)
     f = lambda a: torch.tensor(np.asarray(a), dtype=td, device=tank.device)
-    return tank, f(cs.state), f(cs.amp_scale), f(cs.params)
+    return tank, tank.encode_state(f(cs.state)), f(cs.amp_scale), f(cs.params)
 
 
 GROUPS = {"x": slice(0, 2), "v": slice(2, 4), "C": slice(4, 8), "F": slice(8, 12)}
 
 
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_diff_forward_vs_oracle(require_gpu, dtype):
+@pytest.mark.parametrize("dtype,math", PRECISIONS)
+def test_diff_forward_vs_oracle(require_gpu, dtype, math):
     """One seeded reference tank (~62K live particles), 200 substeps from rest: the final
-    state, and the last state of a stored trajectory (a second run, other atomics order)."""
+    state, and the last state of a stored trajectory (a second run, other atomics order).
+    fp32 buffers (either arithmetic): as accurate as the fp32 oracle; fp64: equal to the
+    fp64 oracle up to summation order."""
     import torch
-    say(f"diff forward {dtype}: reference tank seed0, 200 substeps, vs oracle f32/f64")
+    say(f"diff forward {dtype}/{math}: reference tank seed0, 200 substeps, vs oracle f32/f64")
     cs, _, _, _ = reference_tank_case(200)
-    tank, st, amp, par = _gpu(cs, dtype)
-    out = tank.forward_raw(st, amp, par, cs.t0, cs.steps).cpu().numpy().astype(np.float64)
+    tank, st, amp, par = _gpu(cs, dtype, math)
+    out = tank.forward_raw(st, amp, par, cs.t0, cs.steps)
+    out = tank.decode_state(out.double()).cpu().numpy()
     traj = torch.empty((cs.steps, cs.P, 12), dtype=tank.dtype, device=tank.device)
     grids = torch.empty((cs.steps, 1, 3, tank.G), dtype=tank.dtype, device=tank.device)
     tank.forward_raw(st, amp, par, cs.t0, cs.steps, out=traj, grids=grids)
-    traj = traj.cpu().numpy().astype(np.float64)
+    traj = tank.decode_state(traj.double()).cpu().numpy()
     # B-spline weights sum to one: every stored grid holds the total particle mass,
     # p_mass * (non-payload count + payload density * payload count)
     n_pay = int(np.sum(cs.mat == 2))
@@ -108,7 +113,8 @@ def test_diff_forward_tall_tank(require_gpu, dtype):
     params = torch.tensor([[500.0, con, frq, 2.5, 700.0]], dtype=torch.float32).to(
         tank.device, td)
     amp = torch.ones(tank.P, dtype=td, device=tank.device)
-    out = tank.forward_raw(st, amp, params, 0.0, steps).cpu().numpy().astype(np.float64)
+    out = tank.decode_state(tank.forward_raw(st, amp, params, 0.0, steps).double())
+    out = out.cpu().numpy()
     slot = tank.slots[:, 1]
     consts = oracle.reference_consts(n_grid_y=256, domain_height=2.0)
     o32 = oracle.OracleSim(1, 160000, consts, dtype=np.float32)
@@ -131,18 +137,18 @@ def test_diff_forward_tall_tank(require_gpu, dtype):
             assert err <= 1e-9 * max(np.abs(r64).max(), 1.0), name
 
 
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype):
+@pytest.mark.parametrize("dtype,math", PRECISIONS)
+def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype, math):
     """Perturbed states of every material (2 x 160 particles), 12 substeps: the whole
     adjoint (state, multiplier, parameters) against the fp64 host build."""
     import torch
-    say(f"diff adjoint {dtype}: 2x160 perturbed particles, 12 substeps, vs fp64 harness")
+    say(f"diff adjoint {dtype}/{math}: 2x160 perturbed particles, 12 substeps, vs fp64 harness")
     cs = Case(1, steps=12).round_f32()
     L = load_harness()
     rng = np.random.default_rng(5)
     wout = rng.normal(size=(cs.P, 12))
     gin64, gamp64, gpar64 = cs.backward(L, cs.forward(L), wout)
-    tank, st, amp, par = _gpu(cs, dtype)
+    tank, st, amp, par = _gpu(cs, dtype, math)
     gin, gamp, gpar = tank.backward_raw(st, amp, par, cs.t0, cs.steps,
                                         torch.tensor(wout, dtype=tank.dtype,
                                                      device=tank.device))
@@ -159,10 +165,11 @@ def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype):
     assert np.all(gpar[:, 3] == 0)
 
 
-def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every=0):
+def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every=0,
+                   math="fp64"):
     """Autograd through DiffTank.run: d(payload CoM y displacement) w.r.t. the state,
     the per-particle multiplier and the per-instance parameters."""
-    tank, st, amp, par = _gpu(cs, dtype)
+    tank, st, amp, par = _gpu(cs, dtype, math)
     st.requires_grad_(True)
     amp.requires_grad_(True)
     leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2, 4)]
@@ -196,32 +203,62 @@ def _directional(cs, gg, gst, gamp, gpar):
     return rows
 
 
-@pytest.mark.parametrize("dtype", DTYPES)
-def test_reference_tank_gradients_vs_fd(require_gpu, golden_grad, dtype):
-    """One seeded reference tank, 2000 substeps, checkpoint every 100 (cfg4 shape)."""
-    say(f"reference tank seed0, 2000 substeps, checkpoint 100, {dtype}: adjoint vs fp64 FD")
-    gg = golden_grad
+def _golden(steps: int):
+    import json
+    import os
+    name = "grad_fd_golden.json" if steps == 2000 else f"grad_fd_golden_{steps}.json"
+    with open(os.path.join(os.path.dirname(__file__), "golden", name)) as f:
+        return json.load(f)
+
+
+# Bounds on |AD - FD| / |FD| (best over eps) per (horizon, buffers, arithmetic). fp64: the
+# fp64 adjoint is the derivative of the fp64 model (1e-4). fp32 buffers (F stored as F - I)
+# with fp64 arithmetic: 1e-3 at 200 substeps (measured 7.5e-5); over 2000 substeps the
+# fp32 rounding of the stored states, compounded, moves the linearisation point: 5e-3
+# (measured 1.0e-3, as the host build of the same scheme; the multiplier direction, whose
+# differences come from the fp64 host build, 1.4e-2 against AMP_SCALE_BOUND_2000). fp32
+# arithmetic (the reference's): None = sign and magnitude only (ratio in [0.5, 2]); its
+# sequential host build is off by up to 4% at 200 substeps and 5% at 2000.
+GRAD_BOUNDS = {(200, "float64", "fp64"): 1e-4, (200, "float32", "fp64"): 1e-3,
+               (200, "float32", "fp32"): None, (2000, "float64", "fp64"): 1e-4,
+               (2000, "float32", "fp64"): 5e-3, (2000, "float32", "fp32"): None}
+AMP_SCALE_BOUND_2000 = 3e-2
+
+
+@pytest.mark.parametrize("steps", [200, 2000])
+@pytest.mark.parametrize("dtype,math", PRECISIONS)
+def test_reference_tank_gradients_vs_fd(require_gpu, steps, dtype, math):
+    """One seeded reference tank, 200 or 2000 substeps, checkpoint every 100 (cfg4 shape):
+    directional derivatives of the payload CoM y along 8 seeded directions and the
+    per-particle multiplier direction against fp64 central differences."""
+    say(f"reference tank seed0, {steps} substeps, checkpoint 100, {dtype}/{math}: "
+        "adjoint vs fp64 FD")
+    gg = _golden(steps)
     cs, pos, mat, fib = reference_tank_case(gg["steps"])
     assert cs.P == gg["n_live"]
     np.testing.assert_array_equal(cs.params[0], np.array(gg["params"]))
-    y, gst, gamp, gpar = _gpu_gradients(cs, checkpoint=100, dtype=dtype)
+    y, gst, gamp, gpar = _gpu_gradients(cs, checkpoint=100, dtype=dtype, math=math)
     say(f"payload displacement {y:.9e}; dJ/d(amp, contraction, freq, E) = {gpar}")
+    bound = GRAD_BOUNDS[(steps, dtype, math)]
     worst, worst32 = 0.0, 0.0
     for kind, ad, ad64, ad32, fds in _directional(cs, gg, gst, gamp, gpar):
         r64 = _best(ad64, fds)
         rel = _best(ad, fds)
         say(f"{kind:12s} AD={ad:.9e} AD64(host)={ad64:.9e} |AD64-FD|={r64:.1e} "
             f"|AD-FD|={rel:.1e} |AD32(host)-FD|={_best(ad32, fds):.1e}")
-        assert r64 <= 1e-4, (kind, ad64, fds)
+        # the multiplier's differences come from the fp64 host build (2e-4 at 200 substeps)
+        fd_tol = 2e-4 if kind == "amp_scale" else 1e-4
+        assert r64 <= fd_tol, (kind, ad64, fds)
         worst = max(worst, rel)
         worst32 = max(worst32, _best(ad32, fds))
-        if dtype == "float32":
-            # same sign and magnitude (see the module docstring for why no tighter bound)
+        if bound is None:
             ratio = ad / fds[0]
             assert 0.5 <= ratio <= 2.0, (kind, ad, fds)
-    say(f"worst |AD-FD| {worst:.2e}; sequential fp32 build: {worst32:.2e}")
-    if dtype == "float64":
-        assert worst <= 1e-4, worst
+        elif kind == "amp_scale" and steps == 2000 and dtype == "float32":
+            assert rel <= AMP_SCALE_BOUND_2000, (kind, ad, fds)
+        else:
+            assert rel <= max(bound, 2 * r64), (kind, rel, bound)
+    say(f"worst |AD-FD| {worst:.2e} (bound {bound}); sequential fp32 build: {worst32:.2e}")
 
 
 def test_gradients_independent_of_checkpoint_interval(require_gpu):

@@ -53,7 +53,7 @@ def noalloc_forward(tank: DiffTank, state, amp_scale, params, t0, n):
     out = torch.empty_like(state)
     ins = tank._inputs(amp_scale, params)
     _lib.call("jm_fwd_block_noalloc", ctypes.byref(cfg), ctypes.c_int64(tank.P),
-              64 if tank.dtype == torch.float64 else 32, ctypes.byref(ins),
+              tank.scalar_bits, ctypes.byref(ins),
               ctypes.c_void_p(state.data_ptr()), ctypes.c_double(t0), int(n),
               ctypes.c_void_p(out.data_ptr()),
               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
