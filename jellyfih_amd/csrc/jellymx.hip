@@ -844,8 +844,12 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   const int nx = 128 * cfg->quality;
   const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
   if (ny < 8 || ny % 4 != 0) return fail(JM_ERR_ARG, "n_grid_y must be a multiple of 4, >= 8");
-  if ((int64_t)cfg->n_instances * cfg->n_particles > (int64_t)INT32_MAX / 2)
-    return fail(JM_ERR_ARG, "n_instances * n_particles too large");
+  // the kernels address particle arrays with 32-bit byte offsets (16-byte elements at most)
+  // and pack a base cell's two indices into 16 bits each
+  if ((int64_t)cfg->n_instances * cfg->n_particles > ((int64_t)1 << 28))
+    return fail(JM_ERR_ARG, "n_instances * n_particles above 2^28 slots");
+  if (cfg->quality > 255 || (cfg->n_grid_y > 0 && cfg->n_grid_y >= 32768))
+    return fail(JM_ERR_ARG, "grid above 32767 nodes per axis");
 
   jm_handle* h = new jm_handle();
   h->cfg = *cfg;

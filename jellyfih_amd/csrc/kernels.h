@@ -406,9 +406,16 @@ constexpr int kBoxEmpty = 1 << 29;
 // straight to the global fp32 grid instead, so non-finite values reach exactly its nine
 // nodes, as in fp32.
 constexpr float kFixMax = 0.75f;
+// The window's planes are summed as biased fixed point (fixed_bits_biased, lanes.h): the
+// mass plane recovers each node's emission count from its top bits, which needs its sums
+// below 2^48 units of 2^-50, i.e. below 0.25. A window adds at most 4096 particles' terms
+// of at most 0.5625 m each, so the fixed-point path takes particles of mass below 2^-14
+// (water and jelly 1.5e-5, the payload at density 2.5 3.8e-5; up to density 4); heavier
+// ones scatter to the global grid like out-of-range values.
+constexpr float kFixMassMax = 6.103515625e-05f;  // 2^-14
 
 __device__ __forceinline__ void add_fixed_scaled(unsigned long long* slot, float y) {
-  atomicAdd(slot, fixed_bits(y));
+  atomicAdd(slot, fixed_bits_biased(y));
 }
 
 // Row of node q of a window ww nodes wide, for q < 2^21: (q + 0.5) / ww lies at least
@@ -516,8 +523,32 @@ __device__ __forceinline__ void g2p_global(const float* gp, int G, const PhysCon
   Cs[gi] = C;
 }
 
+// Element i of an array, addressed as base + 32-bit byte offset: with a uniform base the
+// load or store takes the scalar-base form (one 32-bit offset per access instead of 64-bit
+// address arithmetic). Arrays hold fewer than 2^32 bytes (jm_create checks the slot count).
+template <typename T>
+__device__ __forceinline__ T& elem(T* base, uint32_t i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& elem(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     i * (uint32_t)sizeof(T));
+}
+
+// P2G base cell of a particle, packed as b0 | b1 << 16 (both in [0, 2^15) when in range),
+// with its fractional offsets; -1 when the particle is dead or outside the transfer range.
+// in_range is the test of mpm_sim.py:184,316.
+__device__ __forceinline__ int packed_cell(float2 x, const PhysConsts& k, float& fx0,
+                                           float& fx1) {
+  int b0, b1;
+  base_frac(x.x, k.inv_dx, b0, fx0);
+  base_frac(x.y, k.inv_dx, b1, fx1);
+  return in_range(b0, b1, k) ? (b0 | (b1 << 16)) : -1;
+}
+
 template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
-__global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_fused(FusedArgs a) {
   // Grid-window capacity (nodes): a chunk of P sorted particles (4 per cell in water)
   // covers about P/4 cells (P/4/S columns of an S-row strip, or P/512 + 2 rows of 128 in
   // row-major order), plus the 2-node stencil halo and drift since the last sort.
@@ -560,16 +591,29 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     for (int64_t q = lo + tid; q < hi; q += BLOCK) z[q] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 
+  // This chunk's slices of the particle arrays (uniform bases; lanes add 32-bit offsets).
+  float2* const xs = a.s.x + start;
+  float2* const vs = a.s.v + start;
+  float4* const Cs = a.s.C + start;
+  float4* const Fs = a.s.F + start;
+  float* const Jps = a.s.Jp + start;
+  const int* const mats = a.s.mat + start;
+  const float2* const fibs = a.s.fiber + start;
+
   float2 x[PPT], v[PPT];
   float4 C[PPT];
   int mt[PPT];
+  // Base cell (packed_cell) and fractional offsets of each particle, computed once per phase:
+  // at the gathered positions for G2P, then at the advected positions for P2G.
+  int cell[PPT];
+  float f0[PPT], f1[PPT];
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
-    const int li = p * BLOCK + tid;
+    const uint32_t li = p * BLOCK + tid;
     mt[p] = -1;
-    if (li < count) {
-      x[p] = a.s.x[start + li];
-      mt[p] = a.s.mat[start + li];
+    if ((int)li < count) {
+      x[p] = elem(xs, li);
+      mt[p] = elem(mats, li);
     }
   }
 
@@ -578,12 +622,12 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     int mn0 = kBoxEmpty, mx0 = -kBoxEmpty, mn1 = kBoxEmpty, mx1 = -kBoxEmpty;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
+      cell[p] = -1;
+      f0[p] = f1[p] = 0.5f;
       if (mt[p] < 0) continue;
-      int b0, b1;
-      float f0, f1;
-      base_frac(x[p].x, k.inv_dx, b0, f0);
-      base_frac(x[p].y, k.inv_dx, b1, f1);
-      if (!in_range(b0, b1, k)) continue;
+      cell[p] = packed_cell(x[p], k, f0[p], f1[p]);
+      if (cell[p] < 0) continue;
+      const int b0 = cell[p] & 0xffff, b1 = cell[p] >> 16;
       mn0 = min(mn0, b0);
       mx0 = max(mx0, b0);
       mn1 = min(mn1, b1);
@@ -610,16 +654,13 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         if (mt[p] < 0) continue;
-        const int li = p * BLOCK + tid;
-        int b0, b1;
-        float fx0, fx1;
-        base_frac(x[p].x, k.inv_dx, b0, fx0);
-        base_frac(x[p].y, k.inv_dx, b1, fx1);
-        if (in_range(b0, b1, k)) {
-          g2p_gather(lv + (b1 - mn1) * ww + (b0 - mn0), ww, fx0, fx1, k, v[p], C[p]);
+        const uint32_t li = p * BLOCK + tid;
+        if (cell[p] >= 0) {
+          const int b0 = cell[p] & 0xffff, b1 = cell[p] >> 16;
+          g2p_gather(lv + __mul24(b1 - mn1, ww) + (b0 - mn0), ww, f0[p], f1[p], k, v[p], C[p]);
         } else {
-          v[p] = a.s.v[start + li];
-          C[p] = a.s.C[start + li];
+          v[p] = elem(vs, li);
+          C[p] = elem(Cs, li);
         }
         advect(x[p], v[p], k);
       }
@@ -634,11 +675,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       }
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
-        const int li = p * BLOCK + tid;
-        if (li < count) {
-          x[p] = a.s.x[start + li];
-          v[p] = a.s.v[start + li];
-          C[p] = a.s.C[start + li];
+        const uint32_t li = p * BLOCK + tid;
+        if ((int)li < count) {
+          x[p] = elem(xs, li);
+          v[p] = elem(vs, li);
+          C[p] = elem(Cs, li);
         }
       }
     }
@@ -655,9 +696,9 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       if (mt[p] < 0) continue;
-      const int li = p * BLOCK + tid;
-      v[p] = a.s.v[start + li];
-      C[p] = a.s.C[start + li];
+      const uint32_t li = p * BLOCK + tid;
+      v[p] = elem(vs, li);
+      C[p] = elem(Cs, li);
     }
   }
 
@@ -675,12 +716,12 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     int mn0 = kBoxEmpty, mx0 = -kBoxEmpty, mn1 = kBoxEmpty, mx1 = -kBoxEmpty;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
+      cell[p] = -1;
+      f0[p] = f1[p] = 0.5f;
       if (mt[p] < 0) continue;
-      int b0, b1;
-      float f0, f1;
-      base_frac(x[p].x, k.inv_dx, b0, f0);
-      base_frac(x[p].y, k.inv_dx, b1, f1);
-      if (!in_range(b0, b1, k)) continue;
+      cell[p] = packed_cell(x[p], k, f0[p], f1[p]);
+      if (cell[p] < 0) continue;
+      const int b0 = cell[p] & 0xffff, b1 = cell[p] >> 16;
       mn0 = min(mn0, b0);
       mx0 = max(mx0, b0);
       mn1 = min(mn1, b1);
@@ -703,33 +744,27 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     const int lane = tid & 63;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
-      const int li = p * BLOCK + tid;
-      const int gi = start + li;
-      int b0 = 0, b1 = 0;
-      float fx0 = 0.f, fx1 = 0.f;
-      bool inr = false;
-      if (mt[p] >= 0) {
-        base_frac(x[p].x, k.inv_dx, b0, fx0);
-        base_frac(x[p].y, k.inv_dx, b1, fx1);
-        inr = in_range(b0, b1, k);
-        if (!inr && DO_G2P && a.lazy_vc) {
-          a.s.v[gi] = v[p];
-          a.s.C[gi] = C[p];
-        }
+      const uint32_t li = p * BLOCK + tid;
+      const bool inr = cell[p] >= 0;
+      const int b0 = cell[p] & 0xffff, b1 = cell[p] >> 16;
+      const float fx0 = f0[p], fx1 = f1[p];
+      if (mt[p] >= 0 && !inr && DO_G2P && a.lazy_vc) {
+        elem(vs, li) = v[p];
+        elem(Cs, li) = C[p];
       }
       // Particle update for in-range particles; every other lane scatters zeros, computed
-      // from finite stand-ins (mass 0, affine 0, v 0, and its base cell's weights) rather
-      // than by clearing the 21 per-particle factors.
+      // from finite stand-ins (mass 0, affine 0, v 0, offsets 0.5) rather than by clearing
+      // the 21 per-particle factors.
       float mass = 0.f;
       Mat2 aff = {0.f, 0.f, 0.f, 0.f};
       if (inr) {
         float2 fib = make_float2(0.f, 1.f);
-        if (mt[p] == 3) fib = a.s.fiber[gi];
+        if (mt[p] == 3) fib = elem(fibs, li);
         const Mat2 Cm = as_mat(C[p]);
-        P2GOut o = p2g_update(mt[p], as_mat(a.s.F[gi]), Cm, fib.x, fib.y, amp_act, mu_j, la_j,
-                              pdens, k);
-        a.s.F[gi] = as_f4(o.F);
-        if (o.jp_changes) a.s.Jp[gi] = (a.s.Jp[gi] * o.q0) * o.q1;
+        P2GOut o = p2g_update(mt[p], as_mat(elem(Fs, li)), Cm, fib.x, fib.y, amp_act, mu_j,
+                              la_j, pdens, k);
+        elem(Fs, li) = as_f4(o.F);
+        if (o.jp_changes) elem(Jps, li) = (elem(Jps, li) * o.q0) * o.q1;
         mass = o.mass;
         aff = o.affine;
       }
@@ -744,7 +779,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       const bool fixed_ok = inr && use_lds &&
                             fabsf(mv0) + bnd * (fabsf(aff.a00) + fabsf(aff.a01)) < kFixMax &&
                             fabsf(mv1) + bnd * (fabsf(aff.a10) + fabsf(aff.a11)) < kFixMax &&
-                            mass < kFixMax && fx0 + fx1 < 4.0f;
+                            mass < kFixMassMax && fx0 + fx1 < 4.0f;
       const float sc = fixed_ok ? kFixPre : (inr ? 1.0f : 0.0f);
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
@@ -778,9 +813,9 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
       // lane's terms are finite, since out-of-range lanes carry zeros). Lanes outside the
       // fixed-point LDS path carry unique keys and add nothing. Measured at 32 x 80K:
       // 44.3 us per substep against 45.7 us with pairs only.
-      const int key = fixed_ok ? (b1 * nx + b0 + 1) : -(1 + lane);
+      const int key = fixed_ok ? (cell[p] + 1) : -(1 + lane);
       const LaneGroups lg = lane_groups(key, fixed_ok);
-      const int q0 = (b1 - mn1) * ww + (b0 - mn0);
+      const int q0 = __mul24(b1 - mn1, ww) + (b0 - mn0);
 #pragma unroll
       for (int jj = 0; jj < 3; ++jj) {
 #pragma unroll
@@ -801,8 +836,10 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
     for (int q = tid; q < nodes; q += BLOCK) {
       const int jj = window_row(q, rcp_ww), ii = q - jj * ww;
       const int node = (mn1 + jj) * nx + (mn0 + ii);
-      const float c0 = from_fixed_bits(lds64[q]), c1 = from_fixed_bits(lds64[WMAX + q]),
-                  cm = from_fixed_bits(lds64[2 * WMAX + q]);
+      const unsigned long long vm = lds64[2 * WMAX + q];
+      const unsigned n = biased_count(vm);  // emissions into this node (all three planes)
+      const float c0 = from_biased_bits(lds64[q], n), c1 = from_biased_bits(lds64[WMAX + q], n),
+                  cm = from_biased_bits(vm, n);
       if (c0 != 0.f) unsafeAtomicAdd(gw + node, c0);
       if (c1 != 0.f) unsafeAtomicAdd(gw + G + node, c1);
       if (cm != 0.f) unsafeAtomicAdd(gw + 2 * G + node, cm);
@@ -813,11 +850,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused(FusedArgs a) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       if (mt[p] < 0) continue;
-      const int gi = start + p * BLOCK + tid;
-      a.s.x[gi] = x[p];
+      const uint32_t li = p * BLOCK + tid;
+      elem(xs, li) = x[p];
       if (!DO_P2G || !a.lazy_vc) {
-        a.s.v[gi] = v[p];
-        a.s.C[gi] = C[p];
+        elem(vs, li) = v[p];
+        elem(Cs, li) = C[p];
       }
     }
   }

@@ -91,6 +91,28 @@ __device__ __forceinline__ float from_fixed_bits(unsigned long long v) {
   return (float)((double)(long long)v * kFixInv);
 }
 
+// The same encoding without the high-word correction: the bits of y + 1.5 * 2^52 are
+// kFixBias + y as an int64, so a sum of n such values is n * kFixBias + sum(y). Two VALU
+// operations per value. When one of the summed planes holds non-negative values whose sum
+// stays below 2^48 (and n < 8192), its top bits hold n * 0x4338 mod 2^16, from which
+// biased_count recovers n; from_biased_bits then removes n * kFixBias from every plane that
+// received the same n values.
+constexpr unsigned long long kFixBias = 0x4338000000000000ull;
+
+__device__ __forceinline__ unsigned long long fixed_bits_biased(float y) {
+  return __double_as_longlong((double)y + 6755399441055744.0);
+}
+
+// n from a plane of non-negative biased values: (v >> 51) = n * 2151 mod 2^13 (0x4338 =
+// 8 * 2151), and 4951 is the inverse of 2151 modulo 2^13.
+__device__ __forceinline__ unsigned biased_count(unsigned long long v) {
+  return ((unsigned)(v >> 51) * 4951u) & 8191u;
+}
+
+__device__ __forceinline__ float from_biased_bits(unsigned long long v, unsigned n) {
+  return from_fixed_bits(v - ((unsigned long long)(n * 0x43380000u) << 32));
+}
+
 // The same encoding with 40 fractional bits, for sums whose terms reach ~2^11 (frame
 // pixels): every pixel sum fits int64 with room to spare, at 2^-40 resolution.
 constexpr float kFix40Pre = 1099511627776.0f;          // 2^40
