@@ -35,8 +35,8 @@ Also reported on the same line:
   checkpoint every 200, per-instance gradients in one adjoint pass;
 * render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
   map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back;
-* axisym: the axisymmetric variant's rate on the same batch and on the tall view
-  configuration (windows of 8 substeps from fresh loads);
+* axisym: the axisymmetric variant's rate on 32 axisymmetric phenotypes (80K slots) and on
+  the tall view configuration (windows of 8 substeps from fresh loads);
 * tank_build_s: host seconds to build the 32 tanks (phenotype.population, the fill_tank
   restatement with its native steps).
 
@@ -342,10 +342,12 @@ def render_frames(eng, live: int, reps: int = 20) -> dict:
     return out
 
 
-def axisym_rates(pos, mat, fib, genomes, reps: int = 10, n: int = 8) -> dict:
+def axisym_rates(reps: int = 10, n: int = 8) -> dict:
     """The axisymmetric variant (JELLY_AXISYM=1, plain path, one launch per stage for the
-    batch): live particle-substeps/s on the same 32x80K population, and on
-    helpers/view_axisym.py's configuration (one seeded tank, 128x256, H=2, 160K slots).
+    batch): live particle-substeps/s on 32 seeded 80K-slot tanks filled as evolve.py fills
+    them under JELLY_AXISYM=1 (right-half profiles at the axis, make_jelly.py:276-304,343),
+    and on helpers/view_axisym.py's tank (Aurelia's right-half profile at spawn (0.5, 0.40),
+    128x256, H=2, 160K slots, view_axisym.py:19-25,129-135; default actuation parameters).
     The reference's variant diverges within ~10 substeps next to the axis (its affine
     momentum is not weighted by r), so each timed window is `n` substeps from a fresh load,
     `reps` windows, HIP events on the engine's stream."""
@@ -353,18 +355,23 @@ def axisym_rates(pos, mat, fib, genomes, reps: int = 10, n: int = 8) -> dict:
     from jellyfih_amd import phenotype
     from jellyfih_amd.engine import Engine
     out = {"window_substeps": n, "windows": reps}
-    cases = {"batch_32x80k": (pos, mat, fib, genomes, {}),
-             "view_tall_1x160k": None}
-    tall = phenotype.population([0], 160000, 128, domain_height=2.0)
-    cases["view_tall_1x160k"] = (tall[0], tall[1], tall[2], tall[3],
-                                 {"n_grid_y": 256, "domain_height": 2.0})
+    batch = phenotype.population(range(32), 80000, 128, axisym=True)
+    view = phenotype.fill_tank(phenotype.AURELIA_GENOME, 160000, 128,
+                               spawn_offset=phenotype.DEFAULT_SPAWN, domain_height=2.0,
+                               axisym=True)
+    cases = {"batch_32x80k": (batch[0], batch[1], batch[2], batch[3], {}),
+             "view_tall_1x160k": (view[0][None], view[1][None], view[2][None], None,
+                                  {"n_grid_y": 256, "domain_height": 2.0})}
     for name, (P, Mt, Fb, G, kw) in cases.items():
         eng = Engine(P.shape[0], P.shape[1], path="plain", axisym=True, **kw)
         stream = torch.cuda.ExternalStream(eng.stream)
         live = int((Mt >= 0).sum())
         total = 0.0
         for r in range(reps + 1):
-            load_engine(eng, P, Mt, Fb, G)
+            if G is None:
+                eng.load_particles(0, P[0], Mt[0], Fb[0])
+            else:
+                load_engine(eng, P, Mt, Fb, G)
             eng.set_time(0.0)
             eng.substep(1)               # commits the load; not timed
             eng.sync()
@@ -489,7 +496,7 @@ def main() -> int:
     log(f"live particles {live} (slots {M * N})")
 
     if args.only_axisym:
-        print(json.dumps({"axisym": axisym_rates(pos, mat, fib, genomes)}), flush=True)
+        print(json.dumps({"axisym": axisym_rates()}), flush=True)
         return 0
 
     if args.copy_probe:
@@ -589,7 +596,7 @@ def main() -> int:
     log(f"render {rend}")
 
     log("axisymmetric variant (plain path)")
-    axisym = axisym_rates(pos, mat, fib, genomes)
+    axisym = axisym_rates()
     log(f"axisym {axisym}")
 
     plain_value = None

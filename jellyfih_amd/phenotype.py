@@ -14,8 +14,15 @@ Bezier spine -> normals -> thickness envelope -> muscle band -> rasterised bell,
 bridge masks -> right half mirrored -> payload block -> spawn offset; and
 ``make_jelly.py:336-402`` (``fill_tank``): 4-per-cell water lattice minus a KD-tree
 exclusion zone around the robot, padded with dead slots.
+
+Axisymmetric mode (``make_jelly.py:11-14``, ``JELLY_AXISYM=1`` at import, or ``axisym=True``
+per call): the right half of the profile is the whole cross-section (no mirror), the
+payload block spans ``[0, PAYLOAD_WIDTH/2]``, and the default spawn is the axis
+``(0, 0.40)`` (``make_jelly.py:276-280,301-304,343``).
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 from scipy.spatial import cKDTree
@@ -23,6 +30,8 @@ from scipy.spatial import cKDTree
 PAYLOAD_WIDTH = 0.08                    # make_jelly.py:8
 PAYLOAD_HEIGHT = 0.05                   # make_jelly.py:9
 DEFAULT_SPAWN = np.array([0.5, 0.40])   # make_jelly.py:10
+AXISYM_SPAWN = np.array([0.0, 0.40])    # make_jelly.py:11
+AXISYM = os.environ.get("JELLY_AXISYM", "0") == "1"   # make_jelly.py:14
 TALL_SPAWN = np.array([0.5, 0.60])      # evolve.py:136
 
 # Genome bounds (evolve.py:58-59): 9 morphology genes, contraction_frac, freq_mult.
@@ -80,9 +89,12 @@ def _unit_normals(points: np.ndarray) -> np.ndarray:
     return nrm / length
 
 
-def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payload: bool = True):
+def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payload: bool = True,
+                       axisym: bool | None = None):
     """Robot particles of one genome: ``(pos f64 (n,2), mat int (n,), fiber f32 (n,2),
-    self_intersecting)`` (make_jelly.py:74-334, Cartesian mode)."""
+    self_intersecting)`` (make_jelly.py:74-334). ``axisym`` None follows ``AXISYM``."""
+    if axisym is None:
+        axisym = AXISYM
     if spawn_offset is None:
         spawn_offset = DEFAULT_SPAWN
     g = np.asarray(genome, dtype=np.float64)
@@ -182,7 +194,9 @@ def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payl
         _, idx = cKDTree(spine).query(rpos[is_mus])
         rfib[is_mus] = spine_tan[idx]
 
-    if len(rpos) > 0:
+    if axisym:                      # the right half is the cross-section (make_jelly.py:276-280)
+        soft_pos, soft_mat, soft_fib = rpos, rmat, rfib
+    elif len(rpos) > 0:
         lpos = rpos.copy()
         lpos[:, 0] *= -1
         lfib = rfib.copy()
@@ -202,7 +216,10 @@ def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payl
         parts_mat.append(soft_mat)
         parts_fib.append(soft_fib)
     if with_payload:
-        px = np.linspace(-PAYLOAD_WIDTH / 2, PAYLOAD_WIDTH / 2, int(PAYLOAD_WIDTH * raster))
+        if axisym:                  # make_jelly.py:301-302
+            px = np.linspace(0, PAYLOAD_WIDTH / 2, int(PAYLOAD_WIDTH / 2 * raster))
+        else:
+            px = np.linspace(-PAYLOAD_WIDTH / 2, PAYLOAD_WIDTH / 2, int(PAYLOAD_WIDTH * raster))
         py = np.linspace(0, PAYLOAD_HEIGHT, int(PAYLOAD_HEIGHT * raster))
         pgx, pgy = np.meshgrid(px, py)
         pay = np.vstack([pgx.ravel(), pgy.ravel()]).T
@@ -244,12 +261,17 @@ def water_mask(body, cands, margin: float) -> np.ndarray:
 
 def fill_tank(genome, max_particles: int, grid_res: int = 128, spawn_offset=None,
               water_margin: float = 0.005, with_payload: bool = True,
-              domain_height: float = 1.0):
+              domain_height: float = 1.0, axisym: bool | None = None):
     """Fixed-size particle arrays for one instance (make_jelly.py:336-402).
 
-    Returns ``pos (N,2) f32, mat (N,) i32, fiber (N,2) f32, info dict``."""
+    Returns ``pos (N,2) f32, mat (N,) i32, fiber (N,2) f32, info dict``. ``axisym`` None
+    follows ``AXISYM``; the default spawn is then the axis (make_jelly.py:342-343)."""
+    if axisym is None:
+        axisym = AXISYM
+    if spawn_offset is None:
+        spawn_offset = AXISYM_SPAWN if axisym else DEFAULT_SPAWN
     robot_pos, robot_mat, robot_fib, self_int = generate_phenotype(
-        genome, spawn_offset, grid_res=grid_res, with_payload=with_payload)
+        genome, spawn_offset, grid_res=grid_res, with_payload=with_payload, axisym=axisym)
     n_robot = len(robot_pos)
     cands = water_lattice(grid_res, domain_height)
     if n_robot > 0:
@@ -279,7 +301,7 @@ def fill_tank(genome, max_particles: int, grid_res: int = 128, spawn_offset=None
 
 
 def population(seeds, max_particles: int = 80000, grid_res: int = 128,
-               domain_height: float = 1.0, spawn_offset=None):
+               domain_height: float = 1.0, spawn_offset=None, axisym: bool | None = None):
     """Fill one tank per seeded genome; returns (pos (M,N,2), mat (M,N), fiber (M,N,2),
     genomes (M,11), infos)."""
     seeds = list(seeds)
@@ -293,6 +315,6 @@ def population(seeds, max_particles: int = 80000, grid_res: int = 128,
         g = seeded_genome(s)
         genomes[i] = g
         pos[i], mat[i], fib[i], info = fill_tank(g, max_particles, grid_res, spawn_offset,
-                                                 domain_height=domain_height)
+                                                 domain_height=domain_height, axisym=axisym)
         infos.append(info)
     return pos, mat, fib, genomes, infos

@@ -50,11 +50,12 @@ def small_tank(seed: int, n_slots: int, region=(0.3, 0.7, 0.25, 0.65), ppc_spaci
     return pos, mats, fibers
 
 
-def reference_tank(seed, n_slots=80000, domain_height=1.0, spawn=None):
-    """fill_tank of the seeded genome (make_jelly.py:336-402 restated)."""
+def reference_tank(seed, n_slots=80000, domain_height=1.0, spawn=None, axisym=False):
+    """fill_tank of the seeded genome, or of the Aurelia genome for a string seed
+    (make_jelly.py:336-402 restated; axisym: the right-half profile of JELLY_AXISYM=1)."""
     g = phenotype.seeded_genome(seed) if not isinstance(seed, str) else phenotype.AURELIA_GENOME
     pos, mat, fib, info = phenotype.fill_tank(g, n_slots, 128, spawn_offset=spawn,
-                                              domain_height=domain_height)
+                                              domain_height=domain_height, axisym=axisym)
     return pos, mat, fib, g, info
 
 

@@ -9,11 +9,17 @@ genomes, and writes ``phenotype_golden.npz``: per case the genome, the robot par
 bits) and the counts. Nothing on the GPU side reads the reference; tests only read the
 ``.npz``.
 
+The axisymmetric cases (``make_jelly._AXISYM``, read from ``JELLY_AXISYM`` when the module
+is imported) reload ``make_jelly`` with ``JELLY_AXISYM=1``: one at the default spawn (the
+axis), one as helpers/view_axisym.py builds its tank (Aurelia, 160K slots, H = 2, spawn
+(0.5, 0.40)).
+
 Usage: python tests/golden/make_golden.py [--reference /root/reference]
 """
 from __future__ import annotations
 
 import argparse
+import importlib
 import os
 import sys
 
@@ -24,13 +30,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LOWER = np.array([0.0, -0.15, 0.0, -0.2, 0.05, -0.30, 0.025, 0.025, 0.01, 0.05, 0.5])
 UPPER = np.array([0.25, 0.15, 0.3, 0.15, 0.35, +0.10, 0.08, 0.1, 0.04, 0.60, 2.0])
 
-# (name, genome source, max_particles, domain_height, spawn)
+# (name, genome source, max_particles, domain_height, spawn, axisymmetric)
 CASES = [
-    ("aurelia", "aurelia", 80000, 1.0, None),
-    ("seed0", 0, 80000, 1.0, None),
-    ("seed1", 1, 80000, 1.0, None),
-    ("seed2", 2, 80000, 1.0, None),
-    ("tall_seed0", 0, 160000, 2.0, (0.5, 0.60)),
+    ("aurelia", "aurelia", 80000, 1.0, None, False),
+    ("seed0", 0, 80000, 1.0, None, False),
+    ("seed1", 1, 80000, 1.0, None, False),
+    ("seed2", 2, 80000, 1.0, None, False),
+    ("tall_seed0", 0, 160000, 2.0, (0.5, 0.60), False),
+    ("axisym_seed0", 0, 80000, 1.0, None, True),
+    ("axisym_view_aurelia", "aurelia", 160000, 2.0, (0.5, 0.40), True),
 ]
 
 
@@ -40,10 +48,15 @@ def main() -> None:
     ap.add_argument("--out", default=os.path.join(HERE, "phenotype_golden.npz"))
     args = ap.parse_args()
     sys.path.insert(0, args.reference)
+    os.environ["JELLY_AXISYM"] = "0"
     import make_jelly  # noqa: E402  (reference generator, run here only)
 
     out = {}
-    for name, src, nmax, height, spawn in CASES:
+    for name, src, nmax, height, spawn, axisym in CASES:
+        if make_jelly._AXISYM != axisym:
+            os.environ["JELLY_AXISYM"] = "1" if axisym else "0"
+            make_jelly = importlib.reload(make_jelly)
+            assert make_jelly._AXISYM == axisym
         if src == "aurelia":
             genome = np.asarray(make_jelly.AURELIA_GENOME, dtype=np.float64)
         else:
@@ -77,6 +90,7 @@ def main() -> None:
                                          dtype=np.int64)
         out[f"{name}/domain"] = np.array([height] + ([-1.0, -1.0] if spawn is None
                                                      else list(spawn)))
+        out[f"{name}/axisym"] = np.array(int(axisym))
         print(f"{name}: robot={n_robot} water={n_water} muscle={info['muscle_count']} "
               f"self_int={info['self_intersecting']}")
     np.savez_compressed(args.out, **out)

@@ -117,32 +117,49 @@ def test_axisym_40_substeps_with_a_sort():
     eng.close()
 
 
-@pytest.mark.parametrize("case", ["axis", "tall"])
+@pytest.mark.parametrize("case", ["axis", "evolve", "view"])
 def test_axisym_5_substeps(case):
-    """Five substeps of tanks that reach the axis: small tanks, and helpers/view_axisym.py's
-    configuration (one reference tank in the tall domain, 128x256, H = 2, 160K slots)."""
-    say(f"axisym 5 substeps, {case}")
+    """Five substeps of tanks that reach the axis: small tanks; two axisymmetric phenotypes
+    as evolve.py fills them under JELLY_AXISYM=1 (right-half profile at the axis, spawn
+    (0, 0.40), make_jelly.py:276-304,343); and helpers/view_axisym.py's tank (Aurelia's
+    right-half profile at spawn (0.5, 0.40) in the tall domain, 128x256, H = 2, 160K slots,
+    view_axisym.py:19-25,129-135).
+
+    The axis-spawned phenotypes put payload and bell particles at r = 0, where the
+    reference's formulation gives them no grid mass but full affine momentum: the fp64
+    oracle's largest |v| is 1.8e3 after one substep, 6.4e5 after two, 9e17 after three, and
+    fp32 overflows at the fourth. That case compares two substeps."""
+    steps = 2 if case == "evolve" else 5
+    say(f"axisym {steps} substeps, {case}")
+    params = None
     if case == "axis":
         cases, consts, kw = axis_cases(), None, {}
-        params = None
+    elif case == "evolve":
+        tanks = [reference_tank(s, n_slots=80000, axisym=True) for s in (0, 1)]
+        cases = [t[:3] for t in tanks]
+        consts, kw = None, {}
+        params = [(float(np.clip(t[3][9], 0.05, 0.6)), float(np.clip(t[3][10], 0.5, 2.0)))
+                  for t in tanks]
     else:
-        pos, mat, fib, g, info = reference_tank(0, n_slots=160000, domain_height=2.0)
+        pos, mat, fib, g, info = reference_tank("aurelia", n_slots=160000, domain_height=2.0,
+                                                spawn=np.array([0.5, 0.40]), axisym=True)
         cases = [(pos, mat, fib)]
         consts = oracle.reference_consts(n_grid_y=256, domain_height=2.0)
         kw = {"n_grid_y": 256, "domain_height": 2.0}
-        params = (float(np.clip(g[9], 0.05, 0.6)), float(np.clip(g[10], 0.5, 2.0)))
     eng, (o32, o64) = make_pair(cases, consts=consts, dtypes=(np.float32, np.float64), **kw)
     if params:
-        eng.set_param("contraction", params[0])
-        eng.set_param("freq", params[1])
+        con = np.array([p[0] for p in params], np.float32)
+        frq = np.array([p[1] for p in params], np.float32)
+        eng.set_param("contraction", con)
+        eng.set_param("freq", frq)
         for o in (o32, o64):
-            o.contraction[:] = np.float32(params[0])
-            o.freq[:] = np.float32(params[1])
-    eng.substep(5)
-    o32.run(5, nthreads=8)
-    o64.run(5, nthreads=8)
+            o.contraction[:] = con
+            o.freq[:] = frq
+    eng.substep(steps)
+    o32.run(steps, nthreads=8)
+    o64.run(steps, nthreads=8)
     live = o32.material >= 0
-    assert np.isfinite(o64.v[live]).all()
+    assert np.isfinite(o64.v[live]).all() and np.isfinite(o32.v[live]).all()
     for n, a32, a64 in (("x", o32.x, o64.x), ("v", o32.v, o64.v), ("F", o32.F, o64.F),
                         ("C", o32.C, o64.C)):
         fp32_like(n, eng.read(n)[live], a32[live], a64[live])

@@ -1,13 +1,17 @@
 """The input generator against fixtures written from the reference's make_jelly.fill_tank
-(tests/golden/make_golden.py). Bit-exact: positions, materials, fibres, water mask, counts."""
+(tests/golden/make_golden.py). Bit-exact: positions, materials, fibres, water mask, counts,
+in Cartesian and axisymmetric mode (JELLY_AXISYM=1: right half only, make_jelly.py:276-304)."""
 from __future__ import annotations
+
+import os
 
 import numpy as np
 import pytest
 
 from jellyfih_amd import phenotype
 
-CASES = ["aurelia", "seed0", "seed1", "seed2", "tall_seed0"]
+CASES = ["aurelia", "seed0", "seed1", "seed2", "tall_seed0", "axisym_seed0",
+         "axisym_view_aurelia"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -17,8 +21,9 @@ def test_fill_tank_matches_reference_fixture(golden, name):
     dom = golden[f"{name}/domain"]
     spawn = None if dom[1] < 0 else dom[1:]
     nmax = int(counts[0])
+    axisym = bool(golden[f"{name}/axisym"])
     pos, mat, fib, info = phenotype.fill_tank(g, nmax, 128, spawn_offset=spawn,
-                                              domain_height=float(dom[0]))
+                                              domain_height=float(dom[0]), axisym=axisym)
     nr, nw = info["n_robot"], info["n_water"]
     assert (nr, nw, info["muscle_count"], info["n_jelly"]) == tuple(int(c) for c in counts[1:5])
     assert int(info["self_intersecting"]) == int(counts[5])
@@ -30,6 +35,29 @@ def test_fill_tank_matches_reference_fixture(golden, name):
     np.testing.assert_array_equal(pos[nr:nr + nw], cands[keep])
     assert np.all(mat[nr:nr + nw] == 0)
     assert np.all(mat[nr + nw:] == -1) and np.all(pos[nr + nw:] == -1)
+
+
+def test_axisym_mode_from_the_environment():
+    """JELLY_AXISYM=1 at import selects the axisymmetric branch by default, as the reference's
+    module flag does (make_jelly.py:14,342-343): same robot as axisym=True, spawn at the axis,
+    every robot particle at r >= 0, and the payload block only on the right half."""
+    import subprocess
+    import sys
+    code = ("import numpy as np; from jellyfih_amd import phenotype as p; "
+            "a = p.fill_tank(p.AURELIA_GENOME, 80000); "
+            "b = p.fill_tank(p.AURELIA_GENOME, 80000, axisym=True); "
+            "assert p.AXISYM; n = a[3]['n_robot']; "
+            "assert n == b[3]['n_robot'] and np.array_equal(a[0], b[0]); "
+            "pay = a[0][:n][a[1][:n] == 2]; "
+            "assert a[0][:n, 0].min() >= 0 and pay[:, 0].min() == 0 and "
+            "abs(pay[:, 0].max() - 0.04) < 1e-7; print('ok')")
+    env = dict(os.environ, JELLY_AXISYM="1")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr
+    full = phenotype.fill_tank(phenotype.AURELIA_GENOME, 80000, axisym=False)
+    half = phenotype.fill_tank(phenotype.AURELIA_GENOME, 80000, axisym=True)
+    assert half[3]["n_robot"] < full[3]["n_robot"]
 
 
 def test_seeded_genomes_within_bounds():
