@@ -102,6 +102,43 @@ def traffic_from_profiles():
         return json.load(f)
 
 
+def roofline_leg(eng, live: int, groups: int) -> dict:
+    """One launch per substep over every live particle (instance_groups = 1): the fused
+    kernel's average duration from HIP events on the engine's stream around the n - 1
+    fused G2P->P2G kernels of 3 directly launched blocks; then, when `groups` > 1, the tuned
+    concurrent instance groups' wall time per substep, event-timed the same way."""
+    eng.set_tuning(instance_groups=1)
+    single = [eng.timed_block(BLOCK)[0] for _ in range(3)]
+    single_ms = float(np.mean(single))
+    algo = ALGO_BYTES * live
+    achieved = algo / (single_ms * 1e-3)
+    tr = traffic_from_profiles()
+    traffic = tr.get("bytes_per_launch") if tr and tr.get("live_particles") == live else None
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "kernel": "k_fused<256,4,true,true>, one launch per substep over all "
+                          f"{live} live particles (instance_groups=1)",
+                "kernel_ms": single_ms, "kernel_ms_blocks": single,
+                "algorithmic_bytes_per_launch": algo,
+                "traffic_frac": (traffic / (single_ms * 1e-3) / HBM_PEAK) if traffic else None,
+                "traffic_measured": ({k: tr.get(k) for k in ("commit", "command", "source")}
+                                     if traffic else None)}
+    msg = f"fused kernel, one launch {single_ms * 1e3:.1f} us -> {achieved / 1e9:.0f} GB/s"
+    if groups > 1:
+        eng.set_tuning(instance_groups=groups)
+        conc_ms = float(np.mean([eng.timed_block(BLOCK)[0] for _ in range(3)]))
+        roofline["event_timed_concurrent"] = {
+            "instance_groups": groups,
+            "substep_ms": conc_ms, "achieved": algo / (conc_ms * 1e-3) / 1e9,
+            "frac": algo / (conc_ms * 1e-3) / HBM_PEAK,
+            "note": "wall time of one substep whose instance groups' launches run "
+                    "concurrently on separate streams (HIP events on the main stream); not a "
+                    "kernel duration"}
+        msg += f" algorithmic; concurrent groups {conc_ms * 1e3:.1f} us per substep"
+    log(msg)
+    return roofline
+
+
 def cpu_baseline(pos, mat, fib, genomes, budget_s: float = 12.0) -> dict:
     """The oracle fp32 on host cores, one instance per thread, sized to ~budget_s."""
     import oracle
@@ -457,6 +494,8 @@ def main() -> int:
                     help="run only the batched tall-tank forward + backward")
     ap.add_argument("--only-render", action="store_true",
                     help="run only the renderer leg (after the warm-up blocks)")
+    ap.add_argument("--only-roofline", action="store_true",
+                    help="warm up and time the fused kernel with one instance group only")
     ap.add_argument("--only-axisym", action="store_true",
                     help="run only the axisymmetric variant's leg")
     ap.add_argument("--batched-steps", type=int, default=4000)
@@ -510,11 +549,20 @@ def main() -> int:
     eng = Engine(M, N)
     load_engine(eng, pos, mat, fib, genomes)
     tuning = load_tuning(eng)
-    log(f"tuning {tuning}")
+    if args.only_roofline:
+        # every fused dispatch of this process is then one launch over all live particles,
+        # so a kernel trace's average for k_fused<256,4,true,true> is the measured figure
+        eng.set_tuning(instance_groups=1)
+    log(f"tuning {eng.tuning()}")
     warm = max(args.warmup * BLOCK, MIN_WARM)
     log(f"warmup {warm} substeps")
     eng.substep(warm)
     eng.sync()
+    if args.only_roofline:
+        print(json.dumps({"roofline": roofline_leg(eng, live, 1), "warmup_substeps": warm,
+                          "live_particles": live}), flush=True)
+        eng.close()
+        return 0
 
     if args.only_render:
         print(json.dumps({"render": render_frames(eng, live)}), flush=True)
@@ -544,36 +592,7 @@ def main() -> int:
     value = live * substeps / wall
     log(f"median of {REPS}: wall {wall:.4f}s -> {value:.3e} particle-substeps/s")
 
-    # Roofline. One launch per substep over every live particle (instance_groups = 1):
-    # average kernel duration from HIP events on the engine's stream around n - 1 fused
-    # G2P->P2G kernels (3 blocks, launched directly); then the tuned concurrent groups'
-    # wall time per substep, event-timed the same way.
-    eng.set_tuning(instance_groups=1)
-    single = [eng.timed_block(BLOCK)[0] for _ in range(3)]
-    eng.set_tuning(instance_groups=tuning["instance_groups"])
-    concurrent = [eng.timed_block(BLOCK)[0] for _ in range(3)]
-    single_ms = float(np.mean(single))
-    conc_ms = float(np.mean(concurrent))
-    algo = ALGO_BYTES * live
-    achieved = algo / (single_ms * 1e-3)
-    tr = traffic_from_profiles()
-    traffic = tr.get("bytes_per_launch") if tr and tr.get("live_particles") == live else None
-    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "kernel": "k_fused<256,4,G2P,P2G>, one launch per substep over all "
-                          f"{live} live particles (instance_groups=1)",
-                "kernel_ms": single_ms, "algorithmic_bytes_per_launch": algo,
-                "traffic_frac": (traffic / (single_ms * 1e-3) / HBM_PEAK) if traffic else None,
-                "traffic_measured": ({k: tr.get(k) for k in ("commit", "command", "source")}
-                                     if traffic else None),
-                "event_timed_concurrent": {
-                    "instance_groups": tuning["instance_groups"],
-                    "substep_ms": conc_ms, "achieved": algo / (conc_ms * 1e-3) / 1e9,
-                    "frac": algo / (conc_ms * 1e-3) / HBM_PEAK,
-                    "note": "wall time of one substep whose instance groups' launches run "
-                            "concurrently on two streams (HIP events on the main stream)"}}
-    log(f"fused kernel, one launch {single_ms * 1e3:.1f} us -> {achieved / 1e9:.0f} GB/s "
-        f"algorithmic; concurrent groups {conc_ms * 1e3:.1f} us per substep")
+    roofline = roofline_leg(eng, live, tuning["instance_groups"])
 
     episode = None
     if not args.no_episode:
