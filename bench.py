@@ -550,17 +550,20 @@ def main() -> int:
     load_engine(eng, pos, mat, fib, genomes)
     tuning = load_tuning(eng)
     if args.only_roofline:
-        # every fused dispatch of this process is then one launch over all live particles,
-        # so a kernel trace's average for k_fused<256,4,true,true> is the measured figure
-        eng.set_tuning(instance_groups=1)
+        # The warm-up runs with another launch shape (k_fused<256, 8, ...>) and the tuned
+        # shape returns for the timed blocks, so the only k_fused<256,4,true,true> dispatches
+        # of this process are the measured ones, each one launch over all live particles:
+        # a kernel trace's average for that kernel is the figure reported.
+        eng.set_tuning(instance_groups=1, threads=256, ppt=8)
     log(f"tuning {eng.tuning()}")
     warm = max(args.warmup * BLOCK, MIN_WARM)
     log(f"warmup {warm} substeps")
     eng.substep(warm)
     eng.sync()
     if args.only_roofline:
+        eng.set_tuning(threads=tuning["threads"], ppt=tuning["ppt"])
         print(json.dumps({"roofline": roofline_leg(eng, live, 1), "warmup_substeps": warm,
-                          "live_particles": live}), flush=True)
+                          "live_particles": live, "tuning": eng.tuning()}), flush=True)
         eng.close()
         return 0
 
