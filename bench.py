@@ -35,6 +35,9 @@ Also reported on the same line:
   checkpoint every 200, per-instance gradients in one adjoint pass;
 * render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
   map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back;
+* view_mode: evolve.py's view loop through the mpm_sim facade (16 x 80K, 500 substep()
+  calls from Python, then one abyss frame read back and encoded), against the same loop
+  with one engine call per substep;
 * axisym: the axisymmetric variant's rate on 32 axisymmetric phenotypes (80K slots) and on
   the tall view configuration (windows of 8 substeps from fresh loads);
 * tank_build_s: host seconds to build the 32 tanks (phenotype.population, the fill_tank
@@ -322,6 +325,68 @@ WEB_PALETTE = [(0, 0.933, 0.949, 0.957), (1, 0.678, 0.882, 0.875), (3, 0.976, 0.
                (2, 0.961, 0.824, 0.576)]     # mpm_sim.py:60-65
 
 
+def view_mode(frames: int = 4, every: int = 500) -> dict:
+    """evolve.py's view mode through the mpm_sim facade (evolve.py:375-398): 16 instances
+    (the reference's JELLY_INSTANCES default) x 80K slots, seeds 0-15; per frame `every`
+    substep() calls from a Python loop, then the abyss frame (clear, splat, tone map),
+    ti.sync(), frame_buffer.to_numpy() and the 8-bit encode on the host (evolve.py:396).
+    One untimed frame, then `frames` timed ones (wall clock). Reported beside it: the same
+    substep() loop with one engine call per substep (no queueing), as it ran before the
+    facade queued its substeps."""
+    import math
+    import jellyfih_amd.mpm_sim as sim
+    from jellyfih_amd import phenotype
+    M = sim.n_instances
+    genomes = []
+    for i in range(M):
+        g = phenotype.seeded_genome(i)
+        genomes.append(g)
+        pos, mat, fib, _ = phenotype.fill_tank(g, sim.n_particles, 128)
+        sim.load_particles(i, pos, mat, fib)
+        sim.instance_act_contraction[i] = float(np.clip(g[9], 0.05, 0.60))
+        sim.instance_freq[i] = float(np.clip(g[10], 0.5, 2.0))
+    live = int((sim.material.to_numpy() >= 0).sum())
+    sim.sim_time[None] = 0.0
+    radius = max(1.0, sim.res_sub / sim.n_grid * 0.6)       # evolve.py:380
+
+    def frame():
+        t0 = time.perf_counter()
+        for _ in range(every):
+            sim.substep()
+        sim.ti.sync()
+        t1 = time.perf_counter()
+        sim.clear_frame_buffer()
+        sim.render_frame_abyss(sim.res_sub, sim.grid_side, radius)
+        sim.tone_map_and_encode()
+        sim.ti.sync()
+        img = sim.frame_buffer.to_numpy()
+        (np.clip(img, 0, 1) * 255).astype(np.uint8)
+        return t1 - t0, time.perf_counter() - t1
+
+    frame()
+    parts = [frame() for _ in range(frames)]
+    step_s = sum(p[0] for p in parts) / frames
+    draw_s = sum(p[1] for p in parts) / frames
+    e = sim.engine()
+    e.substep(1)
+    e.sync()
+    t0 = time.perf_counter()
+    for _ in range(every):
+        e.substep(1)
+    e.sync()
+    percall_s = time.perf_counter() - t0
+    out = {"instances": M, "live_particles": live, "substeps_per_frame": every,
+           "frames": frames, "frame_s": step_s + draw_s, "substeps_s": step_s,
+           "render_readback_encode_s": draw_s,
+           "particle_substeps_per_s": live * every / step_s,
+           "with_frames_particle_substeps_per_s": live * every / (step_s + draw_s),
+           "unqueued_substeps_s": percall_s,
+           "unqueued_particle_substeps_per_s": live * every / percall_s}
+    e.close()
+    sim._engine = None
+    return out
+
+
 def render_frames(eng, live: int, reps: int = 20) -> dict:
     """One 1024x1024 video frame of the whole batch per pipeline, as evolve.py's view mode
     and the helpers draw it (mpm_sim.py:343-712): the abyss glow splat + tone map, the web
@@ -494,6 +559,9 @@ def main() -> int:
                     help="run only the batched tall-tank forward + backward")
     ap.add_argument("--only-render", action="store_true",
                     help="run only the renderer leg (after the warm-up blocks)")
+    ap.add_argument("--no-view", action="store_true")
+    ap.add_argument("--only-view", action="store_true",
+                    help="run only the facade's view-mode leg")
     ap.add_argument("--only-roofline", action="store_true",
                     help="warm up and time the fused kernel with one instance group only")
     ap.add_argument("--only-axisym", action="store_true",
@@ -525,6 +593,9 @@ def main() -> int:
                                                   checkpoint=args.batched_checkpoint,
                                                   resort=args.diff_resort)
         print(json.dumps(out), flush=True)
+        return 0
+    if args.only_view:
+        print(json.dumps({"view_mode": view_mode()}), flush=True)
         return 0
     M, N = args.instances, args.slots
     log(f"building workload: {M} genomes x {N} slots")
@@ -615,6 +686,11 @@ def main() -> int:
 
     log("renderer (one video frame per pipeline)")
     rend = render_frames(eng, live)
+    view = None
+    if not args.no_view:
+        log("view mode through the facade (500 substep() calls + one frame)")
+        view = view_mode()
+        log(f"view {view}")
     log(f"render {rend}")
 
     log("axisymmetric variant (plain path)")
@@ -686,6 +762,7 @@ def main() -> int:
         "batch16": b16,
         "diff": diff,
         "render": rend,
+        "view_mode": view,
         "axisym": axisym,
         "tank_build_s": t_build,
     }

@@ -13,6 +13,14 @@ those names and their meaning, backed by the HIP engine (``libjellymx.so``):
 * ``substep()``, ``load_particles()``, ``get_payload_stats()``, ``run_batch_headless()``
   and the fitness-buffer kernels behave as in the reference.
 
+``substep()`` queues its substep, as a Taichi kernel launch does. Queued substeps run as one
+block (HIP-graph replay, one fused launch per substep) once a block's worth has
+accumulated, and before anything else reaches the engine: a field read or write, a render,
+a stats call, ``sync()``/``ti.sync()``. A caller that steps in a Python loop
+(``evolve.py:389-390``, ``run_population.py:44-52``) therefore gets the block rate rather
+than two launches per call, and sees the same state at every read as with one launch per
+call.
+
 The device engine is created on first use, so importing this module never touches a GPU.
 The renderer (``mpm_sim.py:343-712``) draws into a device frame buffer read back with
 ``frame_buffer.to_numpy()``. ``JELLY_AXISYM=1`` selects the axisymmetric variant
@@ -73,6 +81,8 @@ grid_side = int(math.ceil(math.sqrt(n_instances)))
 res_sub = video_res // grid_side
 
 _engine = None
+_pending = 0        # substep() calls queued and not yet sent to the engine
+_pending_max = 100  # the engine's block length, read when the engine is created
 
 
 def _tuning_for(instances: int, slots: int, path: str | None = None):
@@ -94,8 +104,22 @@ def _tuning_for(instances: int, slots: int, path: str | None = None):
 
 def engine():
     """The device engine behind this module (created on first use), with the autotuner's
-    launch parameters when they were measured on this batch shape."""
-    global _engine
+    launch parameters when they were measured on this batch shape. Queued substeps run
+    first, so whatever the caller does next sees them."""
+    e = _device()
+    _flush(e)
+    return e
+
+
+def _flush(e) -> None:
+    global _pending
+    if _pending:
+        n, _pending = _pending, 0
+        e.substep(n)
+
+
+def _device():
+    global _engine, _pending_max
     if _engine is None:
         from .engine import Engine
         # the axisymmetric variant (JELLY_AXISYM=1) runs on the plain path
@@ -108,6 +132,7 @@ def engine():
                                                                          n_particles)
         if tuned:
             _engine.set_tuning(**{k: v for k, v in tuned.items() if k in _engine.tuning()})
+        _pending_max = max(1, int(_engine.tuning().get("block_substeps", 100)))
     return _engine
 
 
@@ -242,8 +267,13 @@ water_angle_color = _HostField(0)
 
 # --- kernels -------------------------------------------------------------------------------
 def substep() -> None:
-    """One MLS-MPM substep of every instance (mpm_sim.py:153-340)."""
-    engine().substep(1)
+    """One MLS-MPM substep of every instance (mpm_sim.py:153-340), queued: it runs with the
+    next block of queued substeps, at the latest before the next access to the engine."""
+    global _pending
+    e = _device()
+    _pending += 1
+    if _pending >= _pending_max:
+        _flush(e)
 
 
 def step(n: int) -> None:
@@ -313,7 +343,7 @@ class _TaichiNamespace:
     @staticmethod
     def sync() -> None:
         if _engine is not None:  # nothing can be queued before the engine exists
-            _engine.sync()
+            engine().sync()
 
 
 ti = _TaichiNamespace()

@@ -5,7 +5,9 @@ It does what ``evolve.py`` does with the simulator, with the same calls:
 ``load_batch`` (evolve.py:127-155), ``run_batch_headless`` (evolve.py:474-478), then a few
 view-mode frames (evolve.py:78-87,375-398: ``sim_time[None] = 0``, ``substep()`` loops,
 ``clear_frame_buffer`` / ``render_frame_abyss`` / ``tone_map_and_encode`` /
-``frame_buffer.to_numpy()``, and the web palette's flat passes), and saves what it read back.
+``frame_buffer.to_numpy()``, and the web palette's flat passes), then 137 queued
+``substep()`` calls against the same substeps asked of the engine directly, and saves what
+it read back.
 
 Usage: python -m tests.facade_driver OUT.npz   (JELLY_INSTANCES / JELLY_PARTICLES from env)
 """
@@ -68,7 +70,28 @@ def main(out_path: str) -> int:
     sim.render_vorticity_rdbu(sim.res_sub, sim.grid_side, 0.6)
     rdbu = sim.frame_buffer.to_numpy()
     print("[driver] frames done", flush=True)
-    np.savez(out_path, results=results, muscle_counts=np.array(muscle_counts),
+
+    # 137 queued substep() calls (a block of 100, then 37 at the read) against the same
+    # substeps asked of the engine directly
+    def reload():
+        for i, genome in enumerate(genomes):
+            pos, mat, fiber, _ = phenotype.fill_tank(
+                genome, sim.n_particles, grid_res=int(sim.n_grid),
+                domain_height=sim.domain_height)
+            sim.load_particles(i, pos, mat, fiber)
+        sim.sim_time[None] = 0.0
+
+    reload()
+    for _ in range(137):
+        sim.substep()
+    x_queued, t_queued = sim.x.to_numpy(), float(sim.sim_time[None])
+    reload()
+    sim.engine().substep(100)
+    sim.engine().substep(37)
+    x_direct, t_direct = sim.x.to_numpy(), float(sim.sim_time[None])
+    print("[driver] queued substeps done", flush=True)
+    np.savez(out_path, results=results, x_queued=x_queued, t_queued=t_queued,
+             x_direct=x_direct, t_direct=t_direct, muscle_counts=np.array(muscle_counts),
              genomes=np.array(genomes), hdr=hdr, ldr=ldr, ldr_u8=ldr_u8, web=web, t=t,
              radius=radius,
              grid_v=grid_v, grid_m=grid_m, vort=vort, base=base, rdbu=rdbu,

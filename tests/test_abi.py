@@ -6,6 +6,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from jellyfih_amd import _lib
@@ -148,3 +149,59 @@ def test_facade_uses_the_autotuned_parameters_for_their_batch_shape(tmp_path):
     assert t is not None and t["threads"] * t["ppt"] > 0 and t["sort_interval"] >= 1
     assert s._tuning_for(16, 80000) is None and s._tuning_for(32, 160000) is None
     assert s._tuning_for(32, 80000, path=str(tmp_path / "missing.json")) is None
+
+
+def test_facade_queues_substeps_until_the_next_access(monkeypatch):
+    """substep() queues (a Taichi launch is asynchronous too): queued substeps go to the
+    engine in blocks of the engine's block length, and all of them before any field read,
+    parameter write, render or sync, in call order. Host logic, with a stand-in engine."""
+    from jellyfih_amd import mpm_sim as s
+
+    class Recorder:
+        def __init__(self):
+            self.calls = []
+
+        def tuning(self):
+            return {"block_substeps": 100}
+
+        def substep(self, n):
+            self.calls.append(("substep", n))
+
+        def read(self, name):
+            self.calls.append(("read", name))
+            return np.zeros(1)
+
+        def get_param(self, name):
+            self.calls.append(("get_param", name))
+            return np.zeros(s.n_instances, np.float32)
+
+        def set_param(self, name, values):
+            self.calls.append(("set_param", name))
+
+        def get_time(self):
+            self.calls.append(("get_time",))
+            return 0.0
+
+        def sync(self):
+            self.calls.append(("sync",))
+
+    rec = Recorder()
+    monkeypatch.setattr(s, "_engine", rec)
+    monkeypatch.setattr(s, "_pending", 0)
+    monkeypatch.setattr(s, "_pending_max", 100)
+    for _ in range(250):
+        s.substep()
+    assert rec.calls == [("substep", 100), ("substep", 100)]
+    s.x.to_numpy()
+    assert rec.calls[2:] == [("substep", 50), ("read", "x")]
+    for _ in range(3):
+        s.substep()
+    s.instance_actuation[0] = 1.0
+    s.step(7)
+    s.sim_time[None]
+    s.ti.sync()
+    s.ti.sync()
+    assert rec.calls[4:] == [("substep", 3), ("get_param", "actuation"),
+                             ("set_param", "actuation"), ("substep", 7), ("get_time",),
+                             ("sync",), ("sync",)], rec.calls
+    assert s._pending == 0

@@ -9,7 +9,9 @@ environment), checked against the oracle on the same population:
   frame against the oracle's tone map of that HDR frame (8 ulps) and its 8-bit encode
   (frame_buffer.to_uint8) bit-exact, the web palette bit-exact;
   ``vort_grid`` against the oracle's vorticity of the ``grid_v``/``grid_m`` read back (1e-6)
-  and the RdBu composite against the oracle's over the same tone-mapped frame (4 ulps).
+  and the RdBu composite against the oracle's over the same tone-mapped frame (4 ulps);
+* 137 queued ``substep()`` calls against the same substeps asked of the engine directly:
+  the clock bit-exact, positions within 1e-5.
 """
 from __future__ import annotations
 
@@ -72,6 +74,13 @@ def test_evolve_style_driver(tmp_path):
         render.render_flat_pass(web, d["x"], d["material"], mat_id, (r, g, b),
                                 int(d["res_sub"]), int(d["grid_side"]), float(d["radius"]), 1.0)
     np.testing.assert_array_equal(d["web"], web)
+    # queued substep() calls: the same clock, bit for bit, and the same positions up to the
+    # order of the fp32 global atomics across workgroups
+    assert d["t_queued"] == d["t_direct"]
+    live = d["material"] >= 0
+    dq = np.abs(d["x_queued"][live] - d["x_direct"][live]).max()
+    print(f"[gpu-case] 137 queued substeps vs direct: max |dx| {dq:.2e}", flush=True)
+    assert dq <= 1e-5
     vort = render.compute_vorticity_grid(d["grid_v"], d["grid_m"], 1 / 128, 0.1)
     assert np.abs(vort).max() > 0.02
     assert np.abs(d["vort"] - vort).max() <= 1e-6
