@@ -605,32 +605,72 @@ JMD_HD void g2p_particle(PState<T>& p, const Consts<T>& k, const S* gp0, const S
   });
 }
 
+// The grid-side result of the G2P adjoint of one particle, in separable form: the adjoint
+// of node (b0 + i, b1 + j)'s velocity (after the grid op) is
+//   g0 = wy[j] P0[i] + ey[j] Q0[i],  g1 = wy[j] P1[i] + ey[j] Q1[i],
+// and through that node's grid op (NodeCoef) it is d momentum = (A0 g0, A1 g1) and
+// d mass = B0 g0 + B1 g1. These are linear in g with per-node coefficients, so a caller
+// that sums g over the particles of a node applies the coefficients once per node. Out of
+// range (inr false), every factor is 0.
+template <typename T>
+struct SepFactors {
+  int b0, b1;
+  bool inr;
+  T wy[3], ey[3], P0[3], Q0[3], P1[3], Q1[3];
+};
+
 // Adjoint of g2p_particle. `s` is the particle's state before the substep (its x sets the
 // weights; v, C pass through when out of range), `b_out` the adjoint of the state after.
-// Accumulates the x adjoint and the pass-through v, C adjoints into `b_in`. `coef(i, j)`
-// gives node (i, j)'s NodeCoef; the adjoint of each stencil node's velocity, taken through
-// that node's grid op, goes to `emit(i, j, momentum_x_adj, momentum_y_adj, mass_adj)`.
-template <typename T, typename Coef, typename Emit>
-JMD_HD void g2p_backward_c(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
-                           Coef&& coef, PState<T>& b_in, Emit&& emit) {
-  // forward to the clamp, to know which components were clamped
+// Accumulates the x adjoint and the pass-through v, C adjoints into `b_in` and returns the
+// node-velocity adjoints as SepFactors. `vel(i, j, a0, a1)` gives node (i, j)'s velocity
+// after the grid op. One pass over the stencil: the forward velocity (summed in
+// g2p_particle_v's order, so that the wall-clamp decisions match the forward) and, per
+// stencil column i, the moments over rows j of the node velocities a against wy, ey =
+// wy e1, dwy and dwy e1 (e1 = j - fx1), from which the x adjoint follows in closed form:
+//   d L/d fx0 = sum_i dwx_i (p0_i U0_i + q0 E0_i + p1_i U1_i + q1 E1_i)
+//               - apic wx_i (Cb00 U0_i + Cb10 U1_i)
+//   d L/d fx1 = sum_i wx_i (p0_i D0_i + q0 H0_i + p1_i D1_i + q1 H1_i)
+//               - apic wx_i (Cb01 U0_i + Cb11 U1_i)
+// with p0_i = vb0 + apic Cb00 e0_i, p1_i = vb1 + apic Cb10 e0_i, q0 = apic Cb01,
+// q1 = apic Cb11 (Cb the adjoint of C, vb of v after G2P, e0_i = i - fx0), U/E/D/H the
+// moments of a against wy/ey/dwy/dwy e1. This is the node-by-node sum of
+// w (vb . a + apic a^T Cb e) differentiated in fx, regrouped by stencil column.
+template <typename T, typename Vel>
+JMD_HD void g2p_backward_sep(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
+                             Vel&& vel, PState<T>& b_in, SepFactors<T>& sf) {
   T v0n = s.v0, v1n = s.v1;
-  int b0, b1;
   T fx0, fx1;
-  const bool inr = base_cell(s.x0, s.x1, k, b0, b1, fx0, fx1);
-  T wx[3], wy[3], dwx[3], dwy[3];
-  if (inr) {
+  sf.inr = base_cell(s.x0, s.x1, k, sf.b0, sf.b1, fx0, fx1);
+  T wx[3], dwx[3], dwy[3], dey[3];
+  T U0[3], U1[3], E0[3], E1[3], D0[3], D1[3], H0[3], H1[3];
+  if (sf.inr) {
     weights(fx0, wx, dwx);
-    weights(fx1, wy, dwy);
+    weights(fx1, sf.wy, dwy);
+    for (int jj = 0; jj < 3; ++jj) {
+      const T e1 = T(jj) - fx1;
+      sf.ey[jj] = sf.wy[jj] * e1;
+      dey[jj] = dwy[jj] * e1;
+    }
     v0n = T(0);
     v1n = T(0);
-    for (int ii = 0; ii < 3; ++ii)
+    for (int ii = 0; ii < 3; ++ii) {
+      U0[ii] = U1[ii] = E0[ii] = E1[ii] = D0[ii] = D1[ii] = H0[ii] = H1[ii] = T(0);
       for (int jj = 0; jj < 3; ++jj) {
-        const NodeCoef<T> c = coef(b0 + ii, b1 + jj);
-        T w = wx[ii] * wy[jj];
-        v0n += w * c.v0;
-        v1n += w * c.v1;
+        T a0, a1;
+        vel(sf.b0 + ii, sf.b1 + jj, a0, a1);
+        const T w = wx[ii] * sf.wy[jj];
+        v0n += w * a0;
+        v1n += w * a1;
+        U0[ii] += sf.wy[jj] * a0;
+        U1[ii] += sf.wy[jj] * a1;
+        E0[ii] += sf.ey[jj] * a0;
+        E1[ii] += sf.ey[jj] * a1;
+        D0[ii] += dwy[jj] * a0;
+        D1[ii] += dwy[jj] * a1;
+        H0[ii] += dey[jj] * a0;
+        H1[ii] += dey[jj] * a1;
       }
+    }
   }
   v1n = v1n - k.gravity_dt;
   T xu0 = s.x0 + k.dt * v0n, xu1 = s.x1 + k.dt * v1n;
@@ -640,48 +680,52 @@ JMD_HD void g2p_backward_c(const PState<T>& s, const PState<T>& b_out, const Con
   b_in.x0 += xb0;
   b_in.x1 += xb1;
   T vb0 = b_out.v0 + k.dt * xb0, vb1 = b_out.v1 + k.dt * xb1;  // adjoint of v after G2P
-  if (!inr) {
+  if (!sf.inr) {
     b_in.v0 += vb0;
     b_in.v1 += vb1;
     b_in.C = madd(b_in.C, b_out.C);
+    for (int q = 0; q < 3; ++q)
+      sf.wy[q] = sf.ey[q] = sf.P0[q] = sf.Q0[q] = sf.P1[q] = sf.Q1[q] = T(0);
     return;
   }
   const Mat<T>& Cb = b_out.C;
+  const T as = k.apic_scale;
+  const T q0 = as * Cb.b, q1 = as * Cb.d;
   T f0 = T(0), f1 = T(0);  // d L / d fx
-  for (int ii = 0; ii < 3; ++ii)
-    for (int jj = 0; jj < 3; ++jj) {
-      const int i = b0 + ii, j = b1 + jj;
-      const NodeCoef<T> c = coef(i, j);
-      T w = wx[ii] * wy[jj];
-      T e0 = T(ii) - fx0, e1 = T(jj) - fx1;
-      const T a0 = c.v0, a1 = c.v1;
-      // L_node = vb . g + apic g^T Cb e
-      T ce0 = Cb.a * e0 + Cb.b * e1, ce1 = Cb.c * e0 + Cb.d * e1;  // Cb e
-      const T g0 = w * vb0 + k.apic_scale * w * ce0, g1 = w * vb1 + k.apic_scale * w * ce1;
-      emit(i, j, c.A0 * g0, c.A1 * g1, c.B0 * g0 + c.B1 * g1);
-      T lw = vb0 * a0 + vb1 * a1 + k.apic_scale * (a0 * ce0 + a1 * ce1);
-      f0 += dwx[ii] * wy[jj] * lw;
-      f1 += wx[ii] * dwy[jj] * lw;
-      // d e / d fx = -I: -apic w (Cb^T g)
-      f0 -= k.apic_scale * w * (Cb.a * a0 + Cb.c * a1);
-      f1 -= k.apic_scale * w * (Cb.b * a0 + Cb.d * a1);
-    }
+  for (int ii = 0; ii < 3; ++ii) {
+    const T e0 = T(ii) - fx0;
+    const T p0 = vb0 + as * Cb.a * e0, p1 = vb1 + as * Cb.c * e0;
+    f0 += dwx[ii] * (p0 * U0[ii] + q0 * E0[ii] + p1 * U1[ii] + q1 * E1[ii]);
+    f1 += wx[ii] * (p0 * D0[ii] + q0 * H0[ii] + p1 * D1[ii] + q1 * H1[ii]);
+    f0 -= as * wx[ii] * (Cb.a * U0[ii] + Cb.c * U1[ii]);
+    f1 -= as * wx[ii] * (Cb.b * U0[ii] + Cb.d * U1[ii]);
+    sf.P0[ii] = wx[ii] * p0;
+    sf.Q0[ii] = wx[ii] * q0;
+    sf.P1[ii] = wx[ii] * p1;
+    sf.Q1[ii] = wx[ii] * q1;
+  }
   b_in.x0 += k.inv_dx * f0;
   b_in.x1 += k.inv_dx * f1;
 }
 
-// The same, with each node's NodeCoef computed from momentum gp0, gp1 and mass gm.
-template <typename T, typename S, typename Emit>
-JMD_HD void g2p_backward(const PState<T>& s, const PState<T>& b_out, const Consts<T>& k,
-                         const S* gp0, const S* gp1, const S* gm, PState<T>& b_in,
-                         Emit&& emit) {
-  g2p_backward_c(
-      s, b_out, k,
-      [&](int i, int j) {
-        const int node = j * k.nx + i;
-        return node_coef(T(gp0[node]), T(gp1[node]), T(gm[node]), i, j, k);
-      },
-      b_in, emit);
+// The node-velocity adjoints of SepFactors, one `emit(i, j, g0, g1)` per stencil node
+// (nothing when out of range).
+template <typename T, typename Emit>
+JMD_HD void emit_sep(const SepFactors<T>& sf, Emit&& emit) {
+  if (!sf.inr) return;
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = 0; jj < 3; ++jj)
+      emit(sf.b0 + ii, sf.b1 + jj, sf.wy[jj] * sf.P0[ii] + sf.ey[jj] * sf.Q0[ii],
+           sf.wy[jj] * sf.P1[ii] + sf.ey[jj] * sf.Q1[ii]);
+}
+
+// Node (i, j)'s momentum and mass adjoints from its velocity adjoint (g0, g1), through the
+// node's grid op.
+template <typename T>
+JMD_HD void through_gridop(const NodeCoef<T>& c, T g0, T g1, T& pb0, T& pb1, T& mb) {
+  pb0 = c.A0 * g0;
+  pb1 = c.A1 * g1;
+  mb = c.B0 * g0 + c.B1 * g1;
 }
 
 // Adjoint of p2g_particle (gather from the momentum/mass adjoints pb0, pb1, mb). `b_out`

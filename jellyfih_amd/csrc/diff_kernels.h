@@ -39,7 +39,7 @@ namespace jmd {
 
 constexpr int kDiffBlock = 256;   // threads per workgroup, one particle each
 constexpr int kTileNodes = 1024;  // forward LDS window/tile: 3 planes x 1024 nodes of fp64 (24 KB)
-constexpr int kWinB = 1024;       // adjoint LDS window: 6 staged + 3 accumulated per node
+constexpr int kWinB = 512;        // adjoint LDS window: 6 staged + 2 accumulated per node
 
 template <typename T>
 struct DiffInputs {
@@ -506,7 +506,7 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const A* aF_in, int
         const T* a_full_in, A* m_out, A* aF_out, T* a_full_out, A* zero, int64_t zero_n,
         int P, int G, A t, Consts<A> k, T* gamp, double* gparams) {
   static_assert(sizeof(A) >= sizeof(T), "arithmetic at least as wide as the buffers");
-  __shared__ double acc[3 * kWinB];  // G2P-adjoint scatter tile
+  __shared__ double acc[2 * kWinB];  // G2P-adjoint tile: node-velocity adjoints (x, y)
   __shared__ A cw[6 * kWinB];        // staged NodeCoef of the window
   zero_span(zero, zero_n);
   const int p = blockIdx.x * kDiffBlock + threadIdx.x;
@@ -572,9 +572,8 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const A* aF_in, int
     if (valid && mt < 0) mo = a;  // dead slots pass the adjoint through
     const T* g = grid_prev + (size_t)m * 3 * G;
     const int n = tl.tw * tl.th;
-    const GlobalEmit<A> direct{agrid_prev + (size_t)m * 3 * G, k.nx, G};
     if (tl.use) {
-      // the window's NodeCoef once per node, and the zeroed scatter tile, in LDS
+      // the window's NodeCoef once per node, and the zeroed velocity-adjoint tile, in LDS
       const T* gw = grid_prev + (size_t)tl.inst * 3 * G;
       for (int li = threadIdx.x; li < n; li += kDiffBlock) {
         const int jj = li / tl.tw;
@@ -590,45 +589,80 @@ k_d_bwd(DiffInputs<T> in, const T* st_cur, const T* st_prev, const A* aF_in, int
         cw[5 * n + li] = c.B1;
         acc[li] = 0.0;
         acc[n + li] = 0.0;
-        acc[2 * n + li] = 0.0;
       }
       __syncthreads();
     }
-    const LanePairs pr = lane_pairs(active && tl.use ? b1 * k.nx + b0 + 1 : -(1 + (int)threadIdx.x));
-    if (valid && mt >= 0)
-      g2p_backward_c(
-          sp, a, k,
-          [&](int i, int j) {
-            if (tl.use) {
+    SepFactors<A> sf;
+    sf.inr = false;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      sf.wy[q] = sf.ey[q] = sf.P0[q] = sf.Q0[q] = sf.P1[q] = sf.Q1[q] = A(0);
+    if (valid && mt >= 0) {
+      if (tl.use)
+        g2p_backward_sep(
+            sp, a, k,
+            [&](int i, int j, A& a0, A& a1) {
               const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-              return NodeCoef<A>{cw[li],         cw[n + li],     cw[2 * n + li],
-                                 cw[3 * n + li], cw[4 * n + li], cw[5 * n + li]};
-            }
-            const int node = j * k.nx + i;
-            return node_coef(A(g[node]), A(g[G + node]), A(g[2 * (size_t)G + node]), i, j, k);
-          },
-          mo,
-          [&](int i, int j, A c0, A c1, A c2) {
-            if (tl.use) {
-              const A n0 = next_lane(c0), n1 = next_lane(c1), n2 = next_lane(c2);
-              if (pr.same_next) {
-                c0 += n0;
-                c1 += n1;
-                c2 += n2;
-              }
-              if (pr.head) {
-                const int li = (j - tl.oy) * tl.tw + (i - tl.ox);
-                atomicAdd(&acc[li], double(c0));
-                atomicAdd(&acc[n + li], double(c1));
-                atomicAdd(&acc[2 * n + li], double(c2));
-              }
-            } else {
-              direct(i, j, c0, c1, c2);
-            }
-          });
+              a0 = cw[li];
+              a1 = cw[n + li];
+            },
+            mo, sf);
+      else
+        g2p_backward_sep(
+            sp, a, k,
+            [&](int i, int j, A& a0, A& a1) {
+              node_velocity(g, g + G, g + 2 * (size_t)G, i, j, k, a0, a1);
+            },
+            mo, sf);
+    }
     if (tl.use) {
+      // Every lane runs the same-cell pre-sums (lanes out of range carry zero factors and
+      // unique keys); the first lane of each group of 4 adds the two node-velocity
+      // adjoints of each stencil node into the tile. A non-finite adjoint can reach the
+      // other lanes of its DPP row through f = 0, all of them lanes of this instance,
+      // whose gradient it makes non-finite anyway (the tile holds one instance).
+      const int lane = threadIdx.x & 63;
+      const int key = sf.inr ? sf.b1 * k.nx + sf.b0 + 1 : -(1 + lane);
+      const jm::LaneGroups lg = jm::lane_groups(key, sf.inr);
+      const int q0 = (sf.b1 - tl.oy) * tl.tw + (sf.b0 - tl.ox);
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          A c0 = sf.wy[jj] * sf.P0[ii] + sf.ey[jj] * sf.Q0[ii];
+          A c1 = sf.wy[jj] * sf.P1[ii] + sf.ey[jj] * sf.Q1[ii];
+          jm::presum2(c0, c1, lg.f1, lg.f2);
+          if (lg.emit) {
+            const int li = q0 + jj * tl.tw + ii;
+            atomicAdd(&acc[li], double(c0));
+            atomicAdd(&acc[n + li], double(c1));
+          }
+        }
+      }
       __syncthreads();
-      tile_flush(acc, tl, agrid_prev + (size_t)tl.inst * 3 * G, k.nx, G);
+      // through each node's grid op, once per node, into the instance's adjoint grid
+      A* ga = agrid_prev + (size_t)tl.inst * 3 * G;
+      for (int li = threadIdx.x; li < n; li += kDiffBlock) {
+        const double g0 = acc[li], g1 = acc[n + li];
+        if (g0 == 0.0 && g1 == 0.0) continue;
+        const int jj = li / tl.tw;
+        const int node = (tl.oy + jj) * k.nx + tl.ox + (li - jj * tl.tw);
+        const A pb0 = A(double(cw[2 * n + li]) * g0), pb1 = A(double(cw[3 * n + li]) * g1);
+        const A mb = A(double(cw[4 * n + li]) * g0 + double(cw[5 * n + li]) * g1);
+        if (pb0 != A(0)) unsafeAtomicAdd(ga + node, pb0);
+        if (pb1 != A(0)) unsafeAtomicAdd(ga + G + node, pb1);
+        if (mb != A(0)) unsafeAtomicAdd(ga + 2 * (size_t)G + node, mb);
+      }
+    } else if (sf.inr) {
+      const GlobalEmit<A> direct{agrid_prev + (size_t)m * 3 * G, k.nx, G};
+      emit_sep(sf, [&](int i, int j, A g0, A g1) {
+        const int node = j * k.nx + i;
+        const NodeCoef<A> c = node_coef(A(g[node]), A(g[G + node]), A(g[2 * (size_t)G + node]),
+                                        i, j, k);
+        A pb0, pb1, mb;
+        through_gridop(c, g0, g1, pb0, pb1, mb);
+        direct(i, j, pb0, pb1, mb);
+      });
     }
     if (valid) {
       A* mo_p = m_out + 8 * (size_t)p;

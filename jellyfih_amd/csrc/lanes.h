@@ -68,6 +68,36 @@ __device__ __forceinline__ void presum3(float& c0, float& c1, float& c2, float f
       : "v"(f1), "v"(f2));
 }
 
+// presum3's two steps on two values. With two values the second step's DPP source was
+// written one instruction earlier, so an s_nop 0 completes its two wait states.
+__device__ __forceinline__ void presum2(float& c0, float& c1, float f1, float f2) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %2 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %1, %1, %2 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 0\n\t"
+      "v_fmac_f32_dpp %0, %0, %3 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %1, %1, %3 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(c0), "+v"(c1)
+      : "v"(f1), "v"(f2));
+}
+
+// The same steps in fp64: each lane's value moves as two 32-bit DPP moves (a lane past the
+// end of its DPP row reads 0), then one FMA.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_shl(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ void presum2(double& c0, double& c1, float f1, float f2) {
+  c0 += double(f1) * dpp_row_shl<0x101>(c0);
+  c1 += double(f1) * dpp_row_shl<0x101>(c1);
+  c0 += double(f2) * dpp_row_shl<0x102>(c0);
+  c1 += double(f2) * dpp_row_shl<0x102>(c1);
+}
+
 // LDS scatter sums are 64-bit fixed point with 50 fractional bits: integer LDS atomics
 // run at full rate on gfx950 where fp32 LDS atomics do not, and integer sums are exact and
 // independent of the order of the contributions. fixed_bits encodes |c| < 2 exactly (below);

@@ -152,7 +152,7 @@ int adj_backward(const OracleConsts* c, int M, int P, const int32_t* inst, const
   std::vector<T> times(nsteps + 1);
   times[0] = T(t0);
   for (int s = 0; s < nsteps; ++s) times[s + 1] = times[s] + k.dt;
-  std::vector<T> grid((size_t)M * 3 * G), adj((size_t)M * 3 * G);
+  std::vector<T> grid((size_t)M * 3 * G), adj((size_t)M * 3 * G), vadj((size_t)M * 2 * G);
   std::vector<jmd::PState<T>> bout(P), bin(P);
   for (int p = 0; p < P; ++p) bout[p] = load(grad_out + 12 * p);
   for (int s = nsteps - 1; s >= 0; --s) {
@@ -162,17 +162,40 @@ int adj_backward(const OracleConsts* c, int M, int P, const int32_t* inst, const
     std::vector<jmd::PState<T>> st(P), tmp(P);
     for (int p = 0; p < P; ++p) st[p] = tmp[p] = load(cur + 12 * p);
     p2g_all(k, M, P, inst, mat, fib.data(), amps.data(), par.data(), t, tmp, grid, G);
-    // G2P adjoint, through each node's grid op, into the momentum/mass adjoints
+    // G2P adjoint: node-velocity adjoints summed per node (two planes per instance), then
+    // through each node's grid op into the momentum/mass adjoints, as the GPU kernel does
+    std::fill(vadj.begin(), vadj.end(), 0.0);
     for (int p = 0; p < P; ++p) {
       bin[p] = jmd::PState<T>{0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
       if (mat[p] < 0) {
         bin[p] = bout[p];
         continue;
       }
-      T* g = grid.data() + (size_t)inst[p] * 3 * G;
-      T* a = adj.data() + (size_t)inst[p] * 3 * G;
-      jmd::g2p_backward(st[p], bout[p], k, g, g + G, g + 2 * G, bin[p], Planes{a, G, k.nx});
+      const T* g = grid.data() + (size_t)inst[p] * 3 * G;
+      T* va = vadj.data() + (size_t)inst[p] * 2 * G;
+      jmd::SepFactors<T> sf;
+      jmd::g2p_backward_sep(
+          st[p], bout[p], k,
+          [&](int i, int j, T& a0, T& a1) {
+            jmd::node_velocity(g, g + G, g + 2 * G, i, j, k, a0, a1);
+          },
+          bin[p], sf);
+      jmd::emit_sep(sf, [&](int i, int j, T g0, T g1) {
+        const size_t n = (size_t)j * k.nx + i;
+        va[n] += g0;
+        va[G + n] += g1;
+      });
     }
+    for (int m = 0; m < M; ++m)
+      for (int j = 0; j < k.ny; ++j)
+        for (int i = 0; i < k.nx; ++i) {
+          const size_t n = (size_t)j * k.nx + i;
+          const T* g = grid.data() + (size_t)m * 3 * G;
+          const T* va = vadj.data() + (size_t)m * 2 * G;
+          T* a = adj.data() + (size_t)m * 3 * G;
+          const jmd::NodeCoef<T> c = jmd::node_coef(g[n], g[G + n], g[2 * G + n], i, j, k);
+          jmd::through_gridop(c, va[n], va[G + n], a[n], a[G + n], a[2 * G + n]);
+        }
     // P2G adjoint (gather)
     for (int p = 0; p < P; ++p) {
       if (mat[p] < 0) continue;
