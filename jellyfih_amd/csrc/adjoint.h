@@ -748,30 +748,56 @@ JMD_HD void p2g_backward_particle(const PState<T>& s, int mat, T d0, T d1, T bet
   Mat<T> Fn, A;
   T mass;
   p2g_forward(mat, s.F, s.C, d0, d1, beta, mu_j, la_j, pdens, k, Fn, A, mass);
+  // The gather of the node adjoints (q0, q1, qm) = (pb0, pb1, mb) at node (b0 + i, b1 + j),
+  // with w = wx_i wy_j and dpos = dx (e0_i, e1_j), e0_i = i - fx0, e1_j = j - fx1:
+  //   vb = mass sum w q,  Ab = sum w q dpos^T,
+  //   d L/d fx0 = sum dwx_i wy_j lw - dx w (A^T q)_0,  d L/d fx1 = sum wx_i dwy_j lw - dx w (A^T q)_1,
+  //   lw = q0 (mass v0 + (A dpos)_0) + q1 (mass v1 + (A dpos)_1) + qm mass,
+  // regrouped by stencil column i: per column, the moments over rows j of q0, q1, qm against
+  // wy, ey = wy e1, dwy and dwy e1 (ten sums), folded into the results before the next column.
+  T ey[3], dey[3];
+  for (int jj = 0; jj < 3; ++jj) {
+    const T e1 = T(jj) - fx1;
+    ey[jj] = wy[jj] * e1;
+    dey[jj] = dwy[jj] * e1;
+  }
+  const T s0 = k.dx * A.b, s1 = k.dx * A.d;  // the e1 parts of (A dpos)
   Mat<T> Ab = {T(0), T(0), T(0), T(0)};
   T vb0 = T(0), vb1 = T(0), f0 = T(0), f1 = T(0);
-  for (int ii = 0; ii < 3; ++ii)
+  for (int ii = 0; ii < 3; ++ii) {
+    T U0 = T(0), U1 = T(0), Um = T(0), E0 = T(0), E1 = T(0);
+    T D0 = T(0), D1 = T(0), Dm = T(0), H0 = T(0), H1 = T(0);
     for (int jj = 0; jj < 3; ++jj) {
-      int node = (b1 + jj) * k.nx + (b0 + ii);
-      T w = wx[ii] * wy[jj];
-      T dp0 = (T(ii) - fx0) * k.dx, dp1 = (T(jj) - fx1) * k.dx;
-      T q0 = pb0[node], q1 = pb1[node], qm = mb[node];
-      vb0 += mass * w * q0;
-      vb1 += mass * w * q1;
-      Ab.a += w * q0 * dp0;
-      Ab.b += w * q0 * dp1;
-      Ab.c += w * q1 * dp0;
-      Ab.d += w * q1 * dp1;
-      T m0 = mass * s.v0 + (A.a * dp0 + A.b * dp1), m1 = mass * s.v1 + (A.c * dp0 + A.d * dp1);
-      T lw = q0 * m0 + q1 * m1 + qm * mass;
-      f0 += dwx[ii] * wy[jj] * lw;
-      f1 += wx[ii] * dwy[jj] * lw;
-      // d dpos / d fx = -dx I
-      f0 -= k.dx * w * (A.a * q0 + A.c * q1);
-      f1 -= k.dx * w * (A.b * q0 + A.d * q1);
+      const int node = (b1 + jj) * k.nx + (b0 + ii);
+      const T q0 = pb0[node], q1 = pb1[node], qm = mb[node];
+      U0 += wy[jj] * q0;
+      U1 += wy[jj] * q1;
+      Um += wy[jj] * qm;
+      E0 += ey[jj] * q0;
+      E1 += ey[jj] * q1;
+      D0 += dwy[jj] * q0;
+      D1 += dwy[jj] * q1;
+      Dm += dwy[jj] * qm;
+      H0 += dey[jj] * q0;
+      H1 += dey[jj] * q1;
     }
-  b_in.v0 += vb0;
-  b_in.v1 += vb1;
+    const T e0 = T(ii) - fx0, dp0 = e0 * k.dx;
+    const T r0 = mass * s.v0 + A.a * dp0, r1 = mass * s.v1 + A.c * dp0;  // (A dpos) at e1 = 0
+    vb0 += wx[ii] * U0;
+    vb1 += wx[ii] * U1;
+    Ab.a += wx[ii] * dp0 * U0;
+    Ab.b += wx[ii] * E0;
+    Ab.c += wx[ii] * dp0 * U1;
+    Ab.d += wx[ii] * E1;
+    f0 += dwx[ii] * (r0 * U0 + s0 * E0 + r1 * U1 + s1 * E1 + mass * Um);
+    f1 += wx[ii] * (r0 * D0 + s0 * H0 + r1 * D1 + s1 * H1 + mass * Dm);
+    f0 -= k.dx * wx[ii] * (A.a * U0 + A.c * U1);
+    f1 -= k.dx * wx[ii] * (A.b * U0 + A.d * U1);
+  }
+  Ab.b *= k.dx;
+  Ab.d *= k.dx;
+  b_in.v0 += mass * vb0;
+  b_in.v1 += mass * vb1;
   b_in.x0 += k.inv_dx * f0;
   b_in.x1 += k.inv_dx * f1;
   Mat<T> F0b = {T(0), T(0), T(0), T(0)}, Cb = {T(0), T(0), T(0), T(0)};
