@@ -547,8 +547,12 @@ __device__ __forceinline__ int packed_cell(float2 x, const PhysConsts& k, float&
   return in_range(b0, b1, k) ? (b0 | (b1 << 16)) : -1;
 }
 
+// Register budget: five waves per SIMD where the LDS window allows it (chunks of up to 1024
+// particles in workgroups of at least 256 threads); at 32 x 80K, 256 x 4 runs 39.5 us per
+// launch at five waves, 42.2 us at four and 53.2 us at six (which spills).
 template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_fused(FusedArgs a) {
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))) k_fused(FusedArgs a) {
   // Grid-window capacity (nodes): a chunk of P sorted particles (4 per cell in water)
   // covers about P/4 cells (P/4/S columns of an S-row strip, or P/512 + 2 rows of 128 in
   // row-major order), plus the 2-node stencil halo and drift since the last sort.
