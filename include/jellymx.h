@@ -246,6 +246,30 @@ int jm_polygon_self_intersects(const double* poly, int64_t n, int32_t* out);
 int jm_points_in_polygon_grid(const double* poly, int64_t n, const double* xs, int64_t nx,
                               const double* ys, int64_t ny, uint8_t* out);
 
+/* make_jelly.py:61-334 (generate_phenotype): the robot particles of one genome (genome[0..8]
+ * the morphology genes; n_genes >= 9), bit-identical to the reference's numpy evaluation:
+ * pos (n,2) double with the spawn offset applied (spawn: 2 doubles), mat (n,) 1 jelly,
+ * 2 payload, 3 muscle, fiber (n,2) float; *self_intersecting as make_jelly.py:139-141.
+ * axisym != 0 is make_jelly's JELLY_AXISYM=1 branch (the right half alone, payload over
+ * [0, PAYLOAD_WIDTH/2]). *n_out is set to the particle count; when it exceeds capacity
+ * nothing is written and JM_ERR_ARG is returned (call again with capacity >= *n_out). */
+int jm_generate_phenotype(const double* genome, int32_t n_genes, const double* spawn,
+                          int32_t grid_res, int32_t with_payload, int32_t axisym, double* pos,
+                          int32_t* mat, float* fiber, int64_t capacity, int64_t* n_out,
+                          int32_t* self_intersecting);
+
+/* make_jelly.py:336-402 (fill_tank): the fixed-size particle arrays of one instance, robot
+ * first, then the water lattice minus the exclusion zone (truncated to fit), then dead slots
+ * at (-1,-1) with material -1; fibres (0,1) outside muscle. pos (max_particles,2) float, mat
+ * (max_particles,) int32, fiber (max_particles,2) float. spawn NULL selects the reference's
+ * default ((0.5, 0.40), or (0, 0.40) when axisym != 0). info[5]: n_robot, n_water,
+ * muscle_count, n_jelly, self_intersecting. JM_ERR_ARG when the robot alone exceeds
+ * max_particles. */
+int jm_fill_tank(const double* genome, int32_t n_genes, int64_t max_particles, int32_t grid_res,
+                 const double* spawn, double water_margin, int32_t with_payload,
+                 double domain_height, int32_t axisym, float* pos, int32_t* mat, float* fiber,
+                 int64_t* info);
+
 /* ---- Differentiable path -----------------------------------------------------------
  * No counterpart in the reference (it has no autodiff). A batched substep and its adjoint
  * over caller-owned device buffers (e.g. torch tensors), P particles in any order, in fp32

@@ -1704,6 +1704,364 @@ int jm_points_in_polygon_grid(const double* poly, int64_t n, const double* xs, i
 }
 
 // -----------------------------------------------------------------------------------------
+// The phenotype generator and tank fill (make_jelly.py:61-402) in host code. Every value is
+// computed in double in the reference's operation order with FP contraction off, so that
+// results are bit-identical to the numpy evaluation: linspace as numpy forms it
+// (i * step + start, the last point set to stop), arange as start + i * ((start + step) -
+// start), numpy's scalar ** as C pow, np.gradient with unit spacing, row norms as
+// sqrt(x*x + y*y) and the norm of one 2-vector as numpy's dot product forms it,
+// sqrt(fma(y, y, x*x)); the nearest spine point of a muscle particle (the k-d tree query of
+// make_jelly.py:267-272) by exhaustive search of the 50 points, in squared distance summed
+// x then y, the first minimum kept.
+
+namespace {
+
+constexpr double kPayloadW = 0.08;   // make_jelly.py:8
+constexpr double kPayloadH = 0.05;   // make_jelly.py:9
+
+struct V2 {
+  double x, y;
+};
+
+std::vector<double> np_linspace(double a, double b, int64_t n) {
+#pragma clang fp contract(off)
+  std::vector<double> y((size_t)std::max<int64_t>(n, 0));
+  if (n <= 0) return y;
+  if (n == 1) {
+    y[0] = a;
+    return y;
+  }
+  const double div = (double)(n - 1), delta = b - a, step = delta / div;
+  for (int64_t i = 0; i < n; ++i)
+    y[(size_t)i] = step == 0.0 ? ((double)i / div) * delta + a : (double)i * step + a;
+  y[(size_t)n - 1] = b;
+  return y;
+}
+
+std::vector<double> np_arange(double a, double b, double step) {
+#pragma clang fp contract(off)
+  const double len = std::ceil((b - a) / step);
+  std::vector<double> y(len > 0 ? (size_t)len : 0);
+  if (y.empty()) return y;
+  y[0] = a;
+  const double delta = (a + step) - a;
+  for (size_t i = 1; i < y.size(); ++i) y[i] = a + (double)i * delta;
+  return y;
+}
+
+// make_jelly.py:61-63, term by term and summed left to right
+V2 bezier(V2 p0, V2 p1, V2 p2, V2 p3, double t) {
+#pragma clang fp contract(off)
+  const double u = 1.0 - t;
+  const double c0 = std::pow(u, 3.0), c1 = (3.0 * std::pow(u, 2.0)) * t;
+  const double c2 = (3.0 * u) * std::pow(t, 2.0), c3 = std::pow(t, 3.0);
+  return {((c0 * p0.x + c1 * p1.x) + c2 * p2.x) + c3 * p3.x,
+          ((c0 * p0.y + c1 * p1.y) + c2 * p2.y) + c3 * p3.y};
+}
+
+void pip_grid(const std::vector<V2>& poly, const std::vector<double>& xs,
+              const std::vector<double>& ys, std::vector<uint8_t>& out) {
+  out.resize(xs.size() * ys.size());
+  jm_points_in_polygon_grid(&poly[0].x, (int64_t)poly.size(), xs.data(), (int64_t)xs.size(),
+                            ys.data(), (int64_t)ys.size(), out.data());
+}
+
+std::vector<V2> closed(const std::vector<V2>& a, const std::vector<V2>& b_reversed) {
+  std::vector<V2> p(a);
+  p.insert(p.end(), b_reversed.rbegin(), b_reversed.rend());
+  return p;
+}
+
+struct Phenotype {
+  std::vector<V2> pos;
+  std::vector<int32_t> mat;
+  std::vector<float> fib;  // 2 per particle
+  bool self_int = false;
+};
+
+// make_jelly.py:74-334
+Phenotype generate_phenotype(const double* g, V2 offset, int grid_res, bool with_payload,
+                             bool axisym) {
+#pragma clang fp contract(off)
+  Phenotype ph;
+  const V2 start = {kPayloadW / 2.0, 0.0};
+  const V2 cp1 = {start.x + std::fabs(g[0]), start.y + g[1]};
+  const V2 cp2 = {start.x + std::fabs(g[2]), start.y + g[3]};
+  const V2 endp = {start.x + std::fabs(g[4]), start.y + g[5]};
+  const double t_base = std::fabs(g[6]), t_mid = std::fabs(g[7]), t_tip = std::fabs(g[8]);
+
+  const int S = 50;
+  const std::vector<double> ts = np_linspace(0.0, 1.0, S);
+  std::vector<V2> spine(S), tan(S), stan(S), nrm(S), outer(S), inner(S), medge(S);
+  for (int i = 0; i < S; ++i) spine[i] = bezier(start, cp1, cp2, endp, ts[i]);
+  for (int i = 0; i < S; ++i) {
+    if (i == 0)
+      tan[i] = {(spine[1].x - spine[0].x) / 1.0, (spine[1].y - spine[0].y) / 1.0};
+    else if (i == S - 1)
+      tan[i] = {(spine[i].x - spine[i - 1].x) / 1.0, (spine[i].y - spine[i - 1].y) / 1.0};
+    else
+      tan[i] = {(spine[i + 1].x - spine[i - 1].x) / 2.0, (spine[i + 1].y - spine[i - 1].y) / 2.0};
+  }
+  std::vector<double> half(S);
+  for (int i = 0; i < S; ++i) {
+    double tl = std::sqrt(tan[i].x * tan[i].x + tan[i].y * tan[i].y);
+    if (tl < 1e-10) tl = 1.0;
+    stan[i] = {tan[i].x / tl, tan[i].y / tl};
+    const V2 n = {-tan[i].y, tan[i].x};
+    double nl = std::sqrt(n.x * n.x + n.y * n.y);
+    if (nl == 0.0) nl = 1.0;
+    nrm[i] = {n.x / nl, n.y / nl};
+    // np.interp(ts, [0, 0.5, 1], [t_base, t_mid, t_tip])
+    const double x = ts[i];
+    double tp;
+    if (x == 1.0) {
+      tp = t_tip;
+    } else {
+      const int j = x < 0.5 ? 0 : 1;
+      const double xj = j == 0 ? 0.0 : 0.5, f0 = j == 0 ? t_base : t_mid;
+      const double f1 = j == 0 ? t_mid : t_tip;
+      tp = x == xj ? f0 : ((f1 - f0) / ((j == 0 ? 0.5 : 1.0) - xj)) * (x - xj) + f0;
+    }
+    half[i] = tp / 2.0;
+  }
+  const double spacing = 1.0 / (grid_res * 2.0), min_base_muscle = spacing * 1.2;
+  const double muscle_ratio = 0.2;
+  for (int i = 0; i < S; ++i) {
+    outer[i] = {spine[i].x + nrm[i].x * half[i], spine[i].y + nrm[i].y * half[i]};
+    inner[i] = {spine[i].x - nrm[i].x * half[i], spine[i].y - nrm[i].y * half[i]};
+    const double mt = std::min(min_base_muscle + half[i] * muscle_ratio, half[i] * 0.9);
+    const double d = half[i] - mt;
+    medge[i] = {spine[i].x - nrm[i].x * d, spine[i].y - nrm[i].y * d};
+  }
+  const std::vector<V2> body = closed(outer, inner), muscle = closed(medge, inner);
+  int32_t si0 = 0, si1 = 0;
+  jm_polygon_self_intersects(&body[0].x, (int64_t)body.size(), &si0);
+  if (!si0) jm_polygon_self_intersects(&muscle[0].x, (int64_t)muscle.size(), &si1);
+  ph.self_int = si0 || si1;
+
+  const int raster = grid_res * 2;
+  double bx0 = body[0].x, bx1 = body[0].x, by0 = body[0].y, by1 = body[0].y;
+  for (const V2& p : body) {
+    bx0 = std::min(bx0, p.x);
+    bx1 = std::max(bx1, p.x);
+    by0 = std::min(by0, p.y);
+    by1 = std::max(by1, p.y);
+  }
+  const double lo_x = std::min(bx0, -kPayloadW), hi_x = std::max(bx1, kPayloadW);
+  const double lo_y = std::min(by0, -0.05), hi_y = std::max(by1, kPayloadH);
+  const double pad = 0.05;
+  const std::vector<double> xs =
+      np_linspace(lo_x - pad, hi_x + pad, (int64_t)((hi_x - lo_x + 2 * pad) * raster));
+  const std::vector<double> ys =
+      np_linspace(lo_y - pad, hi_y + pad, (int64_t)((hi_y - lo_y + 2 * pad) * raster));
+  const size_t nx = xs.size(), nc = xs.size() * ys.size();
+  std::vector<uint8_t> in_bell, in_muscle, in_collar(nc, 0), in_collar_mus(nc, 0);
+  pip_grid(body, xs, ys, in_bell);
+  pip_grid(muscle, xs, ys, in_muscle);
+  auto payload_box = [&](size_t c) {
+    return xs[c % nx] <= kPayloadW / 2.0 && ys[c / nx] <= kPayloadH;
+  };
+
+  // collar around the payload socket (make_jelly.py:164-223)
+  const double collar_top = kPayloadH * 0.50, collar_thick = t_base * 0.35;
+  const std::vector<double> cts = np_linspace(0.0, 1.0, 20);
+  const V2 q0 = {kPayloadW / 2.0, collar_top}, q3 = inner[0];
+  const V2 q1 = {kPayloadW / 2.0, collar_top - collar_top * 0.4};
+  const V2 q2 = {q3.x + 0.0, q3.y + std::min(collar_top * 0.3, std::fabs(q3.y - q0.y) * 0.4)};
+  std::vector<V2> collar_in(20);
+  for (int i = 0; i < 20; ++i) collar_in[i] = bezier(q0, q1, q2, q3, cts[i]);
+  const V2 outer0 = outer[0];
+  double bridge_hw;
+  if (outer0.x > kPayloadW / 2.0) {
+    const V2 p0 = {kPayloadW / 2.0 + collar_thick, collar_top}, p3 = outer0;
+    const V2 p1 = {p0.x, p0.y - collar_top * 0.5};
+    const V2 bt = {outer[2].x - outer[0].x, outer[2].y - outer[0].y};
+    const double bt_len = std::sqrt(std::fma(bt.y, bt.y, bt.x * bt.x));
+    const double reach = collar_top * 0.6;
+    const V2 p2 = bt_len > 1e-10 ? V2{p3.x - (bt.x / bt_len) * reach, p3.y - (bt.y / bt_len) * reach}
+                                 : V2{p3.x + 0.0, p3.y + 0.01};
+    std::vector<V2> collar_out(20), collar_mus(20);
+    for (int i = 0; i < 20; ++i) {
+      collar_out[i] = bezier(p0, p1, p2, p3, cts[i]);
+      collar_mus[i] = {collar_in[i].x + muscle_ratio * (collar_out[i].x - collar_in[i].x),
+                       collar_in[i].y + muscle_ratio * (collar_out[i].y - collar_in[i].y)};
+    }
+    pip_grid(closed(collar_out, collar_in), xs, ys, in_collar);
+    pip_grid(closed(collar_mus, collar_in), xs, ys, in_collar_mus);
+    for (size_t c = 0; c < nc; ++c)
+      if (payload_box(c)) in_collar[c] = in_collar_mus[c] = 0;
+    bridge_hw = std::max(outer0.x, kPayloadW / 2.0 + collar_thick);
+  } else {
+    bridge_hw = kPayloadW / 2.0 + 0.015;
+  }
+  // transverse bridge (make_jelly.py:225-238)
+  if (!with_payload) bridge_hw = std::max(bridge_hw, kPayloadW / 2.0 + 0.02);
+  const double bridge_top = with_payload ? 0.0 : kPayloadH;
+
+  // materials, later masks win (make_jelly.py:240-248); the right side (x >= 0)
+  std::vector<V2> rpos;
+  std::vector<int32_t> rmat;
+  for (size_t c = 0; c < nc; ++c) {
+    const double x = xs[c % nx], y = ys[c / nx];
+    int m = 0;
+    if (x >= -bridge_hw && x <= bridge_hw && y >= -0.03 && y <= bridge_top) m = 1;
+    if (in_collar[c]) m = 1;
+    if (in_collar_mus[c]) m = 3;
+    if (in_bell[c]) m = 1;
+    if (in_bell[c] && in_muscle[c]) m = 3;
+    if (x >= 0 && m > 0) {
+      rpos.push_back({x, y});
+      rmat.push_back(m);
+    }
+  }
+  const size_t nr = rpos.size();
+  std::vector<float> rfib(2 * nr);
+  for (size_t q = 0; q < nr; ++q) {
+    rfib[2 * q] = 0.0f;
+    rfib[2 * q + 1] = 1.0f;
+    if (rmat[q] != 3) continue;
+    int best = 0;
+    double bd = 0.0;
+    for (int i = 0; i < S; ++i) {
+      const double dx = rpos[q].x - spine[i].x, dy = rpos[q].y - spine[i].y;
+      const double d2 = dx * dx + dy * dy;
+      if (i == 0 || d2 < bd) {
+        bd = d2;
+        best = i;
+      }
+    }
+    rfib[2 * q] = (float)stan[best].x;
+    rfib[2 * q + 1] = (float)stan[best].y;
+  }
+
+  // right half mirrored (Cartesian), or the right half alone (axisymmetric,
+  // make_jelly.py:274-296); then the payload block and the spawn offset (make_jelly.py:298-325)
+  auto add = [&](V2 p, int32_t m, float f0, float f1) {
+    ph.pos.push_back({p.x + offset.x, p.y + offset.y});
+    ph.mat.push_back(m);
+    ph.fib.push_back(f0);
+    ph.fib.push_back(f1);
+  };
+  for (size_t q = 0; q < nr; ++q) add(rpos[q], rmat[q], rfib[2 * q], rfib[2 * q + 1]);
+  if (!axisym)
+    for (size_t q = 0; q < nr; ++q)
+      add({-rpos[q].x, rpos[q].y}, rmat[q], -rfib[2 * q], rfib[2 * q + 1]);
+  if (with_payload) {
+    const std::vector<double> px =
+        axisym ? np_linspace(0.0, kPayloadW / 2, (int64_t)(kPayloadW / 2 * raster))
+               : np_linspace(-kPayloadW / 2, kPayloadW / 2, (int64_t)(kPayloadW * raster));
+    const std::vector<double> py = np_linspace(0.0, kPayloadH, (int64_t)(kPayloadH * raster));
+    for (double y : py)
+      for (double x : px) add({x, y}, 2, 0.0f, 1.0f);
+  }
+  return ph;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jm_generate_phenotype(const double* genome, int32_t n_genes, const double* spawn,
+                          int32_t grid_res, int32_t with_payload, int32_t axisym, double* pos,
+                          int32_t* mat, float* fiber, int64_t capacity, int64_t* n_out,
+                          int32_t* self_intersecting) {
+  if (!genome || n_genes < 9 || !spawn || grid_res <= 0 || grid_res > 4096 || !n_out ||
+      !self_intersecting || capacity < 0 || (capacity > 0 && (!pos || !mat || !fiber)))
+    return fail(JM_ERR_ARG, "jm_generate_phenotype: bad arguments");
+  for (int i = 0; i < 9; ++i)
+    if (!std::isfinite(genome[i])) return fail(JM_ERR_ARG, "non-finite genome");
+  if (!std::isfinite(spawn[0]) || !std::isfinite(spawn[1]))
+    return fail(JM_ERR_ARG, "non-finite spawn offset");
+  const Phenotype ph =
+      generate_phenotype(genome, {spawn[0], spawn[1]}, grid_res, with_payload != 0, axisym != 0);
+  const int64_t n = (int64_t)ph.pos.size();
+  *n_out = n;
+  *self_intersecting = ph.self_int ? 1 : 0;
+  if (n > capacity) return fail(JM_ERR_ARG, "jm_generate_phenotype: capacity below *n_out");
+  for (int64_t i = 0; i < n; ++i) {
+    pos[2 * i] = ph.pos[(size_t)i].x;
+    pos[2 * i + 1] = ph.pos[(size_t)i].y;
+    mat[i] = ph.mat[(size_t)i];
+    fiber[2 * i] = ph.fib[2 * (size_t)i];
+    fiber[2 * i + 1] = ph.fib[2 * (size_t)i + 1];
+  }
+  return JM_OK;
+}
+
+int jm_fill_tank(const double* genome, int32_t n_genes, int64_t max_particles, int32_t grid_res,
+                 const double* spawn, double water_margin, int32_t with_payload,
+                 double domain_height, int32_t axisym, float* pos, int32_t* mat, float* fiber,
+                 int64_t* info) {
+#pragma clang fp contract(off)
+  if (!genome || n_genes < 9 || max_particles < 0 || grid_res <= 0 || grid_res > 4096 ||
+      !info || (max_particles > 0 && (!pos || !mat || !fiber)) ||
+      !(domain_height > 0.0 && domain_height <= 64.0) || !std::isfinite(water_margin))
+    return fail(JM_ERR_ARG, "jm_fill_tank: bad arguments");
+  for (int i = 0; i < 9; ++i)
+    if (!std::isfinite(genome[i])) return fail(JM_ERR_ARG, "non-finite genome");
+  // make_jelly.py:341-343: the default spawn is the axis in axisymmetric mode
+  const V2 off = spawn ? V2{spawn[0], spawn[1]} : (axisym ? V2{0.0, 0.40} : V2{0.5, 0.40});
+  if (!std::isfinite(off.x) || !std::isfinite(off.y))
+    return fail(JM_ERR_ARG, "non-finite spawn offset");
+  const Phenotype ph = generate_phenotype(genome, off, grid_res, with_payload != 0, axisym != 0);
+  const int64_t n_robot = (int64_t)ph.pos.size();
+  if (n_robot > max_particles)
+    return fail(JM_ERR_ARG, "jm_fill_tank: the robot alone exceeds max_particles");
+  // water lattice, 4 per cell with a 3-spacing margin, minus the exclusion zone around the
+  // robot (make_jelly.py:349-366)
+  const double spacing = 1.0 / (grid_res * 2.0), margin = spacing * 3;
+  const std::vector<double> wx = np_arange(margin, 1.0 - margin, spacing);
+  const std::vector<double> wy = np_arange(margin, domain_height - margin, spacing);
+  std::vector<double> cand(2 * wx.size() * wy.size());
+  size_t c = 0;
+  for (double y : wy)
+    for (double x : wx) {
+      cand[c++] = x;
+      cand[c++] = y;
+    }
+  const int64_t n_cand = (int64_t)(cand.size() / 2);
+  std::vector<uint8_t> keep((size_t)n_cand, 1);
+  if (n_robot > 0)
+    JM_TRY(jm_water_mask(&ph.pos[0].x, n_robot, cand.data(), n_cand, water_margin, keep.data()));
+  int64_t n_water = 0;
+  for (uint8_t k : keep) n_water += k;
+  if (n_robot + n_water > max_particles) n_water = max_particles - n_robot;  // truncate water
+  for (int64_t i = 0; i < max_particles; ++i) {
+    pos[2 * i] = -1.0f;
+    pos[2 * i + 1] = -1.0f;
+    mat[i] = -1;
+    fiber[2 * i] = 0.0f;
+    fiber[2 * i + 1] = 1.0f;
+  }
+  int64_t muscle = 0, jelly = 0;
+  for (int64_t i = 0; i < n_robot; ++i) {
+    pos[2 * i] = (float)ph.pos[(size_t)i].x;
+    pos[2 * i + 1] = (float)ph.pos[(size_t)i].y;
+    mat[i] = ph.mat[(size_t)i];
+    fiber[2 * i] = ph.fib[2 * (size_t)i];
+    fiber[2 * i + 1] = ph.fib[2 * (size_t)i + 1];
+    muscle += ph.mat[(size_t)i] == 3;
+    jelly += ph.mat[(size_t)i] == 1;
+  }
+  int64_t w = n_robot;
+  for (int64_t q = 0; q < n_cand && w < n_robot + n_water; ++q) {
+    if (!keep[(size_t)q]) continue;
+    pos[2 * w] = (float)cand[2 * q];
+    pos[2 * w + 1] = (float)cand[2 * q + 1];
+    mat[w] = 0;
+    ++w;
+  }
+  info[0] = n_robot;
+  info[1] = n_water;
+  info[2] = muscle;
+  info[3] = jelly;
+  info[4] = ph.self_int ? 1 : 0;
+  return JM_OK;
+}
+
+}  // extern "C"
+
+// -----------------------------------------------------------------------------------------
 // Differentiable path (no reference counterpart: the reference has no autodiff).
 
 struct jm_diff {

@@ -3,17 +3,15 @@
 The engine consumes fixed-size particle arrays in the format of the reference's
 ``make_jelly.fill_tank`` (c-m-x-r/jellyfih ``make_jelly.py:336-402``): ``pos (N,2) f32``,
 ``mat (N,) i32`` (0 water, 1 jelly, 2 payload, 3 muscle, -1 dead slot), ``fiber (N,2) f32``,
-robot particles first, then water, then dead slots at ``(-1,-1)``. The reference's own
-generator stays on the host of whoever drives the engine; this module restates it so
-that benchmarks and tests can build identical inputs where the reference is absent.
-It is checked bit-for-bit against fixtures written from the reference
-(``tests/golden``).
-
-Steps follow ``make_jelly.py:74-334`` (``generate_phenotype``):
-Bezier spine -> normals -> thickness envelope -> muscle band -> rasterised bell, collar,
-bridge masks -> right half mirrored -> payload block -> spawn offset; and
-``make_jelly.py:336-402`` (``fill_tank``): 4-per-cell water lattice minus a KD-tree
-exclusion zone around the robot, padded with dead slots.
+robot particles first, then water, then dead slots at ``(-1,-1)``. ``generate_phenotype``
+and ``fill_tank`` keep the reference's names and arguments and run natively
+(``jm_generate_phenotype`` / ``jm_fill_tank`` in libjellymx.so, host code): Bezier spine ->
+normals -> thickness envelope -> muscle band -> rasterised bell, collar and bridge masks ->
+right half mirrored -> payload block -> spawn offset (``make_jelly.py:61-334``), then the
+4-per-cell water lattice minus the exclusion zone around the robot, padded with dead slots
+(``make_jelly.py:336-402``), bit-identical to the reference's numpy evaluation (checked
+against fixtures written from the reference, ``tests/golden``, and against a numpy
+restatement over many genomes). ``population`` builds a batch of tanks on host threads.
 
 Axisymmetric mode (``make_jelly.py:11-14``, ``JELLY_AXISYM=1`` at import, or ``axisym=True``
 per call): the right half of the profile is the whole cross-section (no mirror), the
@@ -25,7 +23,6 @@ from __future__ import annotations
 import os
 
 import numpy as np
-from scipy.spatial import cKDTree
 
 PAYLOAD_WIDTH = 0.08                    # make_jelly.py:8
 PAYLOAD_HEIGHT = 0.05                   # make_jelly.py:9
@@ -45,11 +42,6 @@ AURELIA_GENOME = np.array([0.05, 0.04, 0.18, -0.03, 0.22, -0.12, 0.04, 0.05, 0.0
 def seeded_genome(seed: int) -> np.ndarray:
     """11-gene genome drawn uniformly within the evolve.py bounds from ``default_rng(seed)``."""
     return np.random.default_rng(seed).uniform(GENOME_LOWER, GENOME_UPPER)
-
-
-def _bezier(p0, p1, p2, p3, t):
-    # make_jelly.py:61-63 (same operation order, so results are bit-identical)
-    return (1 - t) ** 3 * p0 + 3 * (1 - t) ** 2 * t * p1 + 3 * (1 - t) * t ** 2 * p2 + t ** 3 * p3
 
 
 def polygon_self_intersects(poly: np.ndarray) -> bool:
@@ -78,161 +70,42 @@ def points_in_polygon_grid(poly, xs, ys) -> np.ndarray:
     return out.astype(bool)
 
 
-def _unit_normals(points: np.ndarray) -> np.ndarray:
-    # make_jelly.py:65-73
-    tan = np.gradient(points, axis=0)
-    nrm = np.zeros_like(tan)
-    nrm[:, 0] = -tan[:, 1]
-    nrm[:, 1] = tan[:, 0]
-    length = np.linalg.norm(nrm, axis=1, keepdims=True)
-    length[length == 0] = 1
-    return nrm / length
-
-
 def generate_phenotype(genome, spawn_offset=None, grid_res: int = 128, with_payload: bool = True,
                        axisym: bool | None = None):
     """Robot particles of one genome: ``(pos f64 (n,2), mat int (n,), fiber f32 (n,2),
-    self_intersecting)`` (make_jelly.py:74-334). ``axisym`` None follows ``AXISYM``."""
+    self_intersecting)`` (make_jelly.py:74-334), by jm_generate_phenotype. ``axisym`` None
+    follows ``AXISYM``."""
+    from . import _lib
+    _lib.load()
     if axisym is None:
         axisym = AXISYM
     if spawn_offset is None:
         spawn_offset = DEFAULT_SPAWN
-    g = np.asarray(genome, dtype=np.float64)
-    start = np.array([PAYLOAD_WIDTH / 2.0, 0.0])
-    cp1 = start + np.array([abs(g[0]), g[1]])
-    cp2 = start + np.array([abs(g[2]), g[3]])
-    end = start + np.array([abs(g[4]), g[5]])
-    t_base, t_mid, t_tip = abs(g[6]), abs(g[7]), abs(g[8])
+    g = np.ascontiguousarray(genome, dtype=np.float64).ravel()
+    sp = np.ascontiguousarray(spawn_offset, dtype=np.float64).ravel()
+    n = np.zeros(1, np.int64)
+    si = np.zeros(1, np.int32)
 
-    ts = np.linspace(0, 1, 50)
-    spine = np.array([_bezier(start, cp1, cp2, end, t) for t in ts])
-    tan = np.gradient(spine, axis=0)
-    tan_len = np.linalg.norm(tan, axis=1, keepdims=True)
-    tan_len[tan_len < 1e-10] = 1.0
-    spine_tan = tan / tan_len
+    def run(cap, checked):
+        pos = np.empty((cap, 2), np.float64)
+        mat = np.empty(cap, np.int32)
+        fib = np.empty((cap, 2), np.float32)
+        args = (g.ctypes.data, len(g), sp.ctypes.data, int(grid_res), int(bool(with_payload)),
+                int(bool(axisym)), pos.ctypes.data, mat.ctypes.data, fib.ctypes.data, cap,
+                n.ctypes.data, si.ctypes.data)
+        if checked:
+            _lib.call("jm_generate_phenotype", *args)
+            return pos, mat, fib
+        return (pos, mat, fib) if _lib.load().jm_generate_phenotype(*args) == 0 else None
 
-    nrm = _unit_normals(spine)
-    half = np.interp(ts, [0, 0.5, 1], [t_base, t_mid, t_tip])[:, None] / 2.0
-    outer = spine + nrm * half
-    inner = spine - nrm * half
-
-    spacing = 1.0 / (grid_res * 2.0)
-    muscle_ratio = 0.2
-    muscle_thick = np.minimum(spacing * 1.2 + half * muscle_ratio, half * 0.9)
-    muscle_edge = spine - nrm * (half - muscle_thick)
-
-    body_poly = np.vstack([outer, inner[::-1]])
-    muscle_poly = np.vstack([muscle_edge, inner[::-1]])
-    self_int = polygon_self_intersects(body_poly) or polygon_self_intersects(muscle_poly)
-
-    raster = grid_res * 2
-    lo_x = min(np.min(body_poly[:, 0]), -PAYLOAD_WIDTH)
-    hi_x = max(np.max(body_poly[:, 0]), PAYLOAD_WIDTH)
-    lo_y = min(np.min(body_poly[:, 1]), -0.05)
-    hi_y = max(np.max(body_poly[:, 1]), PAYLOAD_HEIGHT)
-    pad = 0.05
-    xs = np.linspace(lo_x - pad, hi_x + pad, int((hi_x - lo_x + 2 * pad) * raster))
-    ys = np.linspace(lo_y - pad, hi_y + pad, int((hi_y - lo_y + 2 * pad) * raster))
-    gx, gy = np.meshgrid(xs, ys)
-    cand = np.vstack([gx.ravel(), gy.ravel()]).T
-
-    in_bell = points_in_polygon_grid(body_poly, xs, ys)
-    in_muscle = points_in_polygon_grid(muscle_poly, xs, ys)
-    payload_box = (cand[:, 0] <= PAYLOAD_WIDTH / 2.0) & (cand[:, 1] <= PAYLOAD_HEIGHT)
-
-    # Collar around the payload socket (make_jelly.py:164-223)
-    collar_top = PAYLOAD_HEIGHT * 0.50
-    collar_thick = t_base * 0.35
-    cts = np.linspace(0, 1, 20)
-    q0 = np.array([PAYLOAD_WIDTH / 2.0, collar_top])
-    q3 = inner[0]
-    q1 = np.array([PAYLOAD_WIDTH / 2.0, collar_top - collar_top * 0.4])
-    q2 = q3 + np.array([0.0, min(collar_top * 0.3, abs(q3[1] - q0[1]) * 0.4)])
-    collar_in = np.array([_bezier(q0, q1, q2, q3, t) for t in cts])
-    outer0 = outer[0]
-    if outer0[0] > PAYLOAD_WIDTH / 2.0:
-        p0 = np.array([PAYLOAD_WIDTH / 2.0 + collar_thick, collar_top])
-        p3 = outer0
-        p1 = np.array([p0[0], p0[1] - collar_top * 0.5])
-        bt = outer[min(2, len(outer) - 1)] - outer[0]
-        bt_len = np.linalg.norm(bt)
-        p2 = p3 - (bt / bt_len) * (collar_top * 0.6) if bt_len > 1e-10 else p3 + np.array([0, 0.01])
-        collar_out = np.array([_bezier(p0, p1, p2, p3, t) for t in cts])
-        collar_poly = np.vstack([collar_out, collar_in[::-1]])
-        collar_mus = collar_in + muscle_ratio * (collar_out - collar_in)
-        collar_mus_poly = np.vstack([collar_mus, collar_in[::-1]])
-        in_collar = points_in_polygon_grid(collar_poly, xs, ys) & ~payload_box
-        in_collar_mus = points_in_polygon_grid(collar_mus_poly, xs, ys) & ~payload_box
-        bridge_hw = max(outer0[0], PAYLOAD_WIDTH / 2.0 + collar_thick)
-    else:
-        in_collar = np.zeros(len(cand), dtype=bool)
-        in_collar_mus = np.zeros(len(cand), dtype=bool)
-        bridge_hw = PAYLOAD_WIDTH / 2.0 + 0.015
-
-    # Transverse bridge (make_jelly.py:225-238)
-    if not with_payload:
-        bridge_hw = max(bridge_hw, PAYLOAD_WIDTH / 2.0 + 0.02)
-    bridge_top = PAYLOAD_HEIGHT if not with_payload else 0.0
-    in_bridge = ((cand[:, 0] >= -bridge_hw) & (cand[:, 0] <= bridge_hw)
-                 & (cand[:, 1] >= -0.03) & (cand[:, 1] <= bridge_top))
-
-    # Material assignment, later masks win (make_jelly.py:240-248)
-    mats = np.zeros(len(cand), dtype=int)
-    mats[in_bridge] = 1
-    mats[in_collar] = 1
-    mats[in_collar_mus] = 3
-    mats[in_bell] = 1
-    mats[in_bell & in_muscle] = 3
-
-    right = (cand[:, 0] >= 0) & (mats > 0)
-    rpos = cand[right]
-    rmat = mats[right]
-    rfib = np.zeros((len(rpos), 2), dtype=np.float32)
-    rfib[:, 1] = 1.0
-    if len(rpos) > 0 and np.any(rmat == 3):
-        is_mus = rmat == 3
-        _, idx = cKDTree(spine).query(rpos[is_mus])
-        rfib[is_mus] = spine_tan[idx]
-
-    if axisym:                      # the right half is the cross-section (make_jelly.py:276-280)
-        soft_pos, soft_mat, soft_fib = rpos, rmat, rfib
-    elif len(rpos) > 0:
-        lpos = rpos.copy()
-        lpos[:, 0] *= -1
-        lfib = rfib.copy()
-        lfib[:, 0] *= -1
-        soft_pos = np.vstack([rpos, lpos])
-        soft_mat = np.concatenate([rmat, rmat.copy()])
-        soft_fib = np.vstack([rfib, lfib])
-    else:
-        soft_pos = np.zeros((0, 2))
-        soft_mat = np.zeros(0, dtype=int)
-        soft_fib = np.zeros((0, 2), dtype=np.float32)
-
-    parts_pos, parts_mat, parts_fib = [], [], []
-    offset = np.array(spawn_offset)
-    if len(soft_pos) > 0:
-        parts_pos.append(soft_pos + offset)
-        parts_mat.append(soft_mat)
-        parts_fib.append(soft_fib)
-    if with_payload:
-        if axisym:                  # make_jelly.py:301-302
-            px = np.linspace(0, PAYLOAD_WIDTH / 2, int(PAYLOAD_WIDTH / 2 * raster))
-        else:
-            px = np.linspace(-PAYLOAD_WIDTH / 2, PAYLOAD_WIDTH / 2, int(PAYLOAD_WIDTH * raster))
-        py = np.linspace(0, PAYLOAD_HEIGHT, int(PAYLOAD_HEIGHT * raster))
-        pgx, pgy = np.meshgrid(px, py)
-        pay = np.vstack([pgx.ravel(), pgy.ravel()]).T
-        if len(pay) > 0:
-            parts_pos.append(pay + offset)
-            parts_mat.append(np.full(len(pay), 2, dtype=int))
-            pf = np.zeros((len(pay), 2), dtype=np.float32)
-            pf[:, 1] = 1.0
-            parts_fib.append(pf)
-    if not parts_pos:
-        return np.zeros((0, 2)), np.zeros(0, dtype=int), np.zeros((0, 2), np.float32), True
-    return (np.vstack(parts_pos), np.concatenate(parts_mat).astype(int),
-            np.vstack(parts_fib).astype(np.float32), self_int)
+    # a genome within the evolve.py bounds makes well under 2^16 robot particles; a larger
+    # one is told its count by the first call and runs again with room for it
+    out = run(1 << 16, False)
+    if out is None:
+        out = run(max(int(n[0]), 1), True)
+    pos, mat, fib = out
+    k = int(n[0])
+    return pos[:k].copy(), mat[:k].astype(int), fib[:k].copy(), bool(si[0])
 
 
 def water_lattice(grid_res: int = 128, domain_height: float = 1.0) -> np.ndarray:
@@ -262,41 +135,30 @@ def water_mask(body, cands, margin: float) -> np.ndarray:
 def fill_tank(genome, max_particles: int, grid_res: int = 128, spawn_offset=None,
               water_margin: float = 0.005, with_payload: bool = True,
               domain_height: float = 1.0, axisym: bool | None = None):
-    """Fixed-size particle arrays for one instance (make_jelly.py:336-402).
+    """Fixed-size particle arrays for one instance (make_jelly.py:336-402), by jm_fill_tank.
 
     Returns ``pos (N,2) f32, mat (N,) i32, fiber (N,2) f32, info dict``. ``axisym`` None
     follows ``AXISYM``; the default spawn is then the axis (make_jelly.py:342-343)."""
+    from . import _lib
+    _lib.load()
     if axisym is None:
         axisym = AXISYM
-    if spawn_offset is None:
-        spawn_offset = AXISYM_SPAWN if axisym else DEFAULT_SPAWN
-    robot_pos, robot_mat, robot_fib, self_int = generate_phenotype(
-        genome, spawn_offset, grid_res=grid_res, with_payload=with_payload, axisym=axisym)
-    n_robot = len(robot_pos)
-    cands = water_lattice(grid_res, domain_height)
-    if n_robot > 0:
-        water = cands[water_mask(robot_pos, cands, water_margin)]
-    else:
-        water = cands
-    n_water = len(water)
-    if n_robot + n_water > max_particles:
-        n_water = max_particles - n_robot     # truncate water to fit (make_jelly.py:371-374)
-    pos = np.full((max_particles, 2), -1.0, dtype=np.float32)
-    mat = np.full(max_particles, -1, dtype=np.int32)
-    fib = np.zeros((max_particles, 2), dtype=np.float32)
-    fib[:, 1] = 1.0
-    if n_robot > 0:
-        pos[:n_robot] = robot_pos
-        mat[:n_robot] = robot_mat
-        fib[:n_robot] = robot_fib
-    if n_water > 0:
-        pos[n_robot:n_robot + n_water] = water[:n_water]
-        mat[n_robot:n_robot + n_water] = 0
-    muscle = int(np.sum(mat[:n_robot] == 3))
+    g = np.ascontiguousarray(genome, dtype=np.float64).ravel()
+    sp = None if spawn_offset is None else np.ascontiguousarray(spawn_offset,
+                                                                dtype=np.float64).ravel()
+    pos = np.empty((max_particles, 2), np.float32)
+    mat = np.empty(max_particles, np.int32)
+    fib = np.empty((max_particles, 2), np.float32)
+    info = np.zeros(5, np.int64)
+    _lib.call("jm_fill_tank", g.ctypes.data, len(g), int(max_particles), int(grid_res),
+              None if sp is None else sp.ctypes.data, float(water_margin),
+              int(bool(with_payload)), float(domain_height), int(bool(axisym)),
+              pos.ctypes.data, mat.ctypes.data, fib.ctypes.data, info.ctypes.data)
+    n_robot, n_water, muscle, jelly, self_int = (int(v) for v in info)
     return pos, mat, fib, {
         "n_robot": n_robot, "n_water": n_water, "n_total": n_robot + n_water,
         "n_dead": max_particles - (n_robot + n_water), "muscle_count": muscle,
-        "n_jelly": int(np.sum(mat[:n_robot] == 1)), "self_intersecting": self_int,
+        "n_jelly": jelly, "self_intersecting": bool(self_int),
     }
 
 
@@ -309,12 +171,17 @@ def population(seeds, max_particles: int = 80000, grid_res: int = 128,
     pos = np.empty((M, max_particles, 2), np.float32)
     mat = np.empty((M, max_particles), np.int32)
     fib = np.empty((M, max_particles, 2), np.float32)
-    genomes = np.empty((M, 11))
-    infos = []
-    for i, s in enumerate(seeds):
-        g = seeded_genome(s)
-        genomes[i] = g
-        pos[i], mat[i], fib[i], info = fill_tank(g, max_particles, grid_res, spawn_offset,
-                                                 domain_height=domain_height, axisym=axisym)
-        infos.append(info)
+    genomes = np.stack([seeded_genome(s) for s in seeds]) if M else np.empty((0, 11))
+
+    def one(i):
+        # the native call releases the GIL, so tanks build in parallel
+        pos[i], mat[i], fib[i], info = fill_tank(genomes[i], max_particles, grid_res,
+                                                 spawn_offset, domain_height=domain_height,
+                                                 axisym=axisym)
+        return info
+
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(M, 8, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(workers) as ex:
+        infos = list(ex.map(one, range(M)))
     return pos, mat, fib, genomes, infos

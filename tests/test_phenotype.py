@@ -14,6 +14,80 @@ CASES = ["aurelia", "seed0", "seed1", "seed2", "tall_seed0", "axisym_seed0",
          "axisym_view_aurelia"]
 
 
+def _fixture_args(golden, name):
+    dom = golden[f"{name}/domain"]
+    counts = golden[f"{name}/counts"]
+    return (golden[f"{name}/genome"], int(counts[0]), None if dom[1] < 0 else dom[1:],
+            float(dom[0]), bool(golden[f"{name}/axisym"]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_fill_tank_matches_reference_fixture(golden, name):
+    """The numpy restatement (oracle/phenotype.py, the check for the native generator) is
+    itself pinned by the reference's outputs."""
+    from oracle import phenotype as ref
+    g, nmax, spawn, height, axisym = _fixture_args(golden, name)
+    pos, mat, fib, info = ref.fill_tank(g, nmax, 128, spawn_offset=spawn,
+                                        domain_height=height, axisym=axisym)
+    nr = info["n_robot"]
+    counts = golden[f"{name}/counts"]
+    assert (nr, info["n_water"], info["muscle_count"], info["n_jelly"]) == tuple(
+        int(c) for c in counts[1:5])
+    np.testing.assert_array_equal(pos[:nr], golden[f"{name}/robot_pos"])
+    np.testing.assert_array_equal(mat[:nr], golden[f"{name}/robot_mat"].astype(np.int32))
+    np.testing.assert_array_equal(fib[:nr], golden[f"{name}/robot_fiber"])
+
+
+@pytest.mark.parametrize("mode", ["square", "tall", "axisym", "no_payload"])
+def test_native_fill_tank_matches_the_numpy_restatement(mode):
+    """jm_fill_tank against the numpy restatement, bit for bit (positions, materials, fibre
+    bits, counts), over seeded genomes spanning the evolve.py bounds, in the four fill
+    modes the reference's callers use."""
+    from oracle import phenotype as ref
+    kw = {"square": {}, "tall": {"domain_height": 2.0, "spawn_offset": phenotype.TALL_SPAWN},
+          "axisym": {"axisym": True}, "no_payload": {"with_payload": False}}[mode]
+    nmax = 160000 if mode == "tall" else 80000
+    seeds = range(40) if mode == "square" else range(100, 112)
+    for seed in seeds:
+        g = phenotype.seeded_genome(seed)
+        a = ref.fill_tank(g, nmax, 128, **kw)
+        b = phenotype.fill_tank(g, nmax, 128, axisym=kw.get("axisym", False),
+                                **{k: v for k, v in kw.items() if k != "axisym"})
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+        assert a[3] == b[3], (seed, a[3], b[3])
+
+
+def test_native_generate_phenotype_matches_the_numpy_restatement():
+    from oracle import phenotype as ref
+    for seed in range(200, 216):
+        g = phenotype.seeded_genome(seed)
+        for axisym in (False, True):
+            a = ref.generate_phenotype(g, axisym=axisym)
+            b = phenotype.generate_phenotype(g, axisym=axisym)
+            assert a[0].dtype == b[0].dtype == np.float64
+            np.testing.assert_array_equal(a[0], b[0])
+            np.testing.assert_array_equal(a[1], b[1])
+            np.testing.assert_array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+            assert a[3] == b[3]
+
+
+def test_population_matches_one_tank_at_a_time():
+    pos, mat, fib, genomes, infos = phenotype.population(range(5, 13), 80000, 128)
+    for i, s in enumerate(range(5, 13)):
+        p1, m1, f1, info = phenotype.fill_tank(phenotype.seeded_genome(s), 80000, 128)
+        np.testing.assert_array_equal(pos[i], p1)
+        np.testing.assert_array_equal(mat[i], m1)
+        np.testing.assert_array_equal(fib[i], f1)
+        assert infos[i] == info
+
+
+def test_fill_tank_rejects_a_robot_larger_than_the_tank():
+    with pytest.raises(RuntimeError, match="max_particles"):
+        phenotype.fill_tank(phenotype.AURELIA_GENOME, 100, 128)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_fill_tank_matches_reference_fixture(golden, name):
     g = golden[f"{name}/genome"]
