@@ -214,7 +214,7 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
                                         math=math)
         dev = tank.device
 
-        def episode(grad: bool, recompute: bool = True) -> float:
+        def episode(grad: bool, recompute: bool = True, prefetch: bool = True) -> float:
             st = st0.clone().requires_grad_(grad)
             leaf = lambda v: torch.full((1,), v, dtype=dtype, device=dev, requires_grad=grad)
             amp_scale = torch.ones(tank.P, dtype=dtype, device=dev, requires_grad=grad)
@@ -222,7 +222,7 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
             t0 = time.perf_counter()
             with torch.set_grad_enabled(grad):
                 out = tank.run(st, amp_scale, leaf(500.0), leaf(con), leaf(freq), leaf(700.0),
-                               2.5, 0.0, steps, checkpoint, recompute)
+                               2.5, 0.0, steps, checkpoint, recompute, prefetch=prefetch)
                 y = tank.payload_com_y(out).sum()
                 if grad:
                     y.backward()
@@ -232,12 +232,14 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
         episode(True)                              # warm-up (allocations, code objects)
         fwd = min(episode(False) for _ in range(2))
         fb = min(episode(True) for _ in range(2))
+        fb_serial = min(episode(True, prefetch=False) for _ in range(2))
         episode(True, recompute=False)
         fb_stored = min(episode(True, recompute=False) for _ in range(2))
         res["workload"] = (f"1 reference tank (seed 0, {tank.P} live), {steps} substeps, "
                            f"checkpoint every {checkpoint}, payload CoM y objective")
         res[key] = {
             "forward_ms": fwd * 1e3, "fwd_bwd_ms": fb * 1e3, "fwd_bwd_stored_ms": fb_stored * 1e3,
+            "fwd_bwd_serial_ms": fb_serial * 1e3,
             "fwd_bwd_over_forward": fb / fwd, "fwd_bwd_stored_over_forward": fb_stored / fwd,
             "fwd_bwd_over_fused_forward": fb / fused,
             "fwd_bwd_stored_over_fused_forward": fb_stored / fused}
@@ -287,7 +289,7 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
                                     math=math)
     dev = tank.device
 
-    def episode(grad: bool, n: int = steps, resort: int = 0) -> float:
+    def episode(grad: bool, n: int = steps, resort: int = 0, prefetch: bool = True) -> float:
         st = st0.clone()
         vec = lambda a: torch.tensor(np.asarray(a, np.float32), device=dev).requires_grad_(grad)
         amp, c, f = vec(np.full(M, 500.0)), vec(con), vec(freq)
@@ -295,7 +297,7 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
         t0 = time.perf_counter()
         with torch.set_grad_enabled(grad):
             out = tank.run(st, None, amp, c, f, 700.0, 2.5, 0.0, n, checkpoint,
-                           resort_every=resort)
+                           resort_every=resort, prefetch=prefetch)
             y = tank.payload_com_y(out).sum()
             if grad:
                 y.backward()
@@ -309,6 +311,10 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
     log(f"batched diff forward + backward ({math} arithmetic) {fb:.2f} s")
     out = {"forward_s": fwd, "fwd_bwd_s": fb, "fwd_bwd_over_forward": fb / fwd,
            "fwd_bwd_over_fused_forward": fb / fused}
+    if math == "fp32":
+        fb_serial = episode(True, prefetch=False)
+        log(f"batched diff forward + backward ({math} arithmetic), one stream {fb_serial:.2f} s")
+        out["fwd_bwd_serial_s"] = fb_serial
     if resort:
         episode(True, 2 * checkpoint, 1)
         fwd_rs = episode(False, steps, 1)

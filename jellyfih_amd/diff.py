@@ -45,6 +45,18 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+_SIDE_STREAMS: dict = {}
+
+
+def _side_stream(device: torch.device) -> "torch.cuda.Stream":
+    """One extra stream per device (default priority) for the backward pass's recomputation
+    of the previous block."""
+    s = _SIDE_STREAMS.get(device.index)
+    if s is None:
+        s = _SIDE_STREAMS[device.index] = torch.cuda.Stream(device=device)
+    return s
+
+
 def clock(t0: float, n: int, dt=None, fp64: bool = False) -> float:
     """The clock after ``n`` substeps from ``t0``: ``sim_time += dt`` in fp32 as the
     reference does (``mpm_sim.py:336-340``), or in fp64 for the fp64 path."""
@@ -287,14 +299,39 @@ class DiffTank:
         out = (cs[:, self._seg[1:]] - cs[:, self._seg[:-1]]).t()     # (M, C)
         return out.reshape((self.M,) + tuple(values.shape[1:]))
 
-    def trajectory(self, state, amp_scale, params, t0: float, n: int):
+    def trajectory_buffers(self, n: int):
+        """Uninitialised (traj (n+1, P, 12), grids (n, M, 3, nx*ny)) for :meth:`trajectory`."""
+        return (torch.empty((n + 1, self.P, STATE_COLS), dtype=self.dtype, device=self.device),
+                torch.empty((n, self.M, 3, self.G), dtype=self.dtype, device=self.device))
+
+    def trajectory(self, state, amp_scale, params, t0: float, n: int, side=None, bufs=None):
         """``n`` substeps from ``state`` with every state and grid stored: returns
-        (traj (n+1, P, 12) starting with ``state``, grids (n, M, 3, nx*ny))."""
-        traj = torch.empty((n + 1, self.P, STATE_COLS), dtype=self.dtype, device=self.device)
-        traj[0].copy_(state)
-        grids = torch.empty((n, self.M, 3, self.G), dtype=self.dtype, device=self.device)
-        self.forward_raw(state, amp_scale, params, t0, n, out=traj[1:], grids=grids)
-        return traj, grids
+        (traj (n+1, P, 12) starting with ``state``, grids (n, M, 3, nx*ny)), written into
+        the leading rows of ``bufs`` (from :meth:`trajectory_buffers`) when given.
+
+        With ``side`` (a stream), the substeps run on ``side`` after the current stream's
+        work so far; the returned event marks their end (wait on it before reading the
+        trajectory). The store-mode forward uses none of the handle's scratch that the
+        adjoint uses, so it may run beside :meth:`backward_raw` of another block."""
+        if bufs is None:
+            bufs = self.trajectory_buffers(n)
+        if bufs[0].shape[0] < n + 1 or bufs[1].shape[0] < n:
+            raise ValueError("trajectory buffers hold fewer substeps than the block")
+        traj, grids = bufs[0][:n + 1], bufs[1][:n]
+        if side is None:
+            traj[0].copy_(state)
+            self.forward_raw(state, amp_scale, params, t0, n, out=traj[1:], grids=grids)
+            return traj, grids
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            traj[0].copy_(state)
+            self.forward_raw(state, amp_scale, params, t0, n, out=traj[1:], grids=grids)
+            done = torch.cuda.Event()
+            done.record(side)
+        # the allocator must not hand these to other work before the side stream is done
+        for t in (traj, grids, state, amp_scale, params, self.inst, self.mat, self.fiber):
+            t.record_stream(side)
+        return traj, grids, done
 
     def backward_raw(self, state, amp_scale, params, t0: float, n: int, grad_out,
                      stored=None):
@@ -322,8 +359,8 @@ class DiffTank:
     # -- autograd ---------------------------------------------------------------------
     def run(self, state0: torch.Tensor, amp_scale=None, amp=500.0, contraction=0.2, freq=1.0,
             E_jelly=700.0, payload_density=2.5, t0: float = 0.0, nsteps: int = 100,
-            checkpoint: int = 100, recompute: bool = True, resort_every: int = 0
-            ) -> torch.Tensor:
+            checkpoint: int = 100, recompute: bool = True, resort_every: int = 0,
+            prefetch: bool = True) -> torch.Tensor:
         """``nsteps`` substeps from ``state0`` at time ``t0``, as autograd blocks of
         ``checkpoint`` substeps. Per-instance arguments are scalars or (M,) tensors.
 
@@ -331,6 +368,11 @@ class DiffTank:
         substeps in the backward pass. ``recompute=False`` keeps every block's states and
         grids from the forward pass instead: no second forward, at ((checkpoint + 1) * P * 12
         + checkpoint * M * 3 * nx * ny) scalars of device memory per block.
+
+        ``prefetch`` (with ``recompute``): the backward pass of block b starts the
+        recomputation of block b-1 on a second stream before it runs its own adjoint, so
+        the two run side by side (the recomputation does not depend on the adjoint); at
+        most two blocks' trajectories are held at once. Same kernels, same results.
 
         ``resort_every`` = K > 0: before every K-th block after the first, the particles
         are re-ordered by their current cells (``cell_permutation``), as the fused engine
@@ -354,6 +396,8 @@ class DiffTank:
         t = float(t0) if fp64 else float(np.float32(t0))
         done = blocks = 0
         cur, perm, amp_cur = self, None, amp_scale
+        episode = _Episode() if recompute and prefetch and self.device.type == "cuda" \
+            else None
         while done < nsteps:
             if resort_every > 0 and blocks > 0 and blocks % int(resort_every) == 0:
                 p = cur.cell_permutation(state)
@@ -362,7 +406,7 @@ class DiffTank:
                 cur = self._view(perm)
             n = min(int(checkpoint), nsteps - done)
             state = _Block.apply(state, amp_cur, amp, contraction, freq, E_jelly, pdens,
-                                 cur, t, n, recompute)
+                                 cur, t, n, recompute, episode)
             t = clock(t, n, self.dt, fp64)
             done += n
             blocks += 1
@@ -380,13 +424,66 @@ class DiffTank:
         return (s / self.payload_count.clamp(min=1)).to(state.dtype)
 
 
+class _Episode:
+    """The blocks of one ``DiffTank.run`` with recomputation, so that the backward pass of
+    block b can start block b-1's recomputation (on a side stream) before its own adjoint.
+
+    ``blocks[b]`` = (tank, state, amp_scale, params, t0, n) as block b's forward saw them;
+    ``pending[b]`` = (traj, grids, event, buffers) of a recomputation in flight. Two sets
+    of trajectory buffers, sized for the longest block, serve every block in turn: one is
+    read by the current adjoint while the other is written by the next recomputation."""
+
+    def __init__(self):
+        self.blocks: list = []
+        self.pending: dict = {}
+        self.free: list = []
+        self.n_max = 0
+
+    def add(self, rec) -> int:
+        self.blocks.append(rec)
+        self.n_max = max(self.n_max, rec[5])
+        return len(self.blocks) - 1
+
+    def _buffers(self, tank):
+        return self.free.pop() if self.free else tank.trajectory_buffers(self.n_max)
+
+    def start(self, b: int) -> None:
+        """Block b's recomputation on the side stream, after the current stream's work."""
+        if b < 0 or b in self.pending or self.blocks[b] is None:
+            return
+        tank, state, amp_c, params, t0, n = self.blocks[b]
+        bufs = self._buffers(tank)
+        self.pending[b] = tank.trajectory(state, amp_c, params, t0, n,
+                                          side=_side_stream(tank.device), bufs=bufs) + (bufs,)
+
+    def take(self, b: int):
+        """(traj, grids, buffers) of block b: the recomputation start(b) launched, ordered
+        before the current stream's next work, or else run now on the current stream."""
+        got = self.pending.pop(b, None)
+        if got is not None:
+            traj, grids, done, bufs = got
+            torch.cuda.current_stream(traj.device).wait_event(done)
+        else:
+            tank, state, amp_c, params, t0, n = self.blocks[b]
+            bufs = self._buffers(tank)
+            traj, grids = tank.trajectory(state, amp_c, params, t0, n, bufs=bufs)
+        self.blocks[b] = None     # its inputs are no longer needed here
+        return traj, grids, bufs
+
+    def release(self, bufs) -> None:
+        """Buffers whose readers are all on the current stream, queued: a later
+        recomputation on the side stream waits for them (start orders it after the current
+        stream's work)."""
+        self.free.append(bufs)
+
+
 class _Block(torch.autograd.Function):
     """``n`` substeps as one autograd node; keeps its input state (and, without
     recomputation, the block's stored states and grids)."""
 
     @staticmethod
     def forward(ctx, state, amp_scale, amp, contraction, freq, E_jelly, pdens, tank, t0, n,
-                recompute):
+                recompute, episode=None):
         params = torch.stack([amp, contraction, freq, pdens, E_jelly], 1).detach().contiguous()
         state_c = state.detach().contiguous()
         amp_c = amp_scale.detach().to(tank.dtype).contiguous()
@@ -398,13 +495,22 @@ class _Block(torch.autograd.Function):
             out = ctx.stored[0][-1].clone()
         ctx.save_for_backward(state_c, amp_c, params)
         ctx.tank, ctx.t0, ctx.n = tank, t0, n
+        ctx.episode = episode
+        ctx.index = episode.add((tank, state_c, amp_c, params, t0, n)) if episode else -1
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
         state, amp_c, params = ctx.saved_tensors
+        stored, ep, bufs = ctx.stored, ctx.episode, None
+        if stored is None and ep is not None:
+            traj, grids, bufs = ep.take(ctx.index)
+            stored = (traj, grids)
+            ep.start(ctx.index - 1)    # the previous block's recomputation, beside this adjoint
         gin, gamp, gpar = ctx.tank.backward_raw(state, amp_c, params, ctx.t0, ctx.n, grad_out,
-                                                stored=ctx.stored)
+                                                stored=stored)
+        if bufs is not None:
+            ep.release(bufs)
         ctx.stored = None
         return (gin, gamp, gpar[:, 0], gpar[:, 1], gpar[:, 2], gpar[:, 4], None, None, None,
-                None, None)
+                None, None, None)

@@ -166,7 +166,7 @@ def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype, math):
 
 
 def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every=0,
-                   math="fp64"):
+                   math="fp64", prefetch=True):
     """Autograd through DiffTank.run: d(payload CoM y displacement) w.r.t. the state,
     the per-particle multiplier and the per-instance parameters."""
     tank, st, amp, par = _gpu(cs, dtype, math)
@@ -175,7 +175,7 @@ def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every
     leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2, 4)]
     out = tank.run(st, amp, *leaves[:3], E_jelly=leaves[3], payload_density=par[:, 3],
                    t0=cs.t0, nsteps=cs.steps, checkpoint=checkpoint, recompute=recompute,
-                   resort_every=resort_every)
+                   resort_every=resort_every, prefetch=prefetch)
     y = tank.payload_com_y(out) - tank.payload_com_y(st.detach())
     y.sum().backward()
     g = lambda t: t.grad.detach().cpu().numpy().astype(np.float64)
@@ -262,17 +262,19 @@ def test_reference_tank_gradients_vs_fd(require_gpu, steps, dtype, math):
 
 
 def test_gradients_independent_of_checkpoint_interval(require_gpu):
-    """Recomputing from checkpoints every 100 or 64 substeps, or keeping the stored
-    trajectory, gives the gradients of one 300-substep block, up to the run-to-run noise of
-    atomics order (measured by repeating the 300-substep run). fp64, so that the noise is
-    small. The full state gradient is reported, not asserted: water resting against the
-    walls sits on the wall clamps' kink (zero grid velocity), so the branch its adjoint
-    takes follows that noise."""
+    """Recomputing from checkpoints every 100 or 64 substeps (each block's recomputation
+    beside the next block's adjoint on a second stream, or both on one stream), or keeping
+    the stored trajectory, gives the gradients of one 300-substep block, up to the
+    run-to-run noise of atomics order (measured by repeating the 300-substep run). fp64, so
+    that the noise is small. The full state gradient is reported, not asserted: water
+    resting against the walls sits on the wall clamps' kink (zero grid velocity), so the
+    branch its adjoint takes follows that noise."""
     say("reference tank seed0, 300 substeps, fp64: checkpoint 300 (twice) vs 100 vs 64 vs "
-        "stored")
+        "64 on one stream vs stored")
     cs, _, _, _ = reference_tank_case(300)
     runs = {k: _gpu_gradients(cs, checkpoint=k, dtype="float64") for k in (300, 100, 64)}
     runs["stored"] = _gpu_gradients(cs, checkpoint=100, dtype="float64", recompute=False)
+    runs["64, one stream"] = _gpu_gradients(cs, checkpoint=64, dtype="float64", prefetch=False)
     again = _gpu_gradients(cs, checkpoint=300, dtype="float64")
     body = cs.mat > 0
 
@@ -282,7 +284,7 @@ def test_gradients_independent_of_checkpoint_interval(require_gpu):
     base = groups(runs[300])
     floor = {n: np.linalg.norm(v - base[n]) / np.linalg.norm(base[n])
              for n, v in groups(again).items()}
-    for k in (100, 64, "stored"):
+    for k in (100, 64, "64, one stream", "stored"):
         for n, v in groups(runs[k]).items():
             err = np.linalg.norm(v - base[n]) / np.linalg.norm(base[n])
             say(f"checkpoint {k} vs 300, {n}: {err:.2e} (repeat of 300: {floor[n]:.2e})")
