@@ -38,8 +38,10 @@ Also reported on the same line:
 * view_mode: evolve.py's view loop through the mpm_sim facade (16 x 80K, 500 substep()
   calls from Python, then one abyss frame read back and encoded), against the same loop
   with one engine call per substep;
-* axisym: the axisymmetric variant's rate on 32 axisymmetric phenotypes (80K slots) and on
-  the tall view configuration (windows of 8 substeps from fresh loads);
+* axisym: the axisymmetric variant's rate on the fused and on the plain path, on 32
+  axisymmetric phenotypes (80K slots) and on the tall view configuration (one substep of
+  the intact state after each fresh load: the reference's formulation throws nearly every
+  particle out of the transfer range within ~10 substeps);
 * tank_build_s: host seconds to build the 32 tanks (phenotype.population, the fill_tank
   restatement with its native steps).
 
@@ -450,19 +452,28 @@ def render_frames(eng, live: int, reps: int = 20) -> dict:
     return out
 
 
-def axisym_rates(reps: int = 10, n: int = 8) -> dict:
-    """The axisymmetric variant (JELLY_AXISYM=1, plain path, one launch per stage for the
-    batch): live particle-substeps/s on 32 seeded 80K-slot tanks filled as evolve.py fills
-    them under JELLY_AXISYM=1 (right-half profiles at the axis, make_jelly.py:276-304,343),
-    and on helpers/view_axisym.py's tank (Aurelia's right-half profile at spawn (0.5, 0.40),
+def axisym_rates(reps: int = 10) -> dict:
+    """The axisymmetric variant (JELLY_AXISYM=1) on the fused path (k_fused's axisymmetric
+    instantiation) and on the plain path (one launch per stage for the batch): live
+    particle-substeps/s on 32 seeded 80K-slot tanks filled as evolve.py fills them under
+    JELLY_AXISYM=1 (right-half profiles at the axis, make_jelly.py:276-304,343), and on
+    helpers/view_axisym.py's tank (Aurelia's right-half profile at spawn (0.5, 0.40),
     128x256, H=2, 160K slots, view_axisym.py:19-25,129-135; default actuation parameters).
-    The reference's variant diverges within ~10 substeps next to the axis (its affine
-    momentum is not weighted by r), so each timed window is `n` substeps from a fresh load,
-    `reps` windows, HIP events on the engine's stream."""
+
+    The reference's variant does not weight the affine momentum by r, so particles next to
+    the axis get grid velocities of order 1/r, and within ~10 substeps nearly every particle
+    is thrown against the walls, outside the transfer range (measured: 6,671 of the batch's
+    1,995,267 particles in range after 10 substeps, 12 of the view tank's 126,276 after
+    100), where it does no work. So the rate is taken on the intact state right after a
+    load: per window, a fresh load, then on the fused path one 2-substep block with HIP
+    events around its middle kernel (G2P of substep 0 and P2G of substep 1, every instance
+    group) and on the plain path one untimed substep and then one timed substep (P2G, grid
+    op, G2P); `reps` windows after one warm-up window. The fused path runs without graphs
+    here (a fresh load drops its captured graphs)."""
     import torch
     from jellyfih_amd import phenotype
     from jellyfih_amd.engine import Engine
-    out = {"window_substeps": n, "windows": reps}
+    out = {"windows": reps}
     batch = phenotype.population(range(32), 80000, 128, axisym=True)
     view = phenotype.fill_tank(phenotype.AURELIA_GENOME, 160000, 128,
                                spawn_offset=phenotype.DEFAULT_SPAWN, domain_height=2.0,
@@ -471,29 +482,38 @@ def axisym_rates(reps: int = 10, n: int = 8) -> dict:
              "view_tall_1x160k": (view[0][None], view[1][None], view[2][None], None,
                                   {"n_grid_y": 256, "domain_height": 2.0})}
     for name, (P, Mt, Fb, G, kw) in cases.items():
-        eng = Engine(P.shape[0], P.shape[1], path="plain", axisym=True, **kw)
-        stream = torch.cuda.ExternalStream(eng.stream)
         live = int((Mt >= 0).sum())
-        total = 0.0
-        for r in range(reps + 1):
-            if G is None:
-                eng.load_particles(0, P[0], Mt[0], Fb[0])
-            else:
-                load_engine(eng, P, Mt, Fb, G)
-            eng.set_time(0.0)
-            eng.substep(1)               # commits the load; not timed
-            eng.sync()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            eng.substep(n)
-            e1.record(stream)
-            eng.sync()
-            if r:                        # the first window warms up
-                total += e0.elapsed_time(e1) / 1e3
-        eng.close()
-        out[name] = {"live_particles": live, "particle_substeps_per_s": live * n * reps / total,
-                     "ms_per_substep": total / (n * reps) * 1e3}
+        out[name] = {"live_particles": live}
+        for path in ("fused", "plain"):
+            eng = Engine(P.shape[0], P.shape[1], path=path, axisym=True, **kw)
+            if path == "fused":
+                eng.set_tuning(use_graphs=0)
+            stream = torch.cuda.ExternalStream(eng.stream)
+            times = []
+            for r in range(reps + 1):
+                if G is None:
+                    eng.load_particles(0, P[0], Mt[0], Fb[0])
+                else:
+                    load_engine(eng, P, Mt, Fb, G)
+                eng.set_time(0.0)
+                if path == "fused":
+                    ms = eng.timed_block(2)[0]
+                else:
+                    eng.substep(1)               # commits the load; not timed
+                    eng.sync()
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    eng.substep(1)
+                    e1.record(stream)
+                    eng.sync()
+                    ms = e0.elapsed_time(e1)
+                if r:                            # the first window warms up
+                    times.append(ms / 1e3)
+            eng.close()
+            sec = float(np.median(times))
+            out[name][path] = {"particle_substeps_per_s": live / sec,
+                               "ms_per_substep": sec * 1e3}
     return out
 
 
@@ -699,7 +719,7 @@ def main() -> int:
         log(f"view {view}")
     log(f"render {rend}")
 
-    log("axisymmetric variant (plain path)")
+    log("axisymmetric variant (fused and plain paths)")
     axisym = axisym_rates()
     log(f"axisym {axisym}")
 

@@ -84,7 +84,8 @@ typedef struct jm_config {
   int32_t path;          /* jm_path */
   int32_t axisym;        /* JELLY_AXISYM, mpm_sim.py:17: 1 = axisymmetric (r, z) variant
                             (ring-weighted P2G, hoop stress, axis BC; mpm_sim.py:127-133,
-                            163-165,236-304); runs on JM_PATH_PLAIN only */
+                            163-165,236-304), on either path; the fused path runs it in
+                            chunks of 256 x 2 or 256 x 4 (jm_set_tuning refuses others) */
   double domain_height;  /* JELLY_DOMAIN_H, mpm_sim.py:16 */
   int32_t nan_invalid;   /* 1: jm_run_batch_headless marks an instance invalid (valid = 0)
                             when its final CoM is not finite or any of its live particles

@@ -92,7 +92,8 @@ struct jm_handle {
   float* vort = nullptr;      // renderer: M * G floats, node (i, j) at j * nx + i
   float* rhue = nullptr;      // renderer: 2 * M floats, instance_hue then instance_muscle_hue
   std::vector<float> rhue_host;  // what rhue holds
-  float* hoop = nullptr;      // axisymmetric variant: [m][2][G] stress grids of the hoop term
+  float* hoop = nullptr;      // axisymmetric variant: 3 buffers of [m][2][G] stress grids of
+                              // the hoop term, one beside each grid buffer (plain: buffer 0)
 
   // host mirrors
   std::vector<float> h_param[5];  // amp, contraction, freq, pdens, E_jelly
@@ -422,6 +423,8 @@ int commit(jm_handle* h) {
                          h->stream));
   JM_TRY(build_chunks(h));
   HIP_TRY(hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * sizeof(float), h->stream));
+  if (h->hoop)
+    HIP_TRY(hipMemsetAsync(h->hoop, 0, (size_t)3 * M * 2 * h->G * sizeof(float), h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->acc_base = 0;
   h->last_read = -1;
@@ -510,19 +513,22 @@ jm::FusedArgs fused_args(jm_handle* h, int g) {
 
 hipStream_t group_stream(jm_handle* h, int g) { return g == 0 ? h->stream : h->gstream[g]; }
 
-template <int B, int P>
+template <int B, int P, bool AXI = false>
 void launch_fused_t(jm_handle* h, int g, bool g2p, bool p2g, const jm::FusedArgs& a) {
   dim3 grid(std::max(1, h->grp[g].grid_blocks)), block(B);
   hipStream_t st = group_stream(h, g);
   if (g2p && p2g)
-    hipLaunchKernelGGL((jm::k_fused<B, P, true, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((jm::k_fused<B, P, true, true, AXI>), grid, block, 0, st, a);
   else if (g2p)
-    hipLaunchKernelGGL((jm::k_fused<B, P, true, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((jm::k_fused<B, P, true, false, AXI>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((jm::k_fused<B, P, false, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((jm::k_fused<B, P, false, true, AXI>), grid, block, 0, st, a);
 }
 
-bool tuning_supported(int threads, int ppt) {
+// Launch shapes of the fused kernel; the axisymmetric variant is built for 256 x 2 and
+// 256 x 4 only.
+bool tuning_supported(int threads, int ppt, bool axisym) {
+  if (axisym) return threads == 256 && (ppt == 2 || ppt == 4);
   static const int combos[][2] = {{128, 8}, {128, 16}, {256, 2}, {256, 4}, {256, 8},
                                   {256, 16}, {512, 2}, {512, 4}, {512, 8}};
   for (auto& c : combos)
@@ -532,6 +538,10 @@ bool tuning_supported(int threads, int ppt) {
 
 void launch_fused(jm_handle* h, int g, bool g2p, bool p2g, const jm::FusedArgs& a) {
   const int T = h->tun.threads, P = h->tun.ppt;
+  if (h->cfg.axisym) {
+    if (P == 2) return launch_fused_t<256, 2, true>(h, g, g2p, p2g, a);
+    return launch_fused_t<256, 4, true>(h, g, g2p, p2g, a);
+  }
 #define JM_CASE(TT, PP) \
   if (T == TT && P == PP) return launch_fused_t<TT, PP>(h, g, g2p, p2g, a);
   JM_CASE(128, 8)
@@ -548,6 +558,11 @@ void launch_fused(jm_handle* h, int g, bool g2p, bool p2g, const jm::FusedArgs& 
 
 float* grid_buf(jm_handle* h, int b) { return h->acc + (size_t)b * h->M * 3 * h->G; }
 
+// The stress planes beside grid buffer b (axisymmetric variant), else null.
+float* hoop_buf(jm_handle* h, int b) {
+  return h->hoop ? h->hoop + (size_t)b * h->M * 2 * h->G : nullptr;
+}
+
 // Kernel kk (0..n) of a block of n fused substeps with grid-buffer base b, for instance
 // group g: it gathers from buffer (b+kk-1)%3, scatters into (b+kk)%3 and zeroes (b+kk+1)%3
 // — the next kernel's scatter target, last read two kernels ago (each workgroup touches only
@@ -560,6 +575,9 @@ void enqueue_fused_kernel(jm_handle* h, int g, jm::FusedArgs& a, int kk, int n, 
   a.acc_r = g2p ? grid_buf(h, (b + kk - 1) % 3) : nullptr;
   a.acc_w = p2g ? grid_buf(h, (b + kk) % 3) : nullptr;
   a.acc_z = grid_buf(h, (b + kk + 1) % 3);
+  a.hoop_r = g2p ? hoop_buf(h, (b + kk - 1) % 3) : nullptr;
+  a.hoop_w = p2g ? hoop_buf(h, (b + kk) % 3) : nullptr;
+  a.hoop_z = hoop_buf(h, (b + kk + 1) % 3);
   if (p2g) {  // substep kk reads t_kk, writes t_{kk+1}
     a.t_rd = kk % 2;
     a.t_wr = (kk + 1) % 2;
@@ -608,6 +626,20 @@ void finish_block(jm_handle* h, int n, int b, bool sort) {
   h->last_read = (b + n - 1) % 3;
   h->acc_base = (b + n + 1) % 3;
   h->grid_is_velocity = false;
+}
+
+// Axisymmetric fused path, after its blocks: the grid the last G2P gathered from becomes
+// velocities in place (k_axisym_velocity), so grid reads and the renderer see what the
+// plain path leaves.
+int finish_axisym(jm_handle* h) {
+  if (!h->hoop || h->cfg.path != JM_PATH_FUSED || h->last_read < 0 || h->grid_is_velocity)
+    return JM_OK;
+  hipLaunchKernelGGL(jm::k_axisym_velocity, dim3(blocks_for(h->G, 256), h->M), dim3(256), 0,
+                     h->stream, grid_buf(h, h->last_read), hoop_buf(h, h->last_read), h->M,
+                     h->G, h->k);
+  HIP_TRY(hipGetLastError());
+  h->grid_is_velocity = true;
+  return JM_OK;
 }
 
 // Captured graph of group g's n + 1 kernels of a block with grid-buffer base b.
@@ -673,7 +705,7 @@ int run_fused(jm_handle* h, int64_t steps) {
     finish_block(h, n, b, sort);
     left -= n;
   }
-  return JM_OK;
+  return finish_axisym(h);
 }
 
 jm::PlainArgs plain_args(jm_handle* h, int m0) {
@@ -839,8 +871,6 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
   if (cfg->nan_invalid != 0 && cfg->nan_invalid != 1)
     return fail(JM_ERR_ARG, "nan_invalid must be 0 or 1");
   if (cfg->reserved != 0) return fail(JM_ERR_ARG, "reserved must be 0");
-  if (cfg->axisym && cfg->path != JM_PATH_PLAIN)
-    return fail(JM_ERR_ARG, "the axisymmetric variant runs on JM_PATH_PLAIN");
   const int nx = 128 * cfg->quality;
   const int ny = cfg->n_grid_y > 0 ? cfg->n_grid_y : nx;
   if (ny < 8 || ny % 4 != 0) return fail(JM_ERR_ARG, "n_grid_y must be a multiple of 4, >= 8");
@@ -904,7 +934,7 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
             dm((void**)&h->nonfinite_d, M * 4) &&
             dm((void**)&h->staging, (size_t)M * 2 * h->G * 4) &&
             dm((void**)&h->damp, (size_t)(nx + ny) * 4) &&
-            (!axisym || dm((void**)&h->hoop, (size_t)M * 2 * h->G * 4));
+            (!axisym || dm((void**)&h->hoop, (size_t)3 * M * 2 * h->G * 4));
   if (!ok) return cleanup(fail(JM_ERR_NOMEM, "hipMalloc(engine buffers)"));
   std::vector<float> damp(nx + ny);
   for (int i = 0; i < nx; ++i) damp[i] = jm::damping_factor(i, nx, h->k.damp_x);
@@ -913,6 +943,8 @@ int jm_create(const jm_config* cfg, jm_handle** out) {
     return cleanup(fail(JM_ERR_HIP, "damping table upload"));
   h->k.damp = h->damp;
   if (hipMemsetAsync(h->acc, 0, (size_t)3 * M * 3 * h->G * 4, h->stream) != hipSuccess ||
+      (h->hoop &&
+       hipMemsetAsync(h->hoop, 0, (size_t)3 * M * 2 * h->G * 4, h->stream) != hipSuccess) ||
       hipMemsetAsync(h->t_buf, 0, 8 * kMaxGroups, h->stream) != hipSuccess ||
       hipMemsetAsync(h->stats_d, 0, (size_t)M * 6 * 4, h->stream) != hipSuccess ||
       hipMemsetAsync(h->n_live_d, 0, M * 4, h->stream) != hipSuccess ||
@@ -936,8 +968,9 @@ int jm_set_tuning(jm_handle* h, const jm_tuning* t) {
   if (t->block_substeps < 1 || t->block_substeps > 100000)
     return fail(JM_ERR_ARG, "block_substeps out of range");
   if (t->sort_interval < 1) return fail(JM_ERR_ARG, "sort_interval must be >= 1");
-  if (!tuning_supported(t->threads, t->ppt))
-    return fail(JM_ERR_ARG, "unsupported (threads, ppt) combination");
+  if (!tuning_supported(t->threads, t->ppt, h->cfg.axisym != 0))
+    return fail(JM_ERR_ARG, h->cfg.axisym ? "the axisymmetric variant runs 256 x 2 or 256 x 4"
+                                          : "unsupported (threads, ppt) combination");
   if (t->sort_strip < 0 || t->sort_strip > h->ny)
     return fail(JM_ERR_ARG, "sort_strip must be in [0, grid rows]");
   if (t->instance_groups < 1 || t->instance_groups > kMaxGroups)
@@ -1312,6 +1345,7 @@ int jm_timed_block(jm_handle* h, int32_t n, float* fused_ms, float* block_ms) {
   if (e != hipSuccess) return fail(JM_ERR_HIP, std::string("timed block: ") + hipGetErrorString(e));
   if (rc != JM_OK) return rc == JM_ERR_HIP ? fail(rc, "timed block: enqueue failed") : rc;
   finish_block(h, n, b, sort);
+  JM_TRY(finish_axisym(h));
   if (fused_ms) *fused_ms = mid / (float)(n - 1);
   if (block_ms) *block_ms = all;
   return JM_OK;

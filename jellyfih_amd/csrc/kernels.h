@@ -351,6 +351,11 @@ struct FusedArgs {
   const float* acc_r;  // gather buffer (null when !DO_G2P)
   float* acc_w;        // scatter buffer (null when !DO_P2G)
   float* acc_z;        // buffer zeroed for the substep after next (may be null)
+  // axisymmetric variant: each grid buffer's two stress planes ([m][2][G], mass-weighted
+  // s_rr and s_tt), gathered, scattered and zeroed with acc_r, acc_w, acc_z; else null
+  const float* hoop_r;
+  float* hoop_w;
+  float* hoop_z;
   float* t_buf;
   int t_rd, t_wr, t_adv;  // t_buf[t_wr] = t_buf[t_rd] (+dt if t_adv), when t_wr >= 0
   const int4* chunks;     // (instance, first particle, count, chunk index in instance)
@@ -485,10 +490,12 @@ __device__ __forceinline__ void g2p_gather(const float2* g, int ld, float fx0, f
 }
 
 // G2P + advection of one particle straight from the global grid (grid op applied per
-// node); the slow path for chunks whose window does not fit in LDS. Reads and writes the
-// particle's x, v, C in memory.
-__device__ __forceinline__ void g2p_global(const float* gp, int G, const PhysConsts* kp,
-                                        float2* xs, float2* vs, float4* Cs, int gi) {
+// node, the axisymmetric one when hp, the instance's two stress planes, is given); the
+// slow path for chunks whose window does not fit in LDS. Reads and writes the particle's
+// x, v, C in memory.
+__device__ __forceinline__ void g2p_global(const float* gp, const float* hp, int G,
+                                        const PhysConsts* kp, float2* xs, float2* vs,
+                                        float4* Cs, int gi) {
   const PhysConsts& k = *kp;
   float2 x = xs[gi], v = vs[gi];
   float4 C = Cs[gi];
@@ -504,7 +511,10 @@ __device__ __forceinline__ void g2p_global(const float* gp, int G, const PhysCon
     for (int ii = 0; ii < 3; ++ii)
       for (int jj = 0; jj < 3; ++jj) {
         const int i = b0 + ii, j = b1 + jj, node = j * k.nx + i;
-        const float2 n = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
+        const float2 n =
+            hp ? grid_node_axisym(gp[node], gp[G + node], gp[2 * G + node], hp[node],
+                                  hp[G + node], i, j, k)
+               : grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
         const float w = wx[ii] * wy[jj], d0 = (float)ii - fx0, d1 = (float)jj - fx1;
         v0 += w * n.x;
         v1 += w * n.y;
@@ -550,19 +560,29 @@ __device__ __forceinline__ int packed_cell(float2 x, const PhysConsts& k, float&
 // Register budget: five waves per SIMD where the LDS window allows it (chunks of up to 1024
 // particles in workgroups of at least 256 threads); at 32 x 80K, 256 x 4 runs 39.5 us per
 // launch at five waves, 42.2 us at four and 53.2 us at six (which spills).
-template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G>
+//
+// AXI: the axisymmetric variant (mpm_sim.py:163-165,236-304 with axisym): P2G weights the
+// particle-velocity momentum and the mass by the radius r = x (a particle is a ring) and
+// scatters two more planes, the mass-weighted Kirchhoff s_rr and the hoop stress of the
+// circumferential stretch r / r_ref, into the stress grids; the grid op adds the hoop
+// correction and the axis condition (grid_node_axisym).
+template <int BLOCK, int PPT, bool DO_G2P, bool DO_P2G, bool AXI = false>
 __global__ void __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))) k_fused(FusedArgs a) {
+__attribute__((amdgpu_waves_per_eu(!AXI && BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1)))
+k_fused(FusedArgs a) {
   // Grid-window capacity (nodes): a chunk of P sorted particles (4 per cell in water)
   // covers about P/4 cells (P/4/S columns of an S-row strip, or P/512 + 2 rows of 128 in
   // row-major order), plus the 2-node stencil halo and drift since the last sort.
   // 1280 nodes (30 KB of LDS) for 1024-particle chunks: five workgroups per CU, as many as
   // the kernel's registers allow, with twice the typical window (~400-650 nodes) as
-  // headroom before a chunk falls back to the slow global path.
-  constexpr int WMAX = (BLOCK * PPT <= 1024) ? 1280 : (BLOCK * PPT <= 2048) ? 1536 : 2560;
-  // One LDS buffer: the gather window as float2, then the scatter window as three planes of
+  // headroom before a chunk falls back to the slow global path. The axisymmetric variant's
+  // five planes keep the same 30 KB with 768 nodes.
+  constexpr int NPL = AXI ? 5 : 3;
+  constexpr int WMAX = AXI ? ((BLOCK * PPT <= 1024) ? 768 : 1024)
+                           : (BLOCK * PPT <= 1024) ? 1280 : (BLOCK * PPT <= 2048) ? 1536 : 2560;
+  // One LDS buffer: the gather window as float2, then the scatter window as NPL planes of
   // 64-bit fixed-point accumulators.
-  __shared__ unsigned long long lds64[3 * WMAX];
+  __shared__ unsigned long long lds64[NPL * WMAX];
   float* lds = reinterpret_cast<float*>(lds64);
   __shared__ int red[BLOCK / 64][4];
   const int tid = threadIdx.x;
@@ -593,6 +613,15 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
     const int64_t lo = n4 * cidx / nch, hi = n4 * (cidx + 1) / nch;
     float4* z = reinterpret_cast<float4*>(a.acc_z + (size_t)m * 3 * G);
     for (int64_t q = lo + tid; q < hi; q += BLOCK) z[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if constexpr (AXI) {
+    if (a.hoop_z) {
+      const int nch = a.nchunks[m];
+      const int64_t n4 = (int64_t)2 * G / 4;
+      const int64_t lo = n4 * cidx / nch, hi = n4 * (cidx + 1) / nch;
+      float4* z = reinterpret_cast<float4*>(a.hoop_z + (size_t)m * 2 * G);
+      for (int64_t q = lo + tid; q < hi; q += BLOCK) z[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   }
 
   // This chunk's slices of the particle arrays (uniform bases; lanes add 32-bit offsets).
@@ -642,6 +671,7 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
     const bool any = mn0 <= mx0;
     const bool use_lds = any && ww * wh <= WMAX;
     const float* gp = a.acc_r + (size_t)m * 3 * G;
+    const float* hp = AXI ? a.hoop_r + (size_t)m * 2 * G : nullptr;
     float2* lv = reinterpret_cast<float2*>(lds);
     if (use_lds) {
       const int nodes = ww * wh;
@@ -650,7 +680,11 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
         const int jj = window_row(q, rcp_ww), ii = q - jj * ww;
         const int i = mn0 + ii, j = mn1 + jj;
         const int node = j * nx + i;
-        lv[q] = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
+        if constexpr (AXI)
+          lv[q] = grid_node_axisym(gp[node], gp[G + node], gp[2 * G + node], hp[node],
+                                   hp[G + node], i, j, k);
+        else
+          lv[q] = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
       }
     }
     __syncthreads();
@@ -675,7 +709,7 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
       for (int p = 0; p < PPT; ++p) {
         const int li = p * BLOCK + tid;
         if (li >= count) continue;
-        g2p_global(gp, G, &a.k, a.s.x, a.s.v, a.s.C, start + li);
+        g2p_global(gp, hp, G, &a.k, a.s.x, a.s.v, a.s.C, start + li);
       }
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
@@ -739,12 +773,13 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
     const int nodes = use_lds ? ww * wh : 0;
     const float rcp_ww = __builtin_amdgcn_rcpf((float)ww);
     for (int q = tid; q < nodes; q += BLOCK) {
-      lds64[q] = 0ull;
-      lds64[WMAX + q] = 0ull;
-      lds64[2 * WMAX + q] = 0ull;
+#pragma unroll
+      for (int pl = 0; pl < NPL; ++pl) lds64[pl * WMAX + q] = 0ull;
     }
     __syncthreads();
     float* gw = a.acc_w + (size_t)m * 3 * G;
+    float* hw = AXI ? a.hoop_w + (size_t)m * 2 * G : nullptr;
+    const float* const rrefs = AXI ? a.s.r_ref + start : nullptr;
     const int lane = tid & 63;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
@@ -761,6 +796,9 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
       // the 21 per-particle factors.
       float mass = 0.f;
       Mat2 aff = {0.f, 0.f, 0.f, 0.f};
+      // axisymmetric: mass * r (the ring's weight of momentum and mass), mass * s_rr and
+      // mass * s_tt (mpm_sim.py:236-247,258-267)
+      float mr = 0.f, ms_rr = 0.f, ms_tt = 0.f;
       if (inr) {
         float2 fib = make_float2(0.f, 1.f);
         if (mt[p] == 3) fib = elem(fibs, li);
@@ -771,19 +809,32 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
         if (o.jp_changes) elem(Jps, li) = (elem(Jps, li) * o.q0) * o.q1;
         mass = o.mass;
         aff = o.affine;
+        if constexpr (AXI) {
+          const float rp = x[p].x;
+          const float rr = fmaxf(elem(rrefs, li), k.r_ref_floor);
+          const float lam = fmaxf(rp, k.r_floor) / rr;
+          const float khoop = ((2.0f * o.mu) * (lam - 1.0f)) + ((o.la * o.J) * (o.J - 1.0f));
+          mr = o.mass * rp;
+          ms_rr = o.mass * o.s00;
+          ms_tt = o.mass * khoop;
+        }
       }
-      const float mv0 = mass * (inr ? v[p].x : 0.f), mv1 = mass * (inr ? v[p].y : 0.f);
-      float wx[3], wy[3], ey[3], P0[3], Q0[3], P1[3], Q1[3], Mi[3];
+      // the particle-velocity momentum's weight: m, or m r for a ring
+      const float mw = AXI ? mr : mass;
+      const float mv0 = mw * (inr ? v[p].x : 0.f), mv1 = mw * (inr ? v[p].y : 0.f);
+      float wx[3], wy[3], ey[3], P0[3], Q0[3], P1[3], Q1[3], Mi[3], Si[3], Ti[3];
       bspline(fx0, wx);
       bspline(fx1, wy);
       const float bnd = 1.5f * k.dx;
       // No fmaxf here: it drops NaN operands. Each comparison fails on NaN, and the
       // fractional offsets are tested too (a NaN position truncates to cell 0 with NaN
-      // weights).
+      // weights). The axisymmetric mass plane holds m r < m (r < 1); its stress planes'
+      // terms are bounded like the momentum's.
       const bool fixed_ok = inr && use_lds &&
                             fabsf(mv0) + bnd * (fabsf(aff.a00) + fabsf(aff.a01)) < kFixMax &&
                             fabsf(mv1) + bnd * (fabsf(aff.a10) + fabsf(aff.a11)) < kFixMax &&
-                            mass < kFixMassMax && fx0 + fx1 < 4.0f;
+                            mass < kFixMassMax && fx0 + fx1 < 4.0f &&
+                            (!AXI || (fabsf(ms_rr) < kFixMax && fabsf(ms_tt) < kFixMax));
       const float sc = fixed_ok ? kFixPre : (inr ? 1.0f : 0.0f);
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
@@ -793,7 +844,11 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
         Q0[ii] = ai * aff.a01;
         P1[ii] = ai * mv1 + bi * aff.a10;
         Q1[ii] = ai * aff.a11;
-        Mi[ii] = ai * mass;
+        Mi[ii] = ai * mw;
+        if constexpr (AXI) {
+          Si[ii] = ai * ms_rr;
+          Ti[ii] = ai * ms_tt;
+        }
       }
 #pragma unroll
       for (int jj = 0; jj < 3; ++jj) ey[jj] = wy[jj] * (((float)jj - fx1) * k.dx);
@@ -801,10 +856,21 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
         scatter_global(gw, G, nx, b0, b1, {wy[0], wy[1], wy[2]}, {ey[0], ey[1], ey[2]},
                        {P0[0], P0[1], P0[2]}, {Q0[0], Q0[1], Q0[2]}, {P1[0], P1[1], P1[2]},
                        {Q1[0], Q1[1], Q1[2]}, {Mi[0], Mi[1], Mi[2]});
+        if constexpr (AXI) {
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii) {
+              const int node = (b1 + jj) * nx + (b0 + ii);
+              unsafeAtomicAdd(hw + node, wy[jj] * Si[ii]);
+              unsafeAtomicAdd(hw + G + node, wy[jj] * Ti[ii]);
+            }
+        }
         // done: its neighbour lane adds this lane's (now zero, finite) terms times 0
 #pragma unroll
         for (int ii = 0; ii < 3; ++ii) {
           P0[ii] = Q0[ii] = P1[ii] = Q1[ii] = Mi[ii] = 0.f;
+          if constexpr (AXI) Si[ii] = Ti[ii] = 0.f;
           wy[ii] = ey[ii] = 0.f;
         }
       }
@@ -827,11 +893,21 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
           float c[3] = {wy[jj] * P0[ii] + ey[jj] * Q0[ii], wy[jj] * P1[ii] + ey[jj] * Q1[ii],
                         wy[jj] * Mi[ii]};
           presum3(c[0], c[1], c[2], lg.f1, lg.f2);
+          float s[2] = {0.f, 0.f};
+          if constexpr (AXI) {
+            s[0] = wy[jj] * Si[ii];
+            s[1] = wy[jj] * Ti[ii];
+            presum2(s[0], s[1], lg.f1, lg.f2);
+          }
           if (lg.emit) {
             const int q = q0 + jj * ww + ii;
             add_fixed_scaled(&lds64[q], c[0]);
             add_fixed_scaled(&lds64[WMAX + q], c[1]);
             add_fixed_scaled(&lds64[2 * WMAX + q], c[2]);
+            if constexpr (AXI) {
+              add_fixed_scaled(&lds64[3 * WMAX + q], s[0]);
+              add_fixed_scaled(&lds64[4 * WMAX + q], s[1]);
+            }
           }
         }
       }
@@ -841,12 +917,18 @@ __attribute__((amdgpu_waves_per_eu(BLOCK >= 256 && BLOCK * PPT <= 1024 ? 5 : 1))
       const int jj = window_row(q, rcp_ww), ii = q - jj * ww;
       const int node = (mn1 + jj) * nx + (mn0 + ii);
       const unsigned long long vm = lds64[2 * WMAX + q];
-      const unsigned n = biased_count(vm);  // emissions into this node (all three planes)
+      const unsigned n = biased_count(vm);  // emissions into this node (all planes)
       const float c0 = from_biased_bits(lds64[q], n), c1 = from_biased_bits(lds64[WMAX + q], n),
                   cm = from_biased_bits(vm, n);
       if (c0 != 0.f) unsafeAtomicAdd(gw + node, c0);
       if (c1 != 0.f) unsafeAtomicAdd(gw + G + node, c1);
       if (cm != 0.f) unsafeAtomicAdd(gw + 2 * G + node, cm);
+      if constexpr (AXI) {
+        const float s0 = from_biased_bits(lds64[3 * WMAX + q], n),
+                    s1 = from_biased_bits(lds64[4 * WMAX + q], n);
+        if (s0 != 0.f) unsafeAtomicAdd(hw + node, s0);
+        if (s1 != 0.f) unsafeAtomicAdd(hw + G + node, s1);
+      }
     }
   }
 
@@ -874,6 +956,24 @@ __global__ void k_grid_velocity(const float* acc, float* out, int M, int G, Phys
   float2 g = grid_node(gp[node], gp[G + node], gp[2 * G + node], i, j, k);
   out[(size_t)m * 2 * G + node] = g.x;
   out[(size_t)m * 2 * G + G + node] = g.y;
+}
+
+// Axisymmetric fused path: the grid buffer the last G2P gathered from becomes velocities in
+// place (planes 0-1 through grid_node_axisym with the buffer's stress planes; the mass
+// plane stays), as the plain path's grid op leaves its grid, so that grid reads and the
+// renderer use it without the axisymmetric grid op. No later substep reads that buffer
+// (the next block's first kernel zeroes it).
+__global__ void k_axisym_velocity(float* acc, const float* hoop, int M, int G, PhysConsts k) {
+  const int m = blockIdx.y;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || node >= G) return;
+  float* gp = acc + (size_t)m * 3 * G;
+  const float* hp = hoop + (size_t)m * 2 * G;
+  const int j = node / k.nx, i = node - j * k.nx;
+  const float2 g =
+      grid_node_axisym(gp[node], gp[G + node], gp[2 * G + node], hp[node], hp[G + node], i, j, k);
+  gp[node] = g.x;
+  gp[G + node] = g.y;
 }
 
 // ---------------------------------------------------------------------------------------

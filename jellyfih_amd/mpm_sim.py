@@ -24,7 +24,7 @@ call.
 The device engine is created on first use, so importing this module never touches a GPU.
 The renderer (``mpm_sim.py:343-712``) draws into a device frame buffer read back with
 ``frame_buffer.to_numpy()``. ``JELLY_AXISYM=1`` selects the axisymmetric variant
-(``mpm_sim.py:127-133,163-165,236-304``), which runs on the engine's plain path.
+(``mpm_sim.py:127-133,163-165,236-304``), which runs on the engine's fused path.
 """
 from __future__ import annotations
 
@@ -122,12 +122,10 @@ def _device():
     global _engine, _pending_max
     if _engine is None:
         from .engine import Engine
-        # the axisymmetric variant (JELLY_AXISYM=1) runs on the plain path
         _engine = Engine(n_instances=n_instances, n_particles=n_particles, quality=quality,
                          n_grid_y=n_grid_y, domain_height=domain_height,
                          track_glow=os.environ.get("JELLY_GLOW", "0") == "1",
-                         path="plain" if axisym else "fused", axisym=axisym,
-                         nan_invalid=nan_invalid)
+                         path="fused", axisym=axisym, nan_invalid=nan_invalid)
         tuned = None if (axisym or n_grid_y != n_grid) else _tuning_for(n_instances,
                                                                          n_particles)
         if tuned:

@@ -107,11 +107,16 @@ def test_engine_fails_loudly_without_a_device():
         Engine(2, 100)
 
 
-def test_axisym_needs_the_plain_path():
-    """The axisymmetric variant is refused on the fused path before any device work."""
-    from jellyfih_amd.engine import Engine
-    with pytest.raises(_lib.JmError, match="JM_PATH_PLAIN"):
-        Engine(2, 100, axisym=True)
+def test_axisym_refused_by_the_differentiable_path():
+    """The differentiable path has no axisymmetric variant: jm_diff_create refuses it before
+    any device work (the engine runs it on both of its paths)."""
+    import ctypes
+    cfg = _lib.JmConfig()
+    _lib.call("jm_default_config", ctypes.byref(cfg))
+    cfg.axisym = 1
+    h = ctypes.c_void_p()
+    with pytest.raises(_lib.JmError, match="axisymmetric"):
+        _lib.call("jm_diff_create", ctypes.byref(cfg), ctypes.c_int64(100), 32, ctypes.byref(h))
 
 
 def test_facade_constants_match_reference():

@@ -90,3 +90,55 @@ def test_evolve_style_driver(tmp_path):
     ulps = np.abs(d["rdbu"].astype(np.float64) - rdbu) / np.spacing(
         np.maximum(np.abs(rdbu), 1e-30))
     assert ulps.max() <= 4, ulps.max()
+
+
+AXISYM_DRIVER = r"""
+import sys
+import numpy as np
+import jellyfih_amd.mpm_sim as sim
+from jellyfih_amd import phenotype
+out = sys.argv[1]
+for i in range(sim.n_instances):
+    pos, mat, fib, info = phenotype.fill_tank(phenotype.seeded_genome(i), sim.n_particles,
+                                              grid_res=128, axisym=True)
+    sim.load_particles(i, pos, mat, fib)
+for _ in range(2):
+    sim.substep()
+sim.ti.sync()
+np.savez(out, x=sim.x.to_numpy(), v=sim.v.to_numpy(), F=sim.F.to_numpy(),
+         grid_m=sim.grid_m.to_numpy(), t=sim.sim_time[None],
+         path=sim.engine().path)
+"""
+
+
+def test_axisym_facade(tmp_path):
+    """JELLY_AXISYM=1 through the facade (two axisymmetric phenotypes as evolve.py fills
+    them, make_jelly.py:276-304,343; two substep() calls, the horizon before the reference's
+    formulation overflows fp32 next to the axis): the facade runs the fused path, and x, v, F
+    are as accurate as the fp32 axisymmetric oracle against the fp64 one."""
+    from tests.cases import fp32_like
+    M, N = 2, 80000
+    out = tmp_path / "axi.npz"
+    env = dict(os.environ, JELLY_INSTANCES=str(M), JELLY_PARTICLES=str(N), JELLY_AXISYM="1")
+    print(f"[gpu-case] axisymmetric facade, {M} x {N}, 2 substeps", flush=True)
+    subprocess.run([sys.executable, "-u", "-c", AXISYM_DRIVER, str(out)], cwd=ROOT, env=env,
+                   check=True, timeout=300)
+    d = np.load(out)
+    assert str(d["path"]) == "fused"
+    orcs = [oracle.OracleSim(M, N, dtype=dt, axisym=True) for dt in (np.float32, np.float64)]
+    for i in range(M):
+        pos, mat, fib, _ = phenotype.fill_tank(phenotype.seeded_genome(i), N, grid_res=128,
+                                               axisym=True)
+        for o in orcs:
+            o.load(i, pos, mat, fib)
+    for o in orcs:
+        o.run(2, nthreads=8)
+    o32, o64 = orcs
+    assert float(d["t"]) == float(o32.t[0])
+    live = o32.material >= 0
+    for n in ("x", "v", "F"):
+        fp32_like(n, d[n][live], getattr(o32, n)[live], getattr(o64, n)[live])
+    e_gpu = np.abs(d["grid_m"] - o64.grid_m).max()
+    e_32 = np.abs(o32.grid_m - o64.grid_m).max()
+    print(f"[gpu-case] grid_m: |gpu-f64| {e_gpu:.3e}, |oracle32-f64| {e_32:.3e}", flush=True)
+    assert e_gpu <= 2 * e_32 + 1e-5 * o64.grid_m.max()
