@@ -303,6 +303,15 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
                    jm_diff** out);
 int jm_diff_destroy(jm_diff* d);
 
+/* Declares particles [water_start, n_particles) all water (material 0): jm_bwd_block then
+ * runs them with a water-only instantiation of its kernel (the other materials' SVD
+ * derivative compiled out: more waves per SIMD) and the particles before them with the
+ * general one. The caller packs the water after the body's particles (DiffTank's
+ * order="split"); a non-water particle in the range would be treated as water. Default
+ * n_particles (no water range); results are the same either way. No counterpart in the
+ * reference (which has no adjoint). */
+int jm_diff_set_water_start(jm_diff* d, int64_t water_start);
+
 /* nsteps substeps from state_in at time t0 (the clock advances by dt per substep in the
  * handle's precision; mpm_sim.py:336-340 does it in fp32). grid_out = NULL: out (P,12) is
  * the final state. Otherwise out (nsteps,P,12) receives the state after every substep and

@@ -37,7 +37,7 @@ EXPORTS = (
     "jm_write_field", "jm_live_count", "jm_read_device_order", "jm_sort_now", "jm_timed_block", "jm_sync",
     "jm_stream", "jm_last_error", "jm_water_mask", "jm_polygon_self_intersects",
     "jm_points_in_polygon_grid", "jm_generate_phenotype", "jm_fill_tank",
-    "jm_diff_create", "jm_diff_destroy",
+    "jm_diff_create", "jm_diff_destroy", "jm_diff_set_water_start",
     "jm_fwd_block", "jm_bwd_block", "jm_fwd_block_noalloc",
     "jm_render_clear", "jm_render_abyss", "jm_render_flat", "jm_render_tone_map",
     "jm_render_vorticity_grid", "jm_render_vorticity", "jm_read_frame", "jm_read_vorticity",
@@ -145,6 +145,7 @@ def load() -> ctypes.CDLL:
                          ctypes.c_void_p],
         "jm_diff_create": [P(JmConfig), ctypes.c_int64, ctypes.c_int32, P(H)],
         "jm_diff_destroy": [H],
+        "jm_diff_set_water_start": [H, ctypes.c_int64],
         "jm_fwd_block": [H, P(JmDiffInputs), ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
         "jm_fwd_block_noalloc": [P(JmConfig), ctypes.c_int64, ctypes.c_int32, P(JmDiffInputs),

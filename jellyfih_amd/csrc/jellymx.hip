@@ -2101,6 +2101,7 @@ int jm_fill_tank(const double* genome, int32_t n_genes, int64_t max_particles, i
 struct jm_diff {
   int M = 0, nx = 0, ny = 0, G = 0;
   int64_t P = 0;
+  int64_t water_start = 0;       // particles [water_start, P) are water (jm_diff_set_water_start)
   int bits = 32;                 // scalar type of the buffers: states, grids, parameters
   int math = 32;                 // scalar type of the arithmetic and the carried adjoint
   jmd::Consts<float> kf{};
@@ -2241,6 +2242,33 @@ int fwd_block(D* d, const jm_diff_inputs* in, const T* state_in, double t0_in,
   return JM_OK;
 }
 
+// One adjoint kernel of a block (template flags P2GB, G2PB): the general kernel over the
+// particles before the water range (rounded up to whole workgroups) and the water-only
+// instantiation over the rest, one after the other on the stream; the water launch, the
+// larger, clears the zero span.
+template <typename T, typename A, bool P2GB, bool G2PB, typename D>
+void launch_bwd(const D* d, hipStream_t st, const jmd::DiffInputs<T>& din, const T* st_cur,
+                const T* st_prev, const A* aF_in, const A* m_in, const A* ag_cur,
+                const T* g_prev, A* ag_prev, const T* a_full_in, A* m_out, A* aF_out,
+                T* a_full_out, A* zero, int64_t zero_n, A t, const jmd::Consts<A>& k,
+                T* gamp, double* gparams) {
+  const int P = (int)d->P, G = d->G;
+  const int64_t ws = std::min<int64_t>(d->water_start, d->P);
+  const int W = (int)std::min<int64_t>(P, (ws + jmd::kDiffBlock - 1) / jmd::kDiffBlock *
+                                              jmd::kDiffBlock);
+  const dim3 tb(jmd::kDiffBlock);
+  if (W > 0)
+    hipLaunchKernelGGL((jmd::k_d_bwd<T, A, P2GB, G2PB, false>), dim3(blocks_for(W, jmd::kDiffBlock)),
+                       tb, 0, st, din, st_cur, st_prev, aF_in, 4, m_in, ag_cur, g_prev, ag_prev,
+                       a_full_in, m_out, aF_out, a_full_out, W < P ? (A*)nullptr : zero, zero_n,
+                       0, W, G, t, k, gamp, gparams);
+  if (W < P)
+    hipLaunchKernelGGL((jmd::k_d_bwd<T, A, P2GB, G2PB, true>),
+                       dim3(blocks_for(P - W, jmd::kDiffBlock)), tb, 0, st, din, st_cur, st_prev,
+                       aF_in, 4, m_in, ag_cur, g_prev, ag_prev, a_full_in, m_out, aF_out,
+                       a_full_out, zero, zero_n, W, P, G, t, k, gamp, gparams);
+}
+
 template <typename T, typename A, typename D>
 int bwd_block(D* d, const jm_diff_inputs* in, const T* traj, const T* grids,
               double t0_in, int nsteps, const T* grad_out, T* grad_in, T* grad_amp,
@@ -2248,10 +2276,9 @@ int bwd_block(D* d, const jm_diff_inputs* in, const T* traj, const T* grids,
   const jmd::DiffInputs<T> din = diff_inputs<T>(in);
   const jmd::Consts<A>& k = consts_of<A>(d);
   const T dt = consts_of<T>(d).dt;
-  const int P = (int)d->P, G = d->G, n = nsteps;
+  const int P = (int)d->P, n = nsteps;
   const size_t S = (size_t)P * 12;
-  const int64_t MG = (int64_t)d->M * 3 * G;
-  const dim3 pg(blocks_for(P, jmd::kDiffBlock)), tb(jmd::kDiffBlock);
+  const int64_t MG = (int64_t)d->M * 3 * d->G;
   A* ring = static_cast<A*>(d->agrid);
   A* pb = reinterpret_cast<A*>(static_cast<char*>(d->pbuf) + 8 * (size_t)P * sizeof(T));
   A* mb[2] = {pb, pb + 8 * (size_t)P};                        // m_s, by s % 2
@@ -2263,28 +2290,27 @@ int bwd_block(D* d, const jm_diff_inputs* in, const T* traj, const T* grids,
   hipLaunchKernelGGL(jmd::k_d_fill<A>, dim3(blocks_for(MG, 256 * 8)), dim3(256), 0, st,
                      agrid(n - 1), MG);
   // K_b(n): G2P adjoint of substep n-1, driven by grad_out
-  hipLaunchKernelGGL((jmd::k_d_bwd<T, A, false, true>), pg, tb, 0, st, din, (const T*)nullptr,
-                     traj + (size_t)(n - 1) * S, (const A*)nullptr, 4, (const A*)nullptr,
-                     (const A*)nullptr, grids + (size_t)(n - 1) * MG, agrid(n - 1), grad_out,
-                     mb[(n - 1) % 2], aF[n % 2], (T*)nullptr,
-                     n >= 2 ? agrid(n + 1) : (A*)nullptr, MG, P, G, A(times[n]), k, grad_amp,
-                     grad_params);
+  launch_bwd<T, A, false, true>(d, st, din, (const T*)nullptr, traj + (size_t)(n - 1) * S,
+                                (const A*)nullptr, (const A*)nullptr, (const A*)nullptr,
+                                grids + (size_t)(n - 1) * MG, agrid(n - 1), grad_out,
+                                mb[(n - 1) % 2], aF[n % 2], (T*)nullptr,
+                                n >= 2 ? agrid(n + 1) : (A*)nullptr, MG, A(times[n]), k,
+                                grad_amp, grad_params);
   for (int s = n - 1; s >= 1; --s) {
     // K_b(s): P2G adjoint of substep s, G2P adjoint of substep s-1
-    hipLaunchKernelGGL((jmd::k_d_bwd<T, A, true, true>), pg, tb, 0, st, din,
-                       traj + (size_t)s * S, traj + (size_t)(s - 1) * S,
-                       (const A*)aF[(s + 1) % 2], 4, (const A*)mb[s % 2], (const A*)agrid(s),
-                       grids + (size_t)(s - 1) * MG, agrid(s - 1), (const T*)nullptr,
-                       mb[(s - 1) % 2], aF[s % 2], (T*)nullptr,
-                       s >= 2 ? agrid(s + 1) : (A*)nullptr, MG, P, G, A(times[s]), k,
-                       grad_amp, grad_params);
+    launch_bwd<T, A, true, true>(d, st, din, traj + (size_t)s * S, traj + (size_t)(s - 1) * S,
+                                 (const A*)aF[(s + 1) % 2], (const A*)mb[s % 2],
+                                 (const A*)agrid(s), grids + (size_t)(s - 1) * MG,
+                                 agrid(s - 1), (const T*)nullptr, mb[(s - 1) % 2], aF[s % 2],
+                                 (T*)nullptr, s >= 2 ? agrid(s + 1) : (A*)nullptr, MG,
+                                 A(times[s]), k, grad_amp, grad_params);
   }
   // K_b(0): P2G adjoint of substep 0, writing grad_in
-  hipLaunchKernelGGL((jmd::k_d_bwd<T, A, true, false>), pg, tb, 0, st, din, traj,
-                     (const T*)nullptr, (const A*)aF[1], 4, (const A*)mb[0],
-                     (const A*)agrid(0), (const T*)nullptr, (A*)nullptr, (const T*)nullptr,
-                     (A*)nullptr, (A*)nullptr, grad_in, (A*)nullptr, MG, P, G, A(times[0]), k,
-                     grad_amp, grad_params);
+  launch_bwd<T, A, true, false>(d, st, din, traj, (const T*)nullptr, (const A*)aF[1],
+                                (const A*)mb[0], (const A*)agrid(0), (const T*)nullptr,
+                                (A*)nullptr, (const T*)nullptr, (A*)nullptr, (A*)nullptr,
+                                grad_in, (A*)nullptr, MG, A(times[0]), k, grad_amp,
+                                grad_params);
   HIP_TRY(hipGetLastError());
   return JM_OK;
 }
@@ -2356,6 +2382,7 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
   d->ny = ny;
   d->G = nx * ny;
   d->P = n_particles;
+  d->water_start = n_particles;
   d->bits = bits;
   d->math = math;
   d->kf = diff_consts<float>(*cfg, nx, ny);
@@ -2376,6 +2403,14 @@ int jm_diff_create(const jm_config* cfg, int64_t n_particles, int32_t scalar_bit
 
 int jm_diff_destroy(jm_diff* d) {
   delete d;
+  return JM_OK;
+}
+
+int jm_diff_set_water_start(jm_diff* d, int64_t water_start) {
+  if (!d) return fail(JM_ERR_ARG, "null diff handle");
+  if (water_start < 0 || water_start > d->P)
+    return fail(JM_ERR_ARG, "water_start must be in [0, n_particles]");
+  d->water_start = water_start;
   return JM_OK;
 }
 

@@ -76,19 +76,25 @@ def clock(t0: float, n: int, dt=None, fp64: bool = False) -> float:
 def packing_order(pos, mat, order: str, nx: int, ny: int):
     """(instance, slot) of every live particle (``mat >= 0``) in packing order: instance
     by instance, and within an instance by slot (``order="slot"``) or by grid cell in
-    snaking strips of 8 cell rows, ties by slot (``order="cell"``)."""
+    snaking strips of 8 cell rows, ties by slot (``order="cell"``); or (``order="split"``)
+    the body's particles (material > 0) of every instance first and then the water of every
+    instance, each instance by instance in that cell order."""
     N = mat.shape[1]
     inst, slot = np.divmod(np.flatnonzero(mat >= 0), N)
     if order == "slot":
         return inst, slot
-    if order != "cell":
-        raise ValueError("order must be 'cell' or 'slot'")
+    if order not in ("cell", "split"):
+        raise ValueError("order must be 'cell', 'split' or 'slot'")
     x = np.asarray(pos, np.float32)[inst, slot]
     b = np.trunc(x * np.float32(nx) - np.float32(0.5)).astype(np.int64)
     bx, by = np.clip(b[:, 0], 0, nx - 1), np.clip(b[:, 1], 0, ny - 1)
     s, r = by // 8, by % 8
     key = (s * nx + np.where(s % 2 == 1, nx - 1 - bx, bx)) * 8 + r
-    perm = np.lexsort((slot, key, inst))
+    if order == "split":
+        water = (mat[inst, slot] == 0).astype(np.int64)
+        perm = np.lexsort((slot, key, inst, water))
+    else:
+        perm = np.lexsort((slot, key, inst))
     return inst[perm], slot[perm]
 
 
@@ -105,12 +111,17 @@ class DiffTank:
     the small strains near F = I (the jelly's F - R, the water's J - 1 at stiffness 1e5) to
     a few digits only, which moves gradients by percents within a few hundred substeps; fp32
     storage of the results does not (the fp64 arithmetic over fp32 buffers agrees with fp64
-    finite differences to ~1e-5 over 200 substeps)."""
+    finite differences to ~1e-5 over 200 substeps).
+
+    ``water_start``: when the particles from that index on are all water (as
+    ``from_tanks(order="split")`` packs them), the adjoint runs them with a water-only
+    kernel, which holds more waves per SIMD; None (default) runs every particle with the
+    general kernel. Results are the same."""
 
     def __init__(self, inst, mat, fiber, n_instances: int, quality: int = 1,
                  n_grid_y: int | None = None, domain_height: float = 1.0,
                  device: str | torch.device = "cuda", dtype: torch.dtype = torch.float32,
-                 math: str = "fp64"):
+                 math: str = "fp64", water_start: int | None = None):
         if dtype not in (torch.float32, torch.float64):
             raise ValueError("dtype must be torch.float32 or torch.float64")
         if math not in ("fp32", "fp64") or (dtype == torch.float64 and math != "fp64"):
@@ -166,9 +177,17 @@ class DiffTank:
         self._h = h
         self._L = load()
         self._owner = None      # a view (see _view) keeps the tank that owns the handle
+        self.water_start = self.P
+        if water_start is not None:
+            ws = int(water_start)
+            if not 0 <= ws <= self.P or np.any(mat_np[ws:] != 0):
+                self.close()
+                raise ValueError("water_start: particles from it on must all be water")
+            call("jm_diff_set_water_start", self._h, ctypes.c_int64(ws))
+            self.water_start = ws
 
     @classmethod
-    def from_tanks(cls, pos, mat, fiber, order: str = "cell", **kw):
+    def from_tanks(cls, pos, mat, fiber, order: str = "auto", **kw):
         """Pack reference-layout tanks (``pos (M,N,2)``, ``mat (M,N)`` with -1 for dead
         slots, ``fiber (M,N,2)``) into a DiffTank and its initial state (v = 0, C = 0,
         F = I, as ``load_particles`` sets them, ``mpm_sim.py:715-736``).
@@ -177,8 +196,16 @@ class DiffTank:
         snaking strips of 8 cell rows (the fused engine's sort order): a workgroup's
         particles then touch a compact window of nodes and a cell's particles sit in
         adjacent lanes, which the kernels' LDS tiles and cross-lane pre-sums exploit.
-        ``order="slot"`` keeps slot order. ``tank.slots`` (P, 2) gives the (instance,
-        slot) of every packed particle either way."""
+        ``order="split"`` packs every instance's body particles that way first and then
+        every instance's water, and sets ``water_start`` to the first water particle, so
+        the adjoint runs the water with its water-only kernel. ``order="slot"`` keeps slot
+        order. ``order="auto"`` (default) is "split" with fp64 arithmetic, where the general
+        adjoint kernel holds one wave per SIMD and the water-only one two (the 32-tank tall
+        batch's forward + backward with fp32 buffers: 3.98 -> 3.11 s), and "cell" with fp32
+        arithmetic, where the split does not pay (2.02 against 2.07 s). ``tank.slots``
+        (P, 2) gives the (instance, slot) of every packed particle either way."""
+        if order == "auto":
+            order = "split" if kw.get("math", "fp64") == "fp64" else "cell"
         pos = np.asarray(pos, np.float32)
         mat = np.asarray(mat, np.int32)
         fiber = np.asarray(fiber, np.float32)
@@ -186,8 +213,10 @@ class DiffTank:
         M = mat.shape[0]
         nx = 128 * int(kw.get("quality", 1))
         inst_all, slot_all = packing_order(pos, mat, order, nx, int(kw.get("n_grid_y") or nx))
-        tank = cls(inst_all.astype(np.int32), mat[inst_all, slot_all],
-                   fiber[inst_all, slot_all], M, **kw)
+        mat_all = mat[inst_all, slot_all]
+        if order == "split":
+            kw["water_start"] = int(np.count_nonzero(mat_all != 0))
+        tank = cls(inst_all.astype(np.int32), mat_all, fiber[inst_all, slot_all], M, **kw)
         tank.slots = np.stack([inst_all, slot_all], 1)
         state = np.zeros((inst_all.size, STATE_COLS), np.float32)
         state[:, 0:2] = pos[inst_all, slot_all]
@@ -222,8 +251,10 @@ class DiffTank:
     def cell_permutation(self, state: torch.Tensor) -> torch.Tensor:
         """The permutation (P,) that orders the particles of each instance by grid cell at
         their current positions, in the ``order="cell"`` packing order (snaking strips of 8
-        cell rows), ties by current position in the array; instances keep their blocks."""
-        if self._order is not None:
+        cell rows), ties by current position in the array; instances keep their blocks (and,
+        packed with ``order="split"``, the body and the water keep theirs)."""
+        split = self.water_start < self.P
+        if self._order is not None and not split:
             raise ValueError("re-sorting needs particles packed instance by instance")
         x = state[:, :2].detach().to(torch.float32)
         b = torch.trunc(x * float(self.nx) - 0.5).nan_to_num(0.0).long()
@@ -232,7 +263,10 @@ class DiffTank:
         s, r = by // 8, by % 8
         key = (s * self.nx + torch.where(s % 2 == 1, self.nx - 1 - bx, bx)) * 8 + r
         span = ((self.ny + 7) // 8) * 8 * self.nx
-        return torch.argsort(self.inst.long() * span + key, stable=True)
+        group = self.inst.long()
+        if split:
+            group = group + self.M * (self.mat == 0).long()
+        return torch.argsort(group * span + key, stable=True)
 
     def _view(self, perm: torch.Tensor) -> "DiffTank":
         """This tank with its particles in the order ``perm`` (a within-instance permutation

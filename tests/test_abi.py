@@ -210,3 +210,10 @@ def test_facade_queues_substeps_until_the_next_access(monkeypatch):
                              ("set_param", "actuation"), ("substep", 7), ("get_time",),
                              ("sync",), ("sync",)], rec.calls
     assert s._pending == 0
+
+
+def test_water_start_rejects_a_null_handle():
+    """jm_diff_set_water_start checks its handle before anything else (its range check
+    needs a handle, which needs a device: tests/test_gpu_diff.py)."""
+    L = _lib.load()
+    assert L.jm_diff_set_water_start(None, 0) == _lib.JM_ERR_ARG

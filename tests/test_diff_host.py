@@ -66,3 +66,27 @@ def test_packing_order_matches_engine_sort_order():
         expect = live[oracle.stable_cell_sort(c, pos[m, live], 8)]
         np.testing.assert_array_equal(slot[inst == m], expect)
     assert np.all(np.diff(inst) >= 0)
+
+
+def test_packing_order_split_puts_the_water_last():
+    """``order="split"``: every instance's body particles (material > 0) first, then every
+    instance's water; within each part, instance by instance in the ``order="cell"``
+    order of that instance's particles of the part."""
+    import oracle
+    from jellyfih_amd.diff import packing_order
+    rng = np.random.default_rng(4)
+    M, N = 3, 600
+    pos = rng.uniform(0.0, 1.0, size=(M, N, 2)).astype(np.float32)
+    mat = rng.choice([-1, 0, 0, 0, 1, 2, 3], size=(M, N)).astype(np.int32)
+    inst, slot = packing_order(pos, mat, "split", 128, 128)
+    m_all = mat[inst, slot]
+    assert inst.size == int((mat >= 0).sum())
+    nb = int((mat > 0).sum())
+    assert np.all(m_all[:nb] > 0) and np.all(m_all[nb:] == 0)
+    c = oracle.reference_consts()
+    for part, sel in ((slice(0, nb), lambda mm: mm > 0), (slice(nb, None), lambda mm: mm == 0)):
+        assert np.all(np.diff(inst[part]) >= 0)
+        for m in range(M):
+            live = np.flatnonzero(sel(mat[m]))
+            expect = live[oracle.stable_cell_sort(c, pos[m, live], 8)]
+            np.testing.assert_array_equal(slot[part][inst[part] == m], expect)
