@@ -199,15 +199,20 @@ class DiffTank:
         ``order="split"`` packs every instance's body particles that way first and then
         every instance's water, and sets ``water_start`` to the first water particle, so
         the adjoint runs the water with its water-only kernel. ``order="slot"`` keeps slot
-        order. ``order="auto"`` (default) is "split" with fp64 arithmetic, where the general
-        adjoint kernel holds one wave per SIMD and the water-only one two (the 32-tank tall
-        batch's forward + backward with fp32 buffers: 3.98 -> 3.11 s), and "cell" with fp32
-        arithmetic, where the split does not pay (2.02 against 2.07 s). ``tank.slots``
-        (P, 2) gives the (instance, slot) of every packed particle either way."""
-        if order == "auto":
-            order = "split" if kw.get("math", "fp64") == "fp64" else "cell"
+        order. ``order="auto"`` (default) is "split" for batches of at least 2^20 live
+        particles with fp64 arithmetic, where the general adjoint kernel holds one wave per
+        SIMD and the water-only one two (the 32-tank tall batch's forward + backward with
+        fp32 buffers: 3.98 -> 3.11 s), and "cell" otherwise: with fp32 arithmetic the split
+        does not pay (2.02 against 2.07 s there), and a batch that leaves the chip half
+        empty (one reference tank is 242 workgroups) waits for its slowest workgroup, which
+        the split makes one full of the body's SVD work (one tank, fp32 buffers with fp64
+        arithmetic: forward 20.4 -> 33.6 ms). ``tank.slots`` (P, 2) gives the (instance,
+        slot) of every packed particle either way."""
         pos = np.asarray(pos, np.float32)
         mat = np.asarray(mat, np.int32)
+        if order == "auto":
+            big = int(np.count_nonzero(mat >= 0)) >= (1 << 20)
+            order = "split" if big and kw.get("math", "fp64") == "fp64" else "cell"
         fiber = np.asarray(fiber, np.float32)
         dtype = kw.get("dtype", torch.float32)
         M = mat.shape[0]
