@@ -497,15 +497,19 @@ class _Episode:
 
     def take(self, b: int):
         """(traj, grids, buffers) of block b: the recomputation start(b) launched, ordered
-        before the current stream's next work, or else run now on the current stream."""
+        before the current stream's next work, or else run now on the current stream; None
+        once block b has been taken (a second backward pass through a retained graph
+        recomputes from the block's own saved inputs)."""
         got = self.pending.pop(b, None)
         if got is not None:
             traj, grids, done, bufs = got
             torch.cuda.current_stream(traj.device).wait_event(done)
-        else:
+        elif self.blocks[b] is not None:
             tank, state, amp_c, params, t0, n = self.blocks[b]
             bufs = self._buffers(tank)
             traj, grids = tank.trajectory(state, amp_c, params, t0, n, bufs=bufs)
+        else:
+            return None
         self.blocks[b] = None     # its inputs are no longer needed here
         return traj, grids, bufs
 
@@ -543,8 +547,10 @@ class _Block(torch.autograd.Function):
         state, amp_c, params = ctx.saved_tensors
         stored, ep, bufs = ctx.stored, ctx.episode, None
         if stored is None and ep is not None:
-            traj, grids, bufs = ep.take(ctx.index)
-            stored = (traj, grids)
+            got = ep.take(ctx.index)
+            if got is not None:
+                traj, grids, bufs = got
+                stored = (traj, grids)
             ep.start(ctx.index - 1)    # the previous block's recomputation, beside this adjoint
         gin, gamp, gpar = ctx.tank.backward_raw(state, amp_c, params, ctx.t0, ctx.n, grad_out,
                                                 stored=stored)

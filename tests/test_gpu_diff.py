@@ -399,3 +399,26 @@ def test_split_packing_gives_the_same_gradients(require_gpu, dtype, math):
         say(f"{name} vs cell: max rel {err:.2e} over CoM and 9 gradients "
             f"(cell repeated: {floor:.2e}); bound {bound:.1e}")
         assert err <= bound, (name, err)
+
+
+def test_second_backward_through_a_retained_graph(require_gpu):
+    """A second backward pass through the same graph (``retain_graph=True``) recomputes every
+    block again (the first pass consumed the pipelined recomputations) and adds the same
+    gradients once more: fp64, one reference tank, 300 substeps, checkpoint 100."""
+    import torch
+    say("reference tank seed0, 300 substeps, fp64: backward twice through a retained graph")
+    cs, _, _, _ = reference_tank_case(300)
+    tank, st, amp, par = _gpu(cs, "float64", "fp64")
+    st.requires_grad_(True)
+    leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2)]
+    out = tank.run(st, amp, *leaves, E_jelly=par[:, 4], payload_density=par[:, 3], t0=cs.t0,
+                   nsteps=cs.steps, checkpoint=100)
+    y = tank.payload_com_y(out).sum()
+    y.backward(retain_graph=True)
+    first = [t.grad.clone() for t in [st] + leaves]
+    y.backward()
+    for name, g1, t in zip(("state", "amp", "contraction", "freq"), first, [st] + leaves):
+        err = float((t.grad - 2 * g1).norm() / (2 * g1).norm())
+        say(f"{name}: |g(2 passes) - 2 g(1 pass)| / |2 g| = {err:.2e}")
+        assert err <= 1e-9, name
+    tank.close()
