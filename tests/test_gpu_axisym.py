@@ -188,3 +188,29 @@ def test_axisym_5_substeps(case, path):
                 f"|oracle32-f64| {e_32:.3e}")
             assert e_gpu <= 2.0 * e_32 + 1e-6
     eng.close()
+
+
+def test_axisym_fused_launch_shapes():
+    """The fused axisymmetric kernel is built for 256 x 2 and 256 x 4 chunks:
+    jm_set_tuning refuses other shapes on such an engine, and 256 x 2 with one instance group
+    gives the 256 x 4 two-group run's state after 5 substeps (fp32-like against the oracle)."""
+    from jellyfih_amd import _lib
+    say("axisym fused: launch shapes 256x2 / 256x4, refused 256x8")
+    cases = axis_cases()
+    runs = {}
+    for ppt, groups in ((4, 2), (2, 1)):
+        eng, (o32, o64) = make_pair(cases, dtypes=(np.float32, np.float64), path="fused")
+        if ppt == 4:
+            with pytest.raises(_lib.JmError, match="256 x 2 or 256 x 4"):
+                eng.set_tuning(ppt=8)
+        eng.set_tuning(threads=256, ppt=ppt, instance_groups=groups)
+        eng.substep(5)
+        runs[ppt] = {n: eng.read(n) for n in ("x", "v", "F", "C")}
+        eng.close()
+    o32.run(5, nthreads=8)
+    o64.run(5, nthreads=8)
+    live = o32.material >= 0
+    for ppt, r in runs.items():
+        for n in ("x", "v", "F"):
+            fp32_like(f"{n} (256 x {ppt})", r[n][live], getattr(o32, n)[live],
+                      getattr(o64, n)[live])
