@@ -217,8 +217,10 @@ def _golden(steps: int):
 # fp32 rounding of the stored states, compounded, moves the linearisation point: 5e-3
 # (measured 1.0e-3, as the host build of the same scheme; the multiplier direction, whose
 # differences come from the fp64 host build, 1.4e-2 against AMP_SCALE_BOUND_2000). fp32
-# arithmetic (the reference's): None = sign and magnitude only (ratio in [0.5, 2]); its
-# sequential host build is off by up to 4% at 200 substeps and 5% at 2000.
+# arithmetic (the reference's): None = each direction within a factor 2 of FD, and the
+# worst direction within twice the worst of the sequential fp32 host build of the same
+# algorithm (4.3e-2 at 200 substeps, 5.2e-2 at 2000, in the fixtures) plus 1e-3: fp32
+# rounding of the small strains, not the adjoint, sets that error (measured 4.3e-2, 4.7e-2).
 GRAD_BOUNDS = {(200, "float64", "fp64"): 1e-4, (200, "float32", "fp64"): 1e-3,
                (200, "float32", "fp32"): None, (2000, "float64", "fp64"): 1e-4,
                (2000, "float32", "fp64"): 5e-3, (2000, "float32", "fp32"): None}
@@ -259,6 +261,8 @@ def test_reference_tank_gradients_vs_fd(require_gpu, steps, dtype, math):
         else:
             assert rel <= max(bound, 2 * r64), (kind, rel, bound)
     say(f"worst |AD-FD| {worst:.2e} (bound {bound}); sequential fp32 build: {worst32:.2e}")
+    if bound is None:
+        assert worst <= 2 * worst32 + 1e-3, (worst, worst32)
 
 
 def test_gradients_independent_of_checkpoint_interval(require_gpu):
