@@ -214,3 +214,29 @@ def test_axisym_fused_launch_shapes():
         for n in ("x", "v", "F"):
             fp32_like(f"{n} (256 x {ppt})", r[n][live], getattr(o32, n)[live],
                       getattr(o64, n)[live])
+
+
+def test_axisym_fused_headless_through_the_divergence():
+    """evolve.py's headless run under JELLY_AXISYM=1 on the fused path (32 axis-spawned
+    tanks, 80K slots, 20,000 substeps through run_batch_headless): the reference's
+    formulation throws nearly every particle against the walls within ~10 substeps, with
+    velocities that overflow fp32; the run still completes, every position stays inside
+    the domain (the advection clamp, mpm_sim.py:331-333), and the clock is the fp32
+    accumulation of dt."""
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.diff import clock
+    say("axisym fused, 32 axis-spawned tanks x 80K, run_batch_headless(20000)")
+    pos, mat, fib, genomes, infos = phenotype.population(range(32), 80000, 128, axisym=True)
+    eng = Engine(32, 80000, path="fused", axisym=True)
+    for m in range(32):
+        eng.load_particles(m, pos[m], mat[m], fib[m])
+    res = eng.run_batch_headless(20000)
+    x = eng.read("x")
+    live = mat >= 0
+    say(f"results finite rows {int(np.isfinite(res).all(1).sum())}/32, valid "
+        f"{int(res[:, 4].sum())}, x in [{x[live].min():.4f}, {x[live].max():.4f}]")
+    assert res.shape == (32, 5)
+    assert np.isfinite(x[live]).all()
+    assert x[live].min() >= 0.001 - 1e-7 and x[live].max() <= 0.999 + 1e-7
+    assert eng.get_time() == np.float32(clock(0.0, 20000))
+    eng.close()
