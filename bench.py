@@ -86,13 +86,13 @@ def load_engine(eng, pos, mat, fib, genomes):
     eng.set_param("freq", np.clip(genomes[:, 10], 0.5, 2.0))           # evolve.py:152
 
 
-def load_tuning(eng) -> dict:
-    path = os.path.join(ROOT, "jellyfih_amd", "tuning.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            t = json.load(f)
-        eng.set_tuning(**{k: v for k, v in t.get("tuning", t).items()
-                          if k in eng.tuning()})
+def load_tuning(eng, instances: int, slots: int) -> dict:
+    """The autotuner's launch parameters for this batch shape (jellyfih_amd/tuning.json),
+    when it holds an entry for it; else the engine's own choice."""
+    from jellyfih_amd.autotune import tuning_for
+    t = tuning_for(instances, slots)
+    if t:
+        eng.set_tuning(**{k: v for k, v in t.items() if k in eng.tuning()})
     return eng.tuning()
 
 
@@ -547,6 +547,7 @@ def batch16(reps: int = 3) -> dict:
     pos, mat, fib, genomes, _ = build_workload(16, 80000, range(1, 17))
     live = int((mat >= 0).sum())
     eng = Engine(16, 80000)
+    tuning = load_tuning(eng, 16, 80000)
     load_engine(eng, pos, mat, fib, genomes)
     eng.substep(MIN_WARM)
     eng.sync()
@@ -559,7 +560,7 @@ def batch16(reps: int = 3) -> dict:
     eng.close()
     wall = float(np.median(walls))
     return {"instances": 16, "seeds": [1, 16], "live_particles": live, "substeps": 20000,
-            "seconds": wall, "value": live * 20000 / wall}
+            "seconds": wall, "value": live * 20000 / wall, "tuning": tuning}
 
 
 def main() -> int:
@@ -645,7 +646,7 @@ def main() -> int:
 
     eng = Engine(M, N)
     load_engine(eng, pos, mat, fib, genomes)
-    tuning = load_tuning(eng)
+    tuning = load_tuning(eng, M, N)
     if args.only_roofline:
         # The warm-up runs with another launch shape (k_fused<256, 8, ...>) and the tuned
         # shape returns for the timed blocks, so the only k_fused<256,4,true,true> dispatches

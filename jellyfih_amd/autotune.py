@@ -5,7 +5,8 @@ concurrently launched instance groups, and the XCD-affinity / lazy-v,C switches.
 
 It measures each candidate on the caller's own workload (an Engine already loaded with it)
 with HIP-event-timed blocks, keeps the fastest, and can write it to ``tuning.json`` beside
-this file, which ``bench.py`` and the facade load.
+this file, which ``bench.py`` and the facade load: one entry per batch shape (instances,
+slots per instance), looked up by ``tuning_for``.
 
 Usage: python -m jellyfih_amd.autotune [--instances 32] [--slots 80000] [--write]
 """
@@ -27,6 +28,41 @@ SHAPES = [(256, 4), (256, 8), (512, 4), (512, 8), (128, 8), (128, 16), (256, 16)
 SORT_INTERVALS = [50, 100, 200, 500, 1000]
 SORT_STRIPS = [0, 4, 8, 16]
 BLOCKS = [50, 100, 200]
+
+
+def load_entries(path: str | None = None) -> list:
+    """The entries of a tuning file: ``{"entries": [...]}``, or a single entry at the top
+    level (the file's form when it held one batch shape); [] when it is missing."""
+    try:
+        with open(path or TUNING_PATH) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return []
+    if isinstance(d, dict) and "entries" in d:
+        return list(d["entries"])
+    return [d] if isinstance(d, dict) and "tuning" in d else []
+
+
+def _shape(entry: dict):
+    w = entry.get("workload", {})
+    return w.get("instances"), w.get("slots")
+
+
+def tuning_for(instances: int, slots: int, path: str | None = None):
+    """The launch parameters measured on a batch of ``instances`` x ``slots``, or None."""
+    for e in load_entries(path):
+        if _shape(e) == (instances, slots):
+            return dict(e.get("tuning", {})) or None
+    return None
+
+
+def write_entry(res: dict, path: str | None = None) -> None:
+    """Adds (or replaces) the entry of ``res["workload"]``'s batch shape."""
+    entries = [e for e in load_entries(path) if _shape(e) != _shape(res)]
+    entries.append(res)
+    entries.sort(key=lambda e: tuple(v or 0 for v in _shape(e)))
+    with open(path or TUNING_PATH, "w") as f:
+        json.dump({"entries": entries}, f, indent=1)
 
 
 def _rate(eng, live: int, substeps: int, reps: int = 3) -> float:
@@ -100,8 +136,7 @@ def main() -> int:
     res["workload"] = {"instances": args.instances, "slots": args.slots, "live": live}
     print(json.dumps(res))
     if args.write:
-        with open(TUNING_PATH, "w") as f:
-            json.dump(res, f, indent=1)
+        write_entry(res)
     return 0
 
 

@@ -89,17 +89,8 @@ def _tuning_for(instances: int, slots: int, path: str | None = None):
     """The autotuner's launch parameters (``tuning.json`` beside this module, written by
     ``python -m jellyfih_amd.autotune --write``) when they were measured on this batch
     shape, else None (the engine then picks its launch shape by batch size)."""
-    import json
-    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-    except (OSError, ValueError):
-        return None
-    w = d.get("workload", {})
-    if w.get("instances") != instances or w.get("slots") != slots:
-        return None
-    return dict(d.get("tuning", {})) or None
+    from .autotune import tuning_for
+    return tuning_for(instances, slots, path)
 
 
 def engine():

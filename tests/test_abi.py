@@ -149,11 +149,28 @@ def test_facade_ti_sync_without_an_engine():
 
 
 def test_facade_uses_the_autotuned_parameters_for_their_batch_shape(tmp_path):
+    """The facade takes the tuning file's entry for its own batch shape only; the file holds
+    one entry per shape (autotune.write_entry replaces the same shape's entry), and the
+    single-entry form it had before still reads."""
+    import json
+    from jellyfih_amd import autotune
     from jellyfih_amd import mpm_sim as s
     t = s._tuning_for(32, 80000)
     assert t is not None and t["threads"] * t["ppt"] > 0 and t["sort_interval"] >= 1
-    assert s._tuning_for(16, 80000) is None and s._tuning_for(32, 160000) is None
+    assert s._tuning_for(32, 160000) is None
     assert s._tuning_for(32, 80000, path=str(tmp_path / "missing.json")) is None
+    path = str(tmp_path / "tuning.json")
+    entry = lambda m, n, ppt: {"tuning": {"threads": 256, "ppt": ppt}, "rate": 1.0,
+                               "workload": {"instances": m, "slots": n, "live": 1}}
+    with open(path, "w") as f:
+        json.dump(entry(32, 80000, 4), f)                      # the single-entry form
+    assert s._tuning_for(32, 80000, path=path)["ppt"] == 4
+    autotune.write_entry(entry(16, 80000, 2), path)
+    autotune.write_entry(entry(32, 80000, 8), path)             # replaces the 32 x 80K entry
+    assert len(autotune.load_entries(path)) == 2
+    assert s._tuning_for(16, 80000, path=path)["ppt"] == 2
+    assert s._tuning_for(32, 80000, path=path)["ppt"] == 8
+    assert s._tuning_for(8, 80000, path=path) is None
 
 
 def test_facade_queues_substeps_until_the_next_access(monkeypatch):

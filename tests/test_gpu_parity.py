@@ -290,16 +290,14 @@ def test_generation_batch_with_bench_tuning(require_gpu):
     substeps through run_batch_headless and the fitness of evolve.py:90-124: validity flags
     equal, payload CoM and x within what the fp32 oracle shows against the fp64 oracle
     (2x, + 1e-6), fitness within 1e-4 of the fp64 oracle's."""
-    import json
-    import os
     from jellyfih_amd import phenotype
+    from jellyfih_amd.autotune import tuning_for
     say("32-genome generation batch with the bench tuning, 600 substeps")
     pos, mat, fib, genomes, infos = phenotype.population(range(32), 80000, 128)
     cases = [(pos[m], mat[m], fib[m]) for m in range(32)]
     eng, o32 = make_pair(cases)
-    with open(os.path.join(os.path.dirname(oracle.__file__), "..", "jellyfih_amd",
-                           "tuning.json")) as f:
-        tuning = json.load(f)["tuning"]
+    tuning = tuning_for(32, 80000)
+    assert tuning is not None
     eng.set_tuning(**{k: v for k, v in tuning.items() if k in eng.tuning()})
     say(f"tuning {eng.tuning()}")
     o64 = oracle.OracleSim(32, 80000, dtype=np.float64)
