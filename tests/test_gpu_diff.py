@@ -230,7 +230,8 @@ AMP_SCALE_BOUND_2000 = 3e-2
 @pytest.mark.parametrize("steps", [200, 2000])
 @pytest.mark.parametrize("dtype,math", PRECISIONS)
 def test_reference_tank_gradients_vs_fd(require_gpu, steps, dtype, math):
-    """One seeded reference tank, 200 or 2000 substeps, checkpoint every 100 (cfg4 shape):
+    """One seeded reference tank, 200 or 2000 substeps, checkpoint every 100 (the
+    differentiable single-instance workload):
     directional derivatives of the payload CoM y along 8 seeded directions and the
     per-particle multiplier direction against fp64 central differences."""
     say(f"reference tank seed0, {steps} substeps, checkpoint 100, {dtype}/{math}: "

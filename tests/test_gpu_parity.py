@@ -180,7 +180,8 @@ def test_unsorted_chunks_grid(require_gpu):
 
 
 def test_reference_tank_1000_substeps(require_gpu):
-    """cfg1: one seeded reference tank (~62K live of 80K slots), 1000 substeps."""
+    """The single-instance workload: one seeded reference tank (~62K live of 80K slots),
+    1000 substeps."""
     say("reference tank seed0, 1000 substeps, fused vs oracle f32/f64")
     pos, mat, fib, g, info = reference_tank(0)
     eng = Engine(1, 80000)
