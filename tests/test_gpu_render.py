@@ -34,9 +34,9 @@ def say(msg: str) -> None:
     sys.stdout.flush()
 
 
-def _engine(M=5, N=12000, substeps=300, path="fused", H=1.0, seed=0):
+def _engine(M=5, N=12000, substeps=300, path="fused", H=1.0, seed=0, axisym=False):
     ny = int(128 * H)
-    eng = Engine(M, N, n_grid_y=ny, domain_height=H, path=path)
+    eng = Engine(M, N, n_grid_y=ny, domain_height=H, path=path, axisym=axisym)
     for m in range(M):
         region = (0.3, 0.7, 0.25 * H, 0.25 * H + 0.4)
         pos, mat, fib = small_tank(seed + m, N, region=region)
@@ -148,6 +148,38 @@ def test_vorticity_grid(path):
     say(f"vorticity grid max |d| {err:.3e}, max |ref| {np.abs(want).max():.3e}")
     assert np.abs(want).max() > 1e-3
     assert err <= 1e-6
+    eng.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_axisym_vorticity_and_abyss(path):
+    """The axisymmetric variant's grid after a run holds velocities on both paths (the
+    fused path turns the grid its last G2P read into velocities in place, through the
+    axisymmetric grid op): the vorticity grid against the oracle's curl of the grid read
+    back, and the abyss frame against the oracle's of the state read back (20 substeps,
+    tanks at r >= 0.3, before the formulation diverges)."""
+    say(f"axisym vorticity grid and abyss frame, path={path}")
+    eng = _engine(M=3, substeps=20, path=path, axisym=True)
+    eng.render_vorticity_grid(2e-3)
+    got = eng.read_vorticity()
+    _, _, want = _grid_refs(eng, 2e-3)
+    err = np.abs(got - want).max()
+    say(f"vorticity grid max |d| {err:.3e}, max |ref| {np.abs(want).max():.3e}")
+    assert np.abs(want).max() > 1e-4
+    assert err <= 1e-6
+    st = _state(eng)
+    res, grid_side = 256, 2
+    res_sub = res // grid_side
+    hue = np.array([0.1, 0.5, 0.9], np.float32)
+    eng.render_clear(res, 0.0)
+    eng.render_abyss(res_sub, grid_side, 1.0, hue, hue, 0)
+    frame = eng.read_frame()
+    ref = render.render_frame_abyss(np.zeros((res, res, 3), np.float32), st["x"], st["v"],
+                                    st["Jp"], st["mat"], st["t"], st["freq"],
+                                    st["contraction"], hue, hue, 0, res_sub, grid_side, 1.0,
+                                    eng.domain_height, _consts(eng))
+    assert ref.max() > 0
+    _close_additive(frame, ref, "abyss (axisymmetric)")
     eng.close()
 
 
