@@ -152,3 +152,41 @@ def test_empty_batch_clock(require_gpu, path):
         t = np.float32(t + np.float32(2e-5))
     assert eng.get_time() == t
     eng.close()
+
+
+def test_maximum_slot_count():
+    """The largest batch jm_create accepts, 2^28 slots (1024 instances x 262,144 slots, each
+    with a ~1,000-particle tank, the rest dead): the particles of the last instances sit at
+    the top of the particle arrays (F and C byte offsets just below 2^32). Five substeps on
+    the fused path; instances 0, 511 and 1023 against the oracle (x within twice the fp32
+    oracle's own error against fp64, + 1e-6), every live position finite and in the
+    domain."""
+    M, N = 1024, 262144
+    say(f"maximum slot count: {M} x {N} = {M * N} slots, 5 substeps")
+    eng = Engine(M, N)
+    keep = {0: None, 511: None, 1023: None}
+    for m in range(M):
+        p, mt, f = small_tank(100 + m, N, region=REGIONS[m % len(REGIONS)])
+        eng.load_particles(m, p, mt, f)
+        if m in keep:
+            keep[m] = (p, mt, f)
+    eng.substep(5)
+    x = eng.read("x")
+    mat = eng.read("material")
+    live = mat >= 0
+    assert np.isfinite(x[live]).all()
+    assert x[live].min() >= 0.001 - 1e-7 and x[live].max() <= 0.999 + 1e-7
+    orcs = [oracle.OracleSim(len(keep), N, dtype=d) for d in (np.float32, np.float64)]
+    for o in orcs:
+        for k, (p, mt, f) in enumerate(keep.values()):
+            o.load(k, p, mt, f)
+        o.run(5, nthreads=8)
+    o32, o64 = orcs
+    for k, m in enumerate(keep):
+        lv = o32.material[k] >= 0
+        np.testing.assert_array_equal(mat[m], o32.material[k])
+        e_gpu = np.abs(x[m][lv] - o64.x[k][lv]).max()
+        e_32 = np.abs(o32.x[k][lv] - o64.x[k][lv]).max()
+        say(f"instance {m}: {int(lv.sum())} live, |x-x64| {e_gpu:.2e} (fp32 oracle {e_32:.2e})")
+        assert e_gpu <= 2 * e_32 + 1e-6, m
+    eng.close()
