@@ -317,6 +317,9 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
         fb_serial = episode(True, prefetch=False)
         log(f"batched diff forward + backward ({math} arithmetic), one stream {fb_serial:.2f} s")
         out["fwd_bwd_serial_s"] = fb_serial
+    out["adjoint"] = _adjoint_alone(tank, st0, con, freq, M, checkpoint)
+    log(f"batched adjoint alone ({math} arithmetic) {out['adjoint']['us_per_substep']:.1f} us "
+        f"per substep, {out['adjoint']['roofline']['frac']:.2f} of HBM peak")
     if resort:
         episode(True, 2 * checkpoint, 1)
         fwd_rs = episode(False, steps, 1)
@@ -327,6 +330,44 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
                                        "fwd_bwd_over_fused_forward": fb_rs / fused}
     tank.close()
     return out
+
+
+ADJOINT_BYTES = 156   # per live particle-substep: primal re-read 60 B + carried adjoint 2 x 48 B
+
+
+def _adjoint_alone(tank, st0, con, freq, M, n, reps: int = 3) -> dict:
+    """The adjoint kernels alone over one block of ``n`` substeps whose trajectory is
+    stored (no recomputation), from the state after 400 substeps (an episode's middle,
+    not the rest state): HIP events on the launching stream around ``backward_raw``, the
+    median of ``reps`` after one untimed; per live particle-substep 156 B algorithmic (the
+    primal state re-read, 60 B, and the carried adjoint read and written, 96 B) against
+    the 8 TB/s HBM peak."""
+    import torch
+    dev = tank.device
+    par = torch.tensor(np.stack([np.full(M, 500.0), con, freq, np.full(M, 2.5),
+                                 np.full(M, 700.0)], 1), dtype=tank.dtype, device=dev)
+    amp = torch.ones(tank.P, dtype=tank.dtype, device=dev)
+    s1 = tank.forward_raw(st0, amp, par, 0.0, 400)
+    t1 = 400 * tank.dt
+    stored = tank.trajectory(s1, amp, par, t1, n)
+    g = torch.zeros_like(s1)
+    g[:, 1] = 1.0
+    ts = []
+    for r in range(reps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        tank.backward_raw(s1, amp, par, t1, n, g, stored=stored)
+        e1.record()
+        torch.cuda.synchronize()
+        if r:
+            ts.append(e0.elapsed_time(e1) * 1e-3 / n)
+    del stored
+    sub = float(np.median(ts))
+    live = int((tank.mat >= 0).sum())
+    ach = ADJOINT_BYTES * live / sub / 1e9
+    return {"substeps": n, "us_per_substep": sub * 1e6,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+                         "frac": ach / 8000.0, "bytes_per_particle_substep": ADJOINT_BYTES}}
 
 
 WEB_PALETTE = [(0, 0.933, 0.949, 0.957), (1, 0.678, 0.882, 0.875), (3, 0.976, 0.757, 0.765),
