@@ -28,11 +28,14 @@ Also reported on the same line:
   instance, global atomics, unsorted);
 * diff: one differentiable episode of one seeded reference tank (seed 0, ~62K live
   particles, 2000 substeps, checkpoint every 100), in fp32 and fp64: forward through the
-  autograd blocks, forward + backward of the payload CoM y with recomputation from the
-  checkpoints, the same keeping every block's stored trajectory (no recomputation), and the
-  fused (non-differentiable, fp32) forward of the same episode for comparison; and the
-  batched configuration: 32 tall tanks (128x256, H = 2, 160K slots), 4000 substeps,
-  checkpoint every 200, per-instance gradients in one adjoint pass;
+  autograd blocks, forward + backward of the payload CoM y as DiffTank.run does it by
+  default (the last blocks keep their trajectories from the forward pass as device memory
+  allows, the others are recomputed from their checkpoints), the same with every block
+  recomputed (beside the next block's adjoint, and on one stream), the same keeping every
+  block's stored trajectory (recompute=False), and the fused (non-differentiable, fp32)
+  forward of the same episode for comparison; and the batched configuration: 32 tall tanks
+  (128x256, H = 2, 160K slots), 4000 substeps, checkpoint every 200, per-instance gradients
+  in one adjoint pass, and the adjoint kernels alone over one stored block;
 * render: one 1024x1024 video frame of the batch per renderer pipeline (abyss splat + tone
   map, web-palette flat passes, vorticity grid + RdBu composite) and its read-back;
 * view_mode: evolve.py's view loop through the mpm_sim facade (16 x 80K, 500 substep()
@@ -216,7 +219,8 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
                                         math=math)
         dev = tank.device
 
-        def episode(grad: bool, recompute: bool = True, prefetch: bool = True) -> float:
+        def episode(grad: bool, recompute: bool = True, prefetch: bool = True,
+                    keep_last=None) -> float:
             st = st0.clone().requires_grad_(grad)
             leaf = lambda v: torch.full((1,), v, dtype=dtype, device=dev, requires_grad=grad)
             amp_scale = torch.ones(tank.P, dtype=dtype, device=dev, requires_grad=grad)
@@ -224,7 +228,8 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
             t0 = time.perf_counter()
             with torch.set_grad_enabled(grad):
                 out = tank.run(st, amp_scale, leaf(500.0), leaf(con), leaf(freq), leaf(700.0),
-                               2.5, 0.0, steps, checkpoint, recompute, prefetch=prefetch)
+                               2.5, 0.0, steps, checkpoint, recompute, prefetch=prefetch,
+                               keep_last=keep_last)
                 y = tank.payload_com_y(out).sum()
                 if grad:
                     y.backward()
@@ -234,17 +239,23 @@ def diff_episode(steps: int = 2000, checkpoint: int = 100) -> dict:
         episode(True)                              # warm-up (allocations, code objects)
         fwd = min(episode(False) for _ in range(2))
         fb = min(episode(True) for _ in range(2))
-        fb_serial = min(episode(True, prefetch=False) for _ in range(2))
+        kept = tank.last_kept
+        fb_rec = min(episode(True, keep_last=0) for _ in range(2))
+        fb_serial = min(episode(True, prefetch=False, keep_last=0) for _ in range(2))
         episode(True, recompute=False)
         fb_stored = min(episode(True, recompute=False) for _ in range(2))
         res["workload"] = (f"1 reference tank (seed 0, {tank.P} live), {steps} substeps, "
                            f"checkpoint every {checkpoint}, payload CoM y objective")
+        # fwd_bwd: DiffTank.run's defaults (the last blocks keep their trajectories as
+        # memory allows: here all of them); recompute: every block recomputed beside the
+        # next block's adjoint; serial: the same on one stream; stored: recompute=False
         res[key] = {
-            "forward_ms": fwd * 1e3, "fwd_bwd_ms": fb * 1e3, "fwd_bwd_stored_ms": fb_stored * 1e3,
-            "fwd_bwd_serial_ms": fb_serial * 1e3,
-            "fwd_bwd_over_forward": fb / fwd, "fwd_bwd_stored_over_forward": fb_stored / fwd,
+            "forward_ms": fwd * 1e3, "fwd_bwd_ms": fb * 1e3, "kept_blocks": kept,
+            "fwd_bwd_recompute_ms": fb_rec * 1e3, "fwd_bwd_serial_ms": fb_serial * 1e3,
+            "fwd_bwd_stored_ms": fb_stored * 1e3,
+            "fwd_bwd_over_forward": fb / fwd, "fwd_bwd_recompute_over_forward": fb_rec / fwd,
             "fwd_bwd_over_fused_forward": fb / fused,
-            "fwd_bwd_stored_over_fused_forward": fb_stored / fused}
+            "fwd_bwd_recompute_over_fused_forward": fb_rec / fused}
         tank.close()
     return res
 
@@ -291,7 +302,8 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
                                     math=math)
     dev = tank.device
 
-    def episode(grad: bool, n: int = steps, resort: int = 0, prefetch: bool = True) -> float:
+    def episode(grad: bool, n: int = steps, resort: int = 0, prefetch: bool = True,
+                keep_last=None) -> float:
         st = st0.clone()
         vec = lambda a: torch.tensor(np.asarray(a, np.float32), device=dev).requires_grad_(grad)
         amp, c, f = vec(np.full(M, 500.0)), vec(con), vec(freq)
@@ -299,22 +311,31 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
         t0 = time.perf_counter()
         with torch.set_grad_enabled(grad):
             out = tank.run(st, None, amp, c, f, 700.0, 2.5, 0.0, n, checkpoint,
-                           resort_every=resort, prefetch=prefetch)
+                           resort_every=resort, prefetch=prefetch, keep_last=keep_last)
             y = tank.payload_com_y(out).sum()
             if grad:
                 y.backward()
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    episode(True, checkpoint)   # warm-up: first launches and the allocator's buffers
+    episode(True)   # warm-up: first launches, and the allocator caches the kept blocks
     fwd = episode(False)
     log(f"batched diff forward ({math} arithmetic) {fwd:.2f} s")
     fb = episode(True)
-    log(f"batched diff forward + backward ({math} arithmetic) {fb:.2f} s")
-    out = {"forward_s": fwd, "fwd_bwd_s": fb, "fwd_bwd_over_forward": fb / fwd,
+    kept = tank.last_kept
+    log(f"batched diff forward + backward ({math} arithmetic) {fb:.2f} s, last {kept} of "
+        f"{-(-steps // checkpoint)} blocks kept from the forward pass")
+    fb_rec = episode(True, keep_last=0)
+    log(f"batched diff forward + backward ({math} arithmetic), every block recomputed "
+        f"{fb_rec:.2f} s")
+    # fwd_bwd: DiffTank.run's defaults (the last blocks keep their trajectories as memory
+    # allows); recompute: every block recomputed beside the next block's adjoint
+    out = {"forward_s": fwd, "fwd_bwd_s": fb, "kept_blocks": kept,
+           "fwd_bwd_recompute_s": fb_rec, "fwd_bwd_over_forward": fb / fwd,
+           "fwd_bwd_recompute_over_forward": fb_rec / fwd,
            "fwd_bwd_over_fused_forward": fb / fused}
     if math == "fp32":
-        fb_serial = episode(True, prefetch=False)
+        fb_serial = episode(True, prefetch=False, keep_last=0)
         log(f"batched diff forward + backward ({math} arithmetic), one stream {fb_serial:.2f} s")
         out["fwd_bwd_serial_s"] = fb_serial
     out["adjoint"] = _adjoint_alone(tank, st0, con, freq, M, checkpoint)

@@ -177,6 +177,7 @@ class DiffTank:
         self._h = h
         self._L = load()
         self._owner = None      # a view (see _view) keeps the tank that owns the handle
+        self.last_kept = 0      # blocks whose trajectories the last run() kept
         self.water_start = self.P
         if water_start is not None:
             ws = int(water_start)
@@ -338,6 +339,23 @@ class DiffTank:
         out = (cs[:, self._seg[1:]] - cs[:, self._seg[:-1]]).t()     # (M, C)
         return out.reshape((self.M,) + tuple(values.shape[1:]))
 
+    def block_bytes(self, n: int) -> int:
+        """Device bytes of one block's stored states and grids (:meth:`trajectory_buffers`)."""
+        item = 8 if self.dtype == torch.float64 else 4
+        return ((n + 1) * self.P * STATE_COLS + n * self.M * 3 * self.G) * item
+
+    def blocks_to_keep(self, nsteps: int, checkpoint: int, keep_memory: float) -> int:
+        """How many of the last blocks of :meth:`run` keep their trajectories: as many as
+        fit in ``keep_memory`` times the device memory free now (the caching allocator's
+        unused reserve counted as free) less 2 GiB, at most all of them; 0 off the GPU."""
+        nb = -(-int(nsteps) // int(checkpoint))
+        if keep_memory <= 0 or nb == 0 or self.device.type != "cuda":
+            return 0
+        free, _ = torch.cuda.mem_get_info(self.device)
+        free += torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+        room = float(keep_memory) * free - float(2 << 30)
+        return max(0, min(nb, int(room // self.block_bytes(int(checkpoint)))))
+
     def trajectory_buffers(self, n: int):
         """Uninitialised (traj (n+1, P, 12), grids (n, M, 3, nx*ny)) for :meth:`trajectory`."""
         return (torch.empty((n + 1, self.P, STATE_COLS), dtype=self.dtype, device=self.device),
@@ -399,7 +417,8 @@ class DiffTank:
     def run(self, state0: torch.Tensor, amp_scale=None, amp=500.0, contraction=0.2, freq=1.0,
             E_jelly=700.0, payload_density=2.5, t0: float = 0.0, nsteps: int = 100,
             checkpoint: int = 100, recompute: bool = True, resort_every: int = 0,
-            prefetch: bool = True) -> torch.Tensor:
+            prefetch: bool = True, keep_memory: float = 0.8,
+            keep_last: int | None = None) -> torch.Tensor:
         """``nsteps`` substeps from ``state0`` at time ``t0``, as autograd blocks of
         ``checkpoint`` substeps. Per-instance arguments are scalars or (M,) tensors.
 
@@ -412,6 +431,17 @@ class DiffTank:
         recomputation of block b-1 on a second stream before it runs its own adjoint, so
         the two run side by side (the recomputation does not depend on the adjoint); at
         most two blocks' trajectories are held at once. Same kernels, same results.
+
+        ``keep_memory`` (with ``recompute``): the last blocks keep their states and grids
+        from the forward pass, as with ``recompute=False``, as many as fit in this fraction
+        of the device memory free when the run starts (:meth:`blocks_to_keep`; 0 keeps
+        none), so the backward pass recomputes only the blocks before them. Storing costs
+        the forward pass no extra work (it writes every substep's state and grid either
+        way); the kept blocks' memory is returned as the backward pass consumes them, and
+        the recomputation of the earlier blocks reuses it. One reference tank keeps its
+        whole episode (~0.3 GB per 100-substep block); the 32-tank tall batch keeps 6 of
+        its 20 blocks (41 GB each) on a 288 GB device. ``keep_last`` = K (an int) keeps
+        exactly the last K blocks instead (at most all of them; 0 recomputes every block).
 
         ``resort_every`` = K > 0: before every K-th block after the first, the particles
         are re-ordered by their current cells (``cell_permutation``), as the fused engine
@@ -437,6 +467,15 @@ class DiffTank:
         cur, perm, amp_cur = self, None, amp_scale
         episode = _Episode() if recompute and prefetch and self.device.type == "cuda" \
             else None
+        nblocks = -(-int(nsteps) // int(checkpoint))
+        if not recompute:
+            kept = 0
+        elif keep_last is not None:
+            kept = max(0, min(nblocks, int(keep_last)))
+        else:
+            kept = self.blocks_to_keep(nsteps, checkpoint, keep_memory)
+        keep_from = nblocks - kept
+        self.last_kept = kept
         while done < nsteps:
             if resort_every > 0 and blocks > 0 and blocks % int(resort_every) == 0:
                 p = cur.cell_permutation(state)
@@ -445,7 +484,7 @@ class DiffTank:
                 cur = self._view(perm)
             n = min(int(checkpoint), nsteps - done)
             state = _Block.apply(state, amp_cur, amp, contraction, freq, E_jelly, pdens,
-                                 cur, t, n, recompute, episode)
+                                 cur, t, n, recompute and blocks < keep_from, episode)
             t = clock(t, n, self.dt, fp64)
             done += n
             blocks += 1
@@ -467,7 +506,8 @@ class _Episode:
     """The blocks of one ``DiffTank.run`` with recomputation, so that the backward pass of
     block b can start block b-1's recomputation (on a side stream) before its own adjoint.
 
-    ``blocks[b]`` = (tank, state, amp_scale, params, t0, n) as block b's forward saw them;
+    ``blocks[b]`` = (tank, state, amp_scale, params, t0, n) as block b's forward saw them,
+    or None for a block that kept its trajectory or whose recomputation was taken;
     ``pending[b]`` = (traj, grids, event, buffers) of a recomputation in flight. Two sets
     of trajectory buffers, sized for the longest block, serve every block in turn: one is
     read by the current adjoint while the other is written by the next recomputation."""
@@ -479,8 +519,11 @@ class _Episode:
         self.n_max = 0
 
     def add(self, rec) -> int:
+        """Registers the next block: its inputs, or None for a block that keeps its
+        trajectory (nothing to recompute)."""
         self.blocks.append(rec)
-        self.n_max = max(self.n_max, rec[5])
+        if rec is not None:
+            self.n_max = max(self.n_max, rec[5])
         return len(self.blocks) - 1
 
     def _buffers(self, tank):
@@ -539,18 +582,22 @@ class _Block(torch.autograd.Function):
         ctx.save_for_backward(state_c, amp_c, params)
         ctx.tank, ctx.t0, ctx.n = tank, t0, n
         ctx.episode = episode
-        ctx.index = episode.add((tank, state_c, amp_c, params, t0, n)) if episode else -1
+        ctx.index = -1
+        if episode is not None:
+            ctx.index = episode.add((tank, state_c, amp_c, params, t0, n) if recompute
+                                    else None)
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
         state, amp_c, params = ctx.saved_tensors
         stored, ep, bufs = ctx.stored, ctx.episode, None
-        if stored is None and ep is not None:
-            got = ep.take(ctx.index)
-            if got is not None:
-                traj, grids, bufs = got
-                stored = (traj, grids)
+        if ep is not None:
+            if stored is None:
+                got = ep.take(ctx.index)
+                if got is not None:
+                    traj, grids, bufs = got
+                    stored = (traj, grids)
             ep.start(ctx.index - 1)    # the previous block's recomputation, beside this adjoint
         gin, gamp, gpar = ctx.tank.backward_raw(state, amp_c, params, ctx.t0, ctx.n, grad_out,
                                                 stored=stored)

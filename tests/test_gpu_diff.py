@@ -166,7 +166,7 @@ def test_diff_adjoint_vs_fp64_adjoint(require_gpu, dtype, math):
 
 
 def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every=0,
-                   math="fp64", prefetch=True):
+                   math="fp64", prefetch=True, keep_last=None):
     """Autograd through DiffTank.run: d(payload CoM y displacement) w.r.t. the state,
     the per-particle multiplier and the per-instance parameters."""
     tank, st, amp, par = _gpu(cs, dtype, math)
@@ -175,7 +175,9 @@ def _gpu_gradients(cs, checkpoint, dtype="float32", recompute=True, resort_every
     leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2, 4)]
     out = tank.run(st, amp, *leaves[:3], E_jelly=leaves[3], payload_density=par[:, 3],
                    t0=cs.t0, nsteps=cs.steps, checkpoint=checkpoint, recompute=recompute,
-                   resort_every=resort_every, prefetch=prefetch)
+                   resort_every=resort_every, prefetch=prefetch, keep_last=keep_last)
+    if keep_last is not None and recompute:
+        assert tank.last_kept == min(keep_last, -(-cs.steps // checkpoint))
     y = tank.payload_com_y(out) - tank.payload_com_y(st.detach())
     y.sum().backward()
     g = lambda t: t.grad.detach().cpu().numpy().astype(np.float64)
@@ -240,7 +242,7 @@ def test_reference_tank_gradients_vs_fd(require_gpu, steps, dtype, math):
     cs, pos, mat, fib = reference_tank_case(gg["steps"])
     assert cs.P == gg["n_live"]
     np.testing.assert_array_equal(cs.params[0], np.array(gg["params"]))
-    y, gst, gamp, gpar = _gpu_gradients(cs, checkpoint=100, dtype=dtype, math=math)
+    y, gst, gamp, gpar = _gpu_gradients(cs, checkpoint=100, dtype=dtype, math=math, keep_last=0)
     say(f"payload displacement {y:.9e}; dJ/d(amp, contraction, freq, E) = {gpar}")
     bound = GRAD_BOUNDS[(steps, dtype, math)]
     worst, worst32 = 0.0, 0.0
@@ -268,19 +270,23 @@ def test_reference_tank_gradients_vs_fd(require_gpu, steps, dtype, math):
 
 def test_gradients_independent_of_checkpoint_interval(require_gpu):
     """Recomputing from checkpoints every 100 or 64 substeps (each block's recomputation
-    beside the next block's adjoint on a second stream, or both on one stream), or keeping
-    the stored trajectory, gives the gradients of one 300-substep block, up to the
-    run-to-run noise of atomics order (measured by repeating the 300-substep run). fp64, so
-    that the noise is small. The full state gradient is reported, not asserted: water
+    beside the next block's adjoint on a second stream, or both on one stream), keeping the
+    last 2 of 5 blocks' trajectories from the forward pass and recomputing the other 3, or
+    keeping every block's trajectory, gives the gradients of one 300-substep block, up to
+    the run-to-run noise of atomics order (measured by repeating the 300-substep run). fp64,
+    so that the noise is small. The full state gradient is reported, not asserted: water
     resting against the walls sits on the wall clamps' kink (zero grid velocity), so the
     branch its adjoint takes follows that noise."""
     say("reference tank seed0, 300 substeps, fp64: checkpoint 300 (twice) vs 100 vs 64 vs "
-        "64 on one stream vs stored")
+        "64 on one stream vs 64 keeping the last 2 blocks vs stored")
     cs, _, _, _ = reference_tank_case(300)
-    runs = {k: _gpu_gradients(cs, checkpoint=k, dtype="float64") for k in (300, 100, 64)}
+    runs = {k: _gpu_gradients(cs, checkpoint=k, dtype="float64", keep_last=0)
+            for k in (300, 100, 64)}
     runs["stored"] = _gpu_gradients(cs, checkpoint=100, dtype="float64", recompute=False)
-    runs["64, one stream"] = _gpu_gradients(cs, checkpoint=64, dtype="float64", prefetch=False)
-    again = _gpu_gradients(cs, checkpoint=300, dtype="float64")
+    runs["64, one stream"] = _gpu_gradients(cs, checkpoint=64, dtype="float64", prefetch=False,
+                                            keep_last=0)
+    runs["64, last 2 kept"] = _gpu_gradients(cs, checkpoint=64, dtype="float64", keep_last=2)
+    again = _gpu_gradients(cs, checkpoint=300, dtype="float64", keep_last=0)
     body = cs.mat > 0
 
     def groups(r):
@@ -289,7 +295,7 @@ def test_gradients_independent_of_checkpoint_interval(require_gpu):
     base = groups(runs[300])
     floor = {n: np.linalg.norm(v - base[n]) / np.linalg.norm(base[n])
              for n, v in groups(again).items()}
-    for k in (100, 64, "64, one stream", "stored"):
+    for k in (100, 64, "64, one stream", "64, last 2 kept", "stored"):
         for n, v in groups(runs[k]).items():
             err = np.linalg.norm(v - base[n]) / np.linalg.norm(base[n])
             say(f"checkpoint {k} vs 300, {n}: {err:.2e} (repeat of 300: {floor[n]:.2e})")
@@ -305,9 +311,10 @@ def test_gradients_with_resorting_between_blocks(require_gpu):
     100."""
     say("reference tank seed0, 300 substeps, fp64: re-sorted every block vs not")
     cs, _, _, _ = reference_tank_case(300)
-    base = _gpu_gradients(cs, checkpoint=100, dtype="float64")
-    again = _gpu_gradients(cs, checkpoint=100, dtype="float64")
-    rs = _gpu_gradients(cs, checkpoint=100, dtype="float64", resort_every=1)
+    base = _gpu_gradients(cs, checkpoint=100, dtype="float64", keep_last=0)
+    again = _gpu_gradients(cs, checkpoint=100, dtype="float64", keep_last=0)
+    rs = _gpu_gradients(cs, checkpoint=100, dtype="float64", resort_every=1,
+                        keep_last=0)
     body = cs.mat > 0
 
     def groups(r):
@@ -406,18 +413,22 @@ def test_split_packing_gives_the_same_gradients(require_gpu, dtype, math):
         assert err <= bound, (name, err)
 
 
-def test_second_backward_through_a_retained_graph(require_gpu):
+@pytest.mark.parametrize("keep_last", [0, 1])
+def test_second_backward_through_a_retained_graph(require_gpu, keep_last):
     """A second backward pass through the same graph (``retain_graph=True``) recomputes every
-    block again (the first pass consumed the pipelined recomputations) and adds the same
-    gradients once more: fp64, one reference tank, 300 substeps, checkpoint 100."""
+    block again (the first pass consumed the pipelined recomputations and released the kept
+    trajectories) and adds the same gradients once more: fp64, one reference tank, 300
+    substeps, checkpoint 100, with every block recomputed in the first pass or the last one
+    kept from the forward pass."""
     import torch
-    say("reference tank seed0, 300 substeps, fp64: backward twice through a retained graph")
+    say(f"reference tank seed0, 300 substeps, fp64, last {keep_last} kept: backward twice "
+        "through a retained graph")
     cs, _, _, _ = reference_tank_case(300)
     tank, st, amp, par = _gpu(cs, "float64", "fp64")
     st.requires_grad_(True)
     leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2)]
     out = tank.run(st, amp, *leaves, E_jelly=par[:, 4], payload_density=par[:, 3], t0=cs.t0,
-                   nsteps=cs.steps, checkpoint=100)
+                   nsteps=cs.steps, checkpoint=100, keep_last=keep_last)
     y = tank.payload_com_y(out).sum()
     y.backward(retain_graph=True)
     first = [t.grad.clone() for t in [st] + leaves]
