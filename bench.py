@@ -356,6 +356,29 @@ def _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused, ma
 ADJOINT_BYTES = 156   # per live particle-substep: primal re-read 60 B + carried adjoint 2 x 48 B
 
 
+def adjoint_only(M: int = 32, slots: int = 160000, checkpoint: int = 200) -> dict:
+    """The adjoint kernels alone on the batched configuration's 32 tall tanks: one stored
+    block (``_adjoint_alone``) with fp32 arithmetic (cell packing) and with fp64 arithmetic
+    over fp32 buffers (split packing: body kernel, then water-only kernel), and nothing
+    else of the adjoint's in the process, so that a kernel trace's average per adjoint
+    kernel is its time per substep."""
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.diff import DiffTank
+    pos, mat, fib, genomes, infos = phenotype.population(
+        range(M), slots, 128, domain_height=2.0, spawn_offset=phenotype.TALL_SPAWN)
+    con = np.clip(genomes[:, 9], 0.05, 0.60)
+    freq = np.clip(genomes[:, 10], 0.5, 2.0)
+    out = {}
+    for key, math in (("fp32", "fp32"), ("fp32_buffers_fp64_math", "fp64")):
+        tank, st0 = DiffTank.from_tanks(pos, mat, fib, n_grid_y=256, domain_height=2.0,
+                                        math=math)
+        out[key] = _adjoint_alone(tank, st0, con, freq, M, checkpoint)
+        log(f"adjoint alone ({math} arithmetic) {out[key]['us_per_substep']:.1f} us per "
+            f"substep, {out[key]['roofline']['frac']:.2f} of HBM peak")
+        tank.close()
+    return out
+
+
 def _adjoint_alone(tank, st0, con, freq, M, n, reps: int = 3) -> dict:
     """The adjoint kernels alone over one block of ``n`` substeps whose trajectory is
     stored (no recomputation), from the state after 400 substeps (an episode's middle,
@@ -646,6 +669,8 @@ def main() -> int:
                     help="run only the differentiable legs (no throughput line fields)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched tall-tank forward + backward")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run only the adjoint kernels over one stored block of the batch")
     ap.add_argument("--only-render", action="store_true",
                     help="run only the renderer leg (after the warm-up blocks)")
     ap.add_argument("--no-view", action="store_true")
@@ -674,6 +699,10 @@ def main() -> int:
         print(json.dumps({"diff": {"batched": diff_batched(
             steps=args.batched_steps, checkpoint=args.batched_checkpoint,
             resort=args.diff_resort)}}), flush=True)
+        return 0
+    if args.only_adjoint:
+        print(json.dumps({"diff": {"adjoint_only": adjoint_only(
+            checkpoint=args.batched_checkpoint)}}), flush=True)
         return 0
     if args.only_diff:
         out = {"diff": diff_episode()}
