@@ -413,6 +413,37 @@ def test_split_packing_gives_the_same_gradients(require_gpu, dtype, math):
         assert err <= bound, (name, err)
 
 
+def test_kept_block_count(require_gpu):
+    """DiffTank.blocks_to_keep: a block's bytes are its stored states and grids; no block for
+    a zero fraction or an empty run; every block of a short run of one tank for the default
+    fraction; as many as fit in the given fraction of free memory otherwise (a block of a
+    third to two thirds of the free memory: none at 0.3, at least one at 0.9); and run()
+    keeps that many (tank.last_kept), keep_last overriding it."""
+    import torch
+    say("reference tank seed0: kept block counts against block size and free memory")
+    cs, _, _, _ = reference_tank_case(300)
+    tank, st, amp, par = _gpu(cs, "float32", "fp32")
+    assert tank.block_bytes(100) == (101 * tank.P * 12 + 100 * tank.M * 3 * tank.G) * 4
+    assert tank.blocks_to_keep(300, 100, 0.0) == 0
+    assert tank.blocks_to_keep(0, 100, 0.8) == 0
+    assert tank.blocks_to_keep(300, 100, 0.8) == 3
+    assert tank.blocks_to_keep(250, 100, 0.8) == 3
+    free, _ = torch.cuda.mem_get_info(tank.device)
+    free += torch.cuda.memory_reserved(tank.device) - torch.cuda.memory_allocated(tank.device)
+    n = 1
+    while tank.block_bytes(n) < free / 3:   # the block doubles, less its shared first state
+        n *= 2
+    say(f"free {free / 2**30:.1f} GiB; block of {n} substeps {tank.block_bytes(n) / 2**30:.1f} GiB")
+    assert tank.blocks_to_keep(20 * n, n, 0.3) == 0
+    assert tank.blocks_to_keep(20 * n, n, 0.9) >= 1
+    for kw, want in (({}, 3), ({"keep_last": 1}, 1), ({"keep_memory": 0.0}, 0),
+                     ({"keep_last": 7}, 3)):
+        tank.run(st, amp, par[:, 0], par[:, 1], par[:, 2], par[:, 4], par[:, 3], cs.t0, 300,
+                 100, **kw)
+        assert tank.last_kept == want, (kw, tank.last_kept)
+    tank.close()
+
+
 @pytest.mark.parametrize("keep_last", [0, 1])
 def test_second_backward_through_a_retained_graph(require_gpu, keep_last):
     """A second backward pass through the same graph (``retain_graph=True``) recomputes every
