@@ -11,6 +11,9 @@
   reproducible (larger instances flush several chunks per node in a run-dependent order).
 * The opt-in NaN->invalid flag of jm_run_batch_headless, against the oracle's.
 * The clock of a batch with no live particles still advances one fp32 dt per substep.
+* Limits: the largest batch jm_create accepts (2^28 slots) against the oracle; one whole
+  150,000-substep fitness episode of the 32 x 80K generation batch, checked by what holds
+  at any length (clock, validity, finiteness, domain, the results' own statistics).
 """
 from __future__ import annotations
 
@@ -189,4 +192,51 @@ def test_maximum_slot_count():
         e_32 = np.abs(o32.x[k][lv] - o64.x[k][lv]).max()
         say(f"instance {m}: {int(lv.sum())} live, |x-x64| {e_gpu:.2e} (fp32 oracle {e_32:.2e})")
         assert e_gpu <= 2 * e_32 + 1e-6, m
+    eng.close()
+
+
+def test_full_fitness_episode():
+    """One whole fitness episode of the 32 x 80K generation batch (seeds 0-31 through the
+    fill_tank restatement, the actuation genes and the autotuned launch parameters of the
+    throughput benchmark): run_batch_headless(150000), as evolve.py:48 runs it
+    (mpm_sim.py:802-834). The oracle cannot run this far, so the test checks what holds at
+    any length: the clock equals the fp32 sum of 150,000 steps; every result is finite and
+    every instance valid; materials and live counts are unchanged and every live position
+    is finite and inside the advection clamp [0.001, 0.999]; the initial and final payload
+    CoM equal, bit for bit, the fp32 quotients of jm_compute_stats' sums of the state they
+    were taken from; the fitness of evolve.py:90-124 is finite for every genome."""
+    from jellyfih_amd import phenotype
+    from jellyfih_amd.autotune import tuning_for
+    from jellyfih_amd.diff import clock
+    say("full fitness episode: 32 x 80K, run_batch_headless(150000)")
+    pos, mat, fib, genomes, infos = phenotype.population(range(32), 80000, 128)
+    eng = Engine(32, 80000)
+    tuning = tuning_for(32, 80000)
+    assert tuning is not None
+    eng.set_tuning(**{k: v for k, v in tuning.items() if k in eng.tuning()})
+    for m in range(32):
+        eng.load_particles(m, pos[m], mat[m], fib[m])
+    eng.set_param("contraction", np.clip(genomes[:, 9], 0.05, 0.60))
+    eng.set_param("freq", np.clip(genomes[:, 10], 0.5, 2.0))
+    live = [eng.live_count(m) for m in range(32)]
+    init = oracle.finish_stats(eng.raw_stats("payload"))
+    res = eng.run_batch_headless(150000)
+    final = oracle.finish_stats(eng.raw_stats("payload"))
+    say(f"clock {eng.get_time()!r}; valid {int(res[:, 4].sum())}/32; "
+        f"final y {res[:, 2].min():.4f}..{res[:, 2].max():.4f}")
+    assert eng.get_time() == np.float32(clock(0.0, 150000))
+    assert np.isfinite(res).all()
+    assert np.all(res[:, 4] == 1.0)
+    np.testing.assert_array_equal(res[:, 0], init[:, 0])
+    np.testing.assert_array_equal(res[:, 1], init[:, 1])
+    np.testing.assert_array_equal(res[:, 2], final[:, 0])
+    np.testing.assert_array_equal(res[:, 3], final[:, 1])
+    assert [eng.live_count(m) for m in range(32)] == live
+    np.testing.assert_array_equal(eng.read("material"), mat)
+    x = eng.read("x")[mat >= 0]
+    assert np.isfinite(x).all()
+    assert x.min() >= np.float32(0.001) and x.max() <= np.float32(0.999)
+    muscle = [info["muscle_count"] for info in infos]
+    fit = oracle.compute_fitness(res, muscle, genomes)
+    assert all(f is not None and np.isfinite(f) for f in fit)
     eng.close()
