@@ -469,3 +469,67 @@ def test_second_backward_through_a_retained_graph(require_gpu, keep_last):
         say(f"{name}: |g(2 passes) - 2 g(1 pass)| / |2 g| = {err:.2e}")
         assert err <= 1e-9, name
     tank.close()
+
+
+@pytest.mark.parametrize("dtype,math", PRECISIONS)
+def test_diff_quality_two(require_gpu, dtype, math):
+    """quality = 2 (mpm_sim.py:12,14,19: a 256x256 grid and dt = 1e-5, which the reference
+    fixes at 1 and the engine takes as a parameter): a small tank with a jelly, muscle and
+    payload patch at 4 particles per cell of the finer grid, 200 substeps from t = 0.05 (a
+    contraction phase). The differentiable forward against the oracle built with the same
+    constants (fp32 buffers as accurate as the fp32 oracle; fp64 equal to the fp64 oracle
+    up to summation order), and the gradients of the payload CoM y w.r.t. amplitude,
+    contraction, frequency and E_jelly (every block recomputed, checkpoint 100) against
+    fp64 central differences of that oracle: fp64 within 1e-4, fp32 buffers with fp64
+    arithmetic within 1e-3, fp32 arithmetic within 2e-3 (measured 3e-7, 9e-5 and 2e-4: on
+    this small tank over 0.002 s the fp32 rounding of the water's small strains, which
+    sets fp32's error on the reference tank, stays small)."""
+    import oracle
+    import torch
+    from jellyfih_amd.diff import DiffTank
+    from tests.cases import small_tank
+    from tests.harness import oracle_payload_fd
+    say(f"quality 2 (256x256, dt 1e-5), small tank, 200 substeps, {dtype}/{math}: forward "
+        "and gradients vs the oracle")
+    pos, mat, fib = small_tank(7, 9000, region=(0.30, 0.50, 0.30, 0.45), ppc_spacing=1 / 512)
+    live = mat >= 0
+    P = int(live.sum())
+    state = np.zeros((P, 12))
+    state[:, 0:2] = pos[live]
+    state[:, 8] = state[:, 11] = 1.0
+    params = np.array([[500.0, 0.25, 1.1, 2.5, 700.0]])
+    cs = Case.from_arrays(np.zeros(P, np.int32), mat[live], fib[live], state, np.ones(P),
+                          params, steps=200, t0=0.05).round_f32()
+    cs.c = oracle.reference_consts(quality=2)
+    td = getattr(torch, dtype)
+    tank = DiffTank(cs.inst, cs.mat, cs.fiber, 1, quality=2, dtype=td, math=math)
+    f = lambda a: torch.tensor(np.asarray(a), dtype=td, device=tank.device)
+    st, amp, par = tank.encode_state(f(cs.state)), f(cs.amp_scale), f(cs.params)
+    out = tank.decode_state(tank.forward_raw(st, amp, par, cs.t0, cs.steps).double())
+    out = out.cpu().numpy()
+    o64 = cs.oracle_forward()
+    o32 = cs.oracle_forward(dtype=np.float32).astype(np.float64)
+    for g, sl in GROUPS.items():
+        e_gpu = np.abs(out[:, sl] - o64[:, sl]).max()
+        e_32 = np.abs(o32[:, sl] - o64[:, sl]).max()
+        say(f"{g}: |gpu-f64|={e_gpu:.3e} |oracle32-f64|={e_32:.3e}")
+        if dtype == "float32":
+            assert e_gpu <= 2.0 * e_32 + 1e-6, g
+        else:
+            assert e_gpu <= 1e-9 * max(np.abs(o64[:, sl]).max(), 1.0), g
+    leaves = [par[:, i].clone().requires_grad_(True) for i in (0, 1, 2, 4)]
+    res = tank.run(st, amp, *leaves[:3], E_jelly=leaves[3], payload_density=par[:, 3],
+                   t0=cs.t0, nsteps=cs.steps, checkpoint=100, keep_last=0)
+    tank.payload_com_y(res).sum().backward()
+    scales = [500.0, 0.25, 1.1, 700.0]
+    ad = [float(t.grad[0]) * s for t, s in zip(leaves, scales)]
+    dps = [np.eye(5)[i] * s for i, s in zip((0, 1, 2, 4), scales)]
+    fd = oracle_payload_fd(cs, pos, mat, fib, [np.zeros((P, 2))] * 4, dps, [1e-4, 1e-5],
+                           nthreads=8)
+    bound = {("float64", "fp64"): 1e-4, ("float32", "fp64"): 1e-3,
+             ("float32", "fp32"): 2e-3}[(dtype, math)]
+    for name, a, fds in zip(("amp", "contraction", "freq", "E_jelly"), ad, fd):
+        rel = min(abs(a - x) / abs(x) for x in fds)
+        say(f"d/d {name:11s} (x scale) AD={a: .6e} FD={fds} |AD-FD|/|FD|={rel:.1e}")
+        assert rel <= bound, (name, rel)
+    tank.close()
