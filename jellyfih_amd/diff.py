@@ -3,9 +3,11 @@
 The reference simulator (``mpm_sim.py:153-340``) has no autodiff. This module puts the same
 substep behind ``torch.autograd``: the forward and the adjoint are HIP kernels in
 ``libjellymx.so`` (``jm_fwd_block`` / ``jm_bwd_block``), and :class:`_Block` is one
-``torch.autograd.Function`` per block of ``checkpoint`` substeps. A block keeps only its
-input state (the checkpoint); its backward re-runs the block's forward storing every
-substep's state and grid, then runs the adjoint substeps in reverse.
+``torch.autograd.Function`` per block of ``checkpoint`` substeps. A block keeps its input
+state (the checkpoint); its backward re-runs the block's forward storing every substep's
+state and grid, then runs the adjoint substeps in reverse. The last blocks of a run keep
+their stored states and grids from the forward pass instead, as many as device memory
+allows (``DiffTank.run``'s ``keep_memory`` / ``keep_last``).
 
 State layout, one row of 12 fp32 per live particle:
 ``x0 x1 v0 v1 C00 C01 C10 C11 F00 F01 F10 F11``; with fp32 buffers and fp64 arithmetic
@@ -469,7 +471,7 @@ class DiffTank:
             else None
         nblocks = -(-int(nsteps) // int(checkpoint))
         if not recompute:
-            kept = 0
+            kept = nblocks          # every block keeps its trajectory
         elif keep_last is not None:
             kept = max(0, min(nblocks, int(keep_last)))
         else:

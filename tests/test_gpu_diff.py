@@ -418,7 +418,8 @@ def test_kept_block_count(require_gpu):
     a zero fraction or an empty run; every block of a short run of one tank for the default
     fraction; as many as fit in the given fraction of free memory otherwise (a block of a
     third to two thirds of the free memory: none at 0.3, at least one at 0.9); and run()
-    keeps that many (tank.last_kept), keep_last overriding it."""
+    keeps that many (tank.last_kept), keep_last overriding it; without recomputation
+    every block keeps its trajectory."""
     import torch
     say("reference tank seed0: kept block counts against block size and free memory")
     cs, _, _, _ = reference_tank_case(300)
@@ -437,7 +438,7 @@ def test_kept_block_count(require_gpu):
     assert tank.blocks_to_keep(20 * n, n, 0.3) == 0
     assert tank.blocks_to_keep(20 * n, n, 0.9) >= 1
     for kw, want in (({}, 3), ({"keep_last": 1}, 1), ({"keep_memory": 0.0}, 0),
-                     ({"keep_last": 7}, 3)):
+                     ({"keep_last": 7}, 3), ({"recompute": False, "keep_last": 0}, 3)):
         tank.run(st, amp, par[:, 0], par[:, 1], par[:, 2], par[:, 4], par[:, 3], cs.t0, 300,
                  100, **kw)
         assert tank.last_kept == want, (kw, tank.last_kept)
