@@ -21,6 +21,17 @@ def pytest_collection_finish(session):
         gpu_available()
 
 
+@pytest.fixture(autouse=True, scope="module")
+def _release_torch_cache():
+    """After each test module, return what torch's caching allocator holds to the device:
+    the differentiable tests keep trajectories of up to ~200 GB, and the engine's own
+    allocations in later modules (hipMalloc, outside torch) need that memory."""
+    yield
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.empty_cache()
+
+
 @pytest.fixture(scope="session")
 def golden():
     return np.load(os.path.join(ROOT, "tests", "golden", "phenotype_golden.npz"))
