@@ -291,6 +291,8 @@ def diff_batched(M: int = 32, slots: int = 160000, steps: int = 4000,
     for key, math in (("fp32", "fp32"), ("fp32_buffers_fp64_math", "fp64")):
         out[key] = _batched_precision(pos, mat, fib, con, freq, M, steps, checkpoint, fused,
                                       math, resort and key == "fp32")
+    import torch
+    torch.cuda.empty_cache()   # the kept blocks (~200 GB) back to the device for later legs
     return out
 
 
