@@ -534,3 +534,87 @@ def test_diff_quality_two(require_gpu, dtype, math):
         say(f"d/d {name:11s} (x scale) AD={a: .6e} FD={fds} |AD-FD|/|FD|={rel:.1e}")
         assert rel <= bound, (name, rel)
     tank.close()
+
+
+def _clustered_case(M, N, steps, seed=3):
+    """M instances of N particles of random materials, each instance a patch of the
+    4-per-cell lattice (2 particles per cell side) at (0.43, 0.43) with small jitter, mildly
+    perturbed states (v, C ~ N(0, 0.05), F = I + N(0, 0.003)): particles share grid nodes,
+    so the few-particle dynamics stay well conditioned (scattered particles of water's
+    stiffness blow up within a dozen substeps, and a lone particle's position adjoint is a
+    sum of cancelling terms, rounding-dominated at any precision)."""
+    rng = np.random.default_rng(seed)
+    P = M * N
+    side = int(np.ceil(np.sqrt(N)))
+    k = np.tile(np.arange(N), M)
+    pos = 0.43 + np.stack([k % side, k // side], 1) / 256.0
+    pos += rng.uniform(-0.1, 0.1, (P, 2)) / 256.0
+    mat = rng.choice([0, 0, 1, 2, 3, 3], size=P).astype(np.int32)
+    ang = rng.uniform(0, 2 * np.pi, P)
+    fib = np.stack([np.cos(ang), np.sin(ang)], 1)
+    state = np.zeros((P, 12))
+    state[:, 0:2] = pos
+    state[:, 2:4] = rng.normal(size=(P, 2)) * 0.05
+    state[:, 4:8] = rng.normal(size=(P, 4)) * 0.05
+    state[:, 8:12] = (np.eye(2) + rng.normal(size=(P, 2, 2)) * 0.003).reshape(P, 4)
+    params = np.tile([500.0, 0.25, 1.1, 2.5, 700.0], (M, 1))
+    params[:, 0] *= rng.uniform(0.8, 1.2, M)
+    params[:, 4] *= rng.uniform(0.8, 1.2, M)
+    return Case.from_arrays(np.repeat(np.arange(M), N), mat, fib, state,
+                            rng.uniform(0.5, 1.5, P), params, steps=steps,
+                            t0=0.05).round_f32()
+
+
+@pytest.mark.parametrize("shape", [(40, 3), (5, 7), (2, 300)])
+@pytest.mark.parametrize("dtype,math", PRECISIONS)
+def test_diff_small_and_ragged_batches(require_gpu, dtype, math, shape):
+    """Batches whose workgroups hold few particles or several instances: 40 instances of 3
+    particles and 5 of 7 (one workgroup each), 2 of 300 (the second workgroup spans both),
+    clustered particles of every material (_clustered_case), 12 substeps. A workgroup
+    spanning instances takes the kernels' global-memory paths (no LDS tile; per-particle
+    parameter-adjoint atomics), so these check those paths: the forward's last state and the
+    whole adjoint (state, multiplier, parameters) against the fp64 host build, fp64 within
+    1e-9 and fp32 buffers within three times the fp32 host build's own error, + 1e-6
+    (relative norms; the GPU's global fp32 atomics add in a run-dependent order)."""
+    import torch
+    M, N = shape
+    say(f"diff small/ragged {dtype}/{math}: {M} x {N} particles, 12 substeps, vs fp64 harness")
+    cs = _clustered_case(M, N, 12)
+    L64, L32 = load_harness(), load_harness(fp32=True)
+    rng = np.random.default_rng(9)
+    wout = rng.normal(size=(cs.P, 12))
+    ref, f32 = {}, {}
+    for L, res in ((L64, ref), (L32, f32)):
+        traj = cs.forward(L)
+        gin, gamp, gpar = cs.backward(L, traj, wout)
+        for g, sl in GROUPS.items():
+            res["state " + g] = traj[-1][:, sl]
+            res["adjoint " + g] = gin[:, sl]
+        res["amp_scale"] = gamp
+        for i, n in ((0, "amp"), (1, "contraction"), (2, "freq"), (4, "E")):
+            res[n] = gpar[:, i]
+    tank, st, amp, par = _gpu(cs, dtype, math)
+    out = tank.decode_state(tank.forward_raw(st, amp, par, cs.t0, cs.steps).double())
+    out = out.cpu().numpy()
+    gin, gamp, gpar = tank.backward_raw(st, amp, par, cs.t0, cs.steps,
+                                        torch.tensor(wout, dtype=tank.dtype,
+                                                     device=tank.device))
+    gin, gamp, gpar = (t.cpu().numpy().astype(np.float64) for t in (gin, gamp, gpar))
+    got = {"amp_scale": gamp}
+    for g, sl in GROUPS.items():
+        got["state " + g] = out[:, sl]
+        got["adjoint " + g] = gin[:, sl]
+    for i, n in ((0, "amp"), (1, "contraction"), (2, "freq"), (4, "E")):
+        got[n] = gpar[:, i]
+    for name, b in ref.items():
+        nb = np.linalg.norm(b)
+        if nb == 0.0:     # no muscle (or no jelly) in this draw: the adjoint must be 0 too
+            assert np.linalg.norm(got[name]) == 0.0, name
+            continue
+        err = np.linalg.norm(got[name] - b) / nb
+        e32 = np.linalg.norm(f32[name] - b) / nb
+        tol = 3.0 * e32 + 1e-6 if dtype == "float32" else 1e-9
+        say(f"{name}: |gpu-f64|/|f64| = {err:.3e} (fp32 host build {e32:.3e})")
+        assert err <= tol, name
+    assert np.all(gpar[:, 3] == 0)
+    tank.close()
