@@ -452,3 +452,61 @@ def test_field_write_roundtrip(require_gpu):
         back = eng.read(name)
         live = orc.material >= 0
         np.testing.assert_array_equal(back[live], val[live])
+
+
+def scattered_case(seed=21, n_slots=4000, n=3000):
+    """n jelly and muscle particles scattered uniformly over 80% of the tank (~0.2 per cell;
+    water or payload this sparse is unstable at the reference's dt: its stiffness acts on
+    the mass of one or two particles per node)."""
+    rng = np.random.default_rng(seed)
+    pos = np.full((n_slots, 2), -1.0, np.float32)
+    mat = np.full(n_slots, -1, np.int32)
+    fib = np.zeros((n_slots, 2), np.float32)
+    fib[:, 1] = 1.0
+    pos[:n] = rng.uniform(0.1, 0.9, (n, 2))
+    mat[:n] = rng.choice([1, 1, 1, 3], n)
+    mus = np.flatnonzero(mat == 3)
+    ang = rng.uniform(0, 2 * np.pi, len(mus))
+    fib[mus] = np.stack([np.cos(ang), np.sin(ang)], 1)
+    return pos, mat, fib
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_scattered_instance_through_the_global_grid(require_gpu, path):
+    """An instance whose 3,000 particles are scattered over the whole tank, beside a dense
+    small tank: after the cell sort a 1024-particle chunk of the scattered instance still
+    spans about a third of the tank, a window larger than the LDS holds, so the fused kernel
+    gathers from and scatters to the global grid for it (the plain path always does). One
+    substep's grid (mass within 1e-5 of max against the fp32 oracle), then 100 substeps:
+    x, v, F, C against the fp32/fp64 oracles (fp32_like)."""
+    say(f"scattered instance beside a dense tank, {path}: 1 + 99 substeps vs oracle f32/f64")
+    cases = [scattered_case(), small_tank(22, 4000, region=(0.35, 0.55, 0.35, 0.5))]
+    eng, o32 = make_pair(cases, path=path)
+    o64 = oracle.OracleSim(2, 4000, dtype=np.float64)
+    for m, c in enumerate(cases):
+        o64.load(m, *c)
+    eng.substep(1)
+    o32.run(1)
+    o64.run(1)
+    gm = eng.read("grid_m")
+    assert relmax(gm, o32.grid_m) <= 1e-5, relmax(gm, o32.grid_m)
+    if path == "fused":
+        # the premise: some chunk's window (bounding box of its base cells + 2-node halo)
+        # exceeds the kernel's LDS window (1280 nodes for chunks of up to 1024 particles)
+        tun = eng.tuning()
+        chunk = tun["threads"] * tun["ppt"]
+        assert chunk <= 1024, chunk
+        _, x0 = eng.device_order(0)
+        b = oracle.base_cells(oracle.reference_consts(), x0)
+        boxes = [(np.ptp(b[i:i + chunk, 0]) + 3) * (np.ptp(b[i:i + chunk, 1]) + 3)
+                 for i in range(0, len(b), chunk)]
+        say(f"chunk of {chunk}: window nodes {boxes}")
+        assert max(boxes) > 1280, boxes
+    eng.substep(99)
+    o32.run(99, nthreads=8)
+    o64.run(99, nthreads=8)
+    live = o32.material >= 0
+    for n, a32, a64 in (("x", o32.x, o64.x), ("v", o32.v, o64.v), ("F", o32.F, o64.F),
+                        ("C", o32.C, o64.C)):
+        fp32_like(n, eng.read(n)[live], a32[live], a64[live])
+    eng.close()
