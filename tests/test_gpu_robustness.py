@@ -10,6 +10,8 @@
   every grid node receives one fp32 global add per substep and clean runs are bit-for-bit
   reproducible (larger instances flush several chunks per node in a run-dependent order).
 * The opt-in NaN->invalid flag of jm_run_batch_headless, against the oracle's.
+* Finite contributions beyond the fused kernel's fixed-point bound (particles at 2e5)
+  scatter through global atomics beside their chunk's LDS window, against the oracle.
 * The clock of a batch with no live particles still advances one fp32 dt per substep.
 * Limits: the largest batch jm_create accepts (2^28 slots) against the oracle; one whole
   150,000-substep fitness episode of the 32 x 80K generation batch, checked by what holds
@@ -53,6 +55,12 @@ def make_engine(cases, path="fused", nan_invalid=False):
     return eng
 
 
+def relmax(a, b):
+    """max |a - b| relative to max |b|."""
+    den = max(float(np.max(np.abs(b))), 1e-30)
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)))) / den
+
+
 def nan_slots(cases, m, picks=(5, 300, 700)):
     live = np.flatnonzero(cases[m][1] >= 0)
     return live[list(picks)]
@@ -94,6 +102,59 @@ def test_nan_propagation_matches_oracle(require_gpu, path):
         # the grid the last G2P read: NaN at the same nodes as the oracle's grid
         gv = eng.read("grid_v")
         np.testing.assert_array_equal(np.isnan(gv).any(-1), np.isnan(orc.grid_v).any(-1))
+    eng.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_contributions_beyond_the_fixed_point_bound(require_gpu, path):
+    """Three particles of one instance moving at 2e5 (momentum m v ~ 3 against the fused
+    kernel's fixed-point bound of 0.75, so they scatter through global fp32 atomics while the
+    rest of their chunk uses the LDS window), one substep from t = 0.05: the grid's mass
+    within 1e-5 of max of the fp32 oracle's; grid momentum and the particles' v within twice
+    the fp32 oracle's error against fp64 (relative to max, + 1e-6); the three particles'
+    x (on the walls or near them) within twice the fp32 oracle's error against fp64, + 1e-6;
+    x of every other particle within 1e-5."""
+    say(f"contributions beyond the fixed-point bound, path={path}")
+    cases = one_chunk_cases()
+    eng = make_engine(cases, path)
+    o32 = oracle.OracleSim(len(cases), N_SLOTS)
+    o64 = oracle.OracleSim(len(cases), N_SLOTS, dtype=np.float64)
+    for m, (p, mt, f) in enumerate(cases):
+        o32.load(m, p, mt, f)
+        o64.load(m, p, mt, f)
+    fast = nan_slots(cases, 2)
+    v = np.zeros((len(cases), N_SLOTS, 2), np.float32)
+    v[2, fast] = np.float32([[2e5, -1e5], [-2e5, 5e4], [1e5, 2e5]])
+    eng.write("v", v)
+    eng.set_time(0.05)
+    for o in (o32, o64):
+        o.v[...] = v
+        o.t[0] = np.float32(0.05)
+    eng.substep(1)
+    o32.run(1)
+    o64.run(1)
+    gm, gv = eng.read("grid_m"), eng.read("grid_v")
+    assert relmax(gm, o32.grid_m) <= 1e-5, relmax(gm, o32.grid_m)
+    mom = gv.astype(np.float64) * gm[..., None]
+    mom64 = o64.grid_v * o64.grid_m[..., None]
+    mom32 = o32.grid_v.astype(np.float64) * o32.grid_m[..., None]
+    e_gpu, e_32 = relmax(mom, mom64), relmax(mom32, mom64)
+    say(f"grid momentum: |gpu-f64| {e_gpu:.3e}, |oracle32-f64| {e_32:.3e} (relative to max)")
+    assert e_gpu <= 2.0 * e_32 + 1e-6
+    live = o64.material >= 0
+    vg = eng.read("v")
+    e_gpu, e_32 = relmax(vg[live], o64.v[live]), relmax(o32.v[live], o64.v[live])
+    say(f"v: |gpu-f64| {e_gpu:.3e}, |oracle32-f64| {e_32:.3e} (relative to max)")
+    assert e_gpu <= 2.0 * e_32 + 1e-6
+    x = eng.read("x")
+    e_gpu = float(np.max(np.abs(x[2, fast] - o64.x[2, fast])))
+    e_32 = float(np.max(np.abs(o32.x[2, fast] - o64.x[2, fast])))
+    say(f"x of the fast particles {x[2, fast].tolist()}: |gpu-f64| {e_gpu:.3e}, "
+        f"|oracle32-f64| {e_32:.3e}")
+    assert e_gpu <= 2.0 * e_32 + 1e-6
+    others = live.copy()
+    others[2, fast] = False
+    assert float(np.max(np.abs(x[others] - o64.x[others]))) < 1e-5
     eng.close()
 
 
