@@ -510,3 +510,53 @@ def test_scattered_instance_through_the_global_grid(require_gpu, path):
                         ("C", o32.C, o64.C)):
         fp32_like(n, eng.read(n)[live], a32[live], a64[live])
     eng.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_per_instance_parameters(require_gpu, path):
+    """Per-instance inputs of evolve.py and the facade (mpm_sim.py:116-125,142-149,197-201;
+    E_jelly, a module constant in the reference, taken per instance at run time): four small
+    tanks with actuation 500 / 0 / 1000 / 250, payload density 2.5 / 1 / 5 / 2.5 and E_jelly
+    700 / 300 / 1400 / 700 (the payload of density 5 is heavier than the fixed-point mass
+    bound, so it scatters through global atomics), 300 substeps from t = 0.05 (a contraction
+    phase): x, v, F, C against the fp32/fp64 oracles with the same parameters (fp32_like)."""
+    say(f"per-instance actuation, payload density, E_jelly, {path}: 300 substeps vs oracle")
+    regions = [(0.30, 0.50, 0.30, 0.45), (0.55, 0.75, 0.10, 0.25), (0.10, 0.30, 0.55, 0.70),
+               (0.60, 0.80, 0.60, 0.75)]
+    cases = [small_tank(30 + m, 4000, region=r) for m, r in enumerate(regions)]
+    amp = np.float32([500.0, 0.0, 1000.0, 250.0])
+    pdens = np.float32([2.5, 1.0, 5.0, 2.5])
+    e_jelly = np.float32([700.0, 300.0, 1400.0, 700.0])
+    eng, o32 = make_pair(cases, path=path)
+    o64 = oracle.OracleSim(4, 4000, dtype=np.float64)
+    for m, c in enumerate(cases):
+        o64.load(m, *c)
+    eng.set_param("actuation", amp)
+    eng.set_param("payload_density", pdens)
+    eng.set_param("e_jelly", e_jelly)
+    eng.set_time(0.05)
+    for o in (o32, o64):
+        o.amp[:] = amp
+        o.pdens[:] = pdens
+        o.set_e_jelly(e_jelly)
+        o.t[0] = np.float32(0.05)
+    np.testing.assert_array_equal(eng.get_param("payload_density"), pdens)
+    np.testing.assert_array_equal(eng.get_param("e_jelly"), e_jelly)
+    eng.substep(300)
+    o32.run(300, nthreads=4)
+    o64.run(300, nthreads=4)
+    live = o32.material >= 0
+    for n, a32, a64 in (("x", o32.x, o64.x), ("v", o32.v, o64.v), ("F", o32.F, o64.F),
+                        ("C", o32.C, o64.C)):
+        fp32_like(n, eng.read(n)[live], a32[live], a64[live])
+    # the parameters move the particles: instance 0 (the defaults) equals a default run,
+    # each other instance differs from it
+    d = oracle.OracleSim(4, 4000, dtype=np.float64)
+    for m, c in enumerate(cases):
+        d.load(m, *c)
+    d.t[0] = np.float32(0.05)
+    d.run(300, nthreads=4)
+    moved = [float(np.abs(o64.x[m][live[m]] - d.x[m][live[m]]).max()) for m in range(4)]
+    say(f"max |x - x(default parameters)| per instance: {moved}")
+    assert moved[0] == 0.0 and all(v > 1e-6 for v in moved[1:]), moved
+    eng.close()
