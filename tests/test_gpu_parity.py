@@ -560,3 +560,33 @@ def test_per_instance_parameters(require_gpu, path):
     say(f"max |x - x(default parameters)| per instance: {moved}")
     assert moved[0] == 0.0 and all(v > 1e-6 for v in moved[1:]), moved
     eng.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_headless_without_payload(require_gpu, path):
+    """run_batch_headless's statistics when an instance has no payload (mpm_sim.py:764-800,
+    826-834): one small tank with a payload, one whose body has none (its CoM falls back to
+    the body's: jelly and muscle), one of water only (no body: count 0, invalid); 200
+    substeps (the run resets the clock to 0, mpm_sim.py:808). Validity flags equal to the
+    oracle's, CoM within 1e-5, and the
+    payload and body statistics of the final state equal the oracle's sums within fp32
+    rounding."""
+    say(f"headless without payload, {path}: 3 tanks, 200 substeps vs oracle")
+    with_payload = small_tank(40, 4000, region=(0.30, 0.50, 0.30, 0.45))
+    no_payload = small_tank(41, 4000, region=(0.55, 0.75, 0.10, 0.25))
+    no_payload[1][no_payload[1] == 2] = 1           # payload block becomes jelly
+    water = small_tank(42, 4000, region=(0.10, 0.30, 0.55, 0.70), robot=False)
+    cases = [with_payload, no_payload, water]
+    eng, orc = make_pair(cases, path=path)
+    r_gpu = eng.run_batch_headless(200)
+    r_ref = orc.run_batch_headless(200, nthreads=3)
+    say(f"gpu {r_gpu.tolist()}\noracle {r_ref.tolist()}")
+    np.testing.assert_array_equal(r_gpu[:, 4], r_ref[:, 4])
+    np.testing.assert_array_equal(r_gpu[:, 4], [1.0, 1.0, 0.0])
+    assert np.max(np.abs(r_gpu[:, :4] - r_ref[:, :4])) <= 1e-5
+    for mode in ("payload", "body"):
+        g = eng.raw_stats(mode).astype(np.float64)
+        o = np.asarray(orc.raw_stats(mode == "body"), np.float64)
+        np.testing.assert_array_equal(g[:, 1], o[:, 1])          # counts
+        assert np.max(np.abs(g[:, [0, 2]] - o[:, [0, 2]])) <= 1e-5 * max(o[:, 1].max(), 1.0)
+    eng.close()
