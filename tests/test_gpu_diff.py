@@ -307,27 +307,32 @@ def test_gradients_with_resorting_between_blocks(require_gpu):
     """Re-sorting the particles by cell before every checkpoint block (a gather of the state
     and multiplier; autograd scatters the adjoints back) gives the objective and gradients
     of the unsorted run, in the caller's particle order, up to the run-to-run noise of
-    atomics order (measured by repeating the unsorted run). fp64, 300 substeps, blocks of
-    100."""
-    say("reference tank seed0, 300 substeps, fp64: re-sorted every block vs not")
+    atomics order (measured by repeating the unsorted run), also with the last 2 blocks
+    keeping their trajectories from the forward pass (kept on the re-ordered views). fp64,
+    300 substeps, blocks of 100."""
+    say("reference tank seed0, 300 substeps, fp64: re-sorted every block (recomputed, or the "
+        "last 2 blocks kept) vs not")
     cs, _, _, _ = reference_tank_case(300)
     base = _gpu_gradients(cs, checkpoint=100, dtype="float64", keep_last=0)
     again = _gpu_gradients(cs, checkpoint=100, dtype="float64", keep_last=0)
     rs = _gpu_gradients(cs, checkpoint=100, dtype="float64", resort_every=1,
                         keep_last=0)
+    rs_kept = _gpu_gradients(cs, checkpoint=100, dtype="float64", resort_every=1,
+                             keep_last=2)
     body = cs.mat > 0
 
     def groups(r):
         return {"body v": r[1][body, 2:4], "amp_scale": r[2], "params": r[3]}
 
-    assert abs(rs[0] - base[0]) <= 1e-9 * abs(base[0]) + 1e-15, (rs[0], base[0])
     b = groups(base)
     floor = {n: np.linalg.norm(v - b[n]) / np.linalg.norm(b[n]) for n, v in groups(again).items()}
-    for n, v in groups(rs).items():
-        err = np.linalg.norm(v - b[n]) / np.linalg.norm(b[n])
-        say(f"re-sorted vs not, {n}: {err:.2e} (repeat: {floor[n]:.2e})")
-        if n != "body v":
-            assert err <= max(10 * floor[n], 1e-9) and err <= 1e-6, (n, err)
+    for label, r in (("re-sorted", rs), ("re-sorted, last 2 kept", rs_kept)):
+        assert abs(r[0] - base[0]) <= 1e-9 * abs(base[0]) + 1e-15, (label, r[0], base[0])
+        for n, v in groups(r).items():
+            err = np.linalg.norm(v - b[n]) / np.linalg.norm(b[n])
+            say(f"{label} vs not, {n}: {err:.2e} (repeat: {floor[n]:.2e})")
+            if n != "body v":
+                assert err <= max(10 * floor[n], 1e-9) and err <= 1e-6, (label, n, err)
 
 
 def test_batched_gradients_match_per_instance(require_gpu):
