@@ -383,8 +383,10 @@ def test_glow_pass(require_gpu, path):
 
 
 def test_variants_agree(require_gpu):
-    """Graphs on/off, lazy v/C on/off, XCD affinity, 1-3 concurrent instance groups and
-    several launch shapes."""
+    """Graphs on/off, lazy v/C on/off, XCD affinity, 1-4 concurrent instance groups,
+    several launch shapes, and block lengths that do not divide the 60 substeps (37: a block
+    of 37 then one of 23, with sorts every 13 substeps, off the block boundaries) or graph
+    one substep at a time."""
     say("fused variants")
     cases = ragged_cases(12000)[:2] * 5          # 10 instances (affinity needs >= 8)
     base = None
@@ -392,7 +394,8 @@ def test_variants_agree(require_gpu):
                 dict(threads=256, ppt=4), dict(threads=512, ppt=8), dict(threads=128, ppt=16),
                 dict(threads=256, ppt=2), dict(threads=512, ppt=2),
                 dict(sort_interval=7, block_substeps=5), dict(instance_groups=1),
-                dict(instance_groups=3, use_graphs=0), dict(instance_groups=4)]
+                dict(instance_groups=3, use_graphs=0), dict(instance_groups=4),
+                dict(block_substeps=37, sort_interval=13), dict(block_substeps=1)]
     for var in variants:
         say(f"variant {var}")
         eng, orc = make_pair(cases, **var)
