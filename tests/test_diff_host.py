@@ -1,5 +1,6 @@
 """Host-side logic of the differentiable path (jellyfih_amd/diff.py) that runs without a
-GPU: the per-instance reductions and the fp32 clock."""
+GPU: the per-instance reductions, the fp32 clock, the packing orders and the sizing of the
+blocks a run keeps from its forward pass."""
 from __future__ import annotations
 
 import numpy as np
@@ -90,3 +91,41 @@ def test_packing_order_split_puts_the_water_last():
             live = np.flatnonzero(sel(mat[m]))
             expect = live[oracle.stable_cell_sort(c, pos[m, live], 8)]
             np.testing.assert_array_equal(slot[part][inst[part] == m], expect)
+
+
+class _Sized:
+    """The fields block_bytes and blocks_to_keep read."""
+
+    def __init__(self, P, M, G, dtype=torch.float32, device="cuda"):
+        self.P, self.M, self.G, self.dtype = P, M, G, dtype
+        self.device = torch.device(device)
+
+    block_bytes = DiffTank.block_bytes
+
+
+def test_blocks_to_keep_arithmetic(monkeypatch):
+    """The number of kept blocks: as many blocks of (checkpoint + 1) P 12 + checkpoint M 3 G
+    scalars as fit in keep_memory times the free memory (plus the caching allocator's unused
+    reserve) less 2 GiB, at most all blocks of the run; none for a zero fraction, an empty
+    run or a tank off the GPU. The 32-tank tall batch at checkpoint 200 on a 288 GB device
+    keeps 5 of its 20 blocks (41.4 GB each)."""
+    GiB = 1 << 30
+    tall = _Sized(P=4034949, M=32, G=128 * 256)
+    assert tall.block_bytes(200) == (201 * 4034949 * 12 + 200 * 32 * 3 * 128 * 256) * 4
+    assert _Sized(10, 1, 16, dtype=torch.float64).block_bytes(3) == (4 * 10 * 12 + 3 * 48) * 8
+    state = {"free": 285 * 10**9, "reserved": 0, "allocated": 0}
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda dev=None: (state["free"], 288 * 10**9))
+    monkeypatch.setattr(torch.cuda, "memory_reserved", lambda dev=None: state["reserved"])
+    monkeypatch.setattr(torch.cuda, "memory_allocated", lambda dev=None: state["allocated"])
+    keep = lambda obj, n, c, f: DiffTank.blocks_to_keep(obj, n, c, f)
+    b = tall.block_bytes(200)
+    assert keep(tall, 4000, 200, 0.8) == int((0.8 * 285e9 - 2 * GiB) // b) == 5
+    assert keep(tall, 4000, 200, 0.0) == 0
+    assert keep(tall, 0, 200, 0.8) == 0
+    assert keep(tall, 400, 200, 0.8) == 2           # at most the run's 2 blocks
+    assert keep(tall, 450, 200, 0.8) == 3           # a short last block counts as a block
+    state.update(free=100 * 10**9, reserved=150 * 10**9, allocated=20 * 10**9)
+    assert keep(tall, 4000, 200, 0.8) == int((0.8 * 230e9 - 2 * GiB) // b) == 4
+    state.update(free=GiB, reserved=0, allocated=0)
+    assert keep(tall, 4000, 200, 0.8) == 0          # less than 2 GiB of room: none
+    assert keep(_Sized(4034949, 32, 128 * 256, device="cpu"), 4000, 200, 0.8) == 0
